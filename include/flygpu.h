@@ -1,0 +1,242 @@
+/*
+ * flygpu.h -- C-ABI of libflygpu, the MI355X (gfx950) implementation of openFLY's saddle-search hot path:
+ * periodic cell-list neighbour lists, EAM/Finnis-Sinclair energy, gradient and Hessian-vector products,
+ * and the L-BFGS / dimer steps that drive them.
+ *
+ * Conventions
+ *  - Every function returns a status code (FLYG_OK on success).  The message of the last failure on a
+ *    context is returned by flyg_ctx_last_error().  No C++ exception crosses this boundary.
+ *  - Per-atom vectors are interleaved xyz, 3*n doubles, exactly the layout of the reference's SoA
+ *    properties Position / Delta / PotentialGradient / Axis (property.hpp:53-85, adaptor.hpp:46-109).
+ *  - Arguments tagged `where` are host pointers when where == FLYG_HOST and device pointers (on the
+ *    context's device, ordered on the context's stream) when where == FLYG_DEVICE.
+ *  - The box basis is a 3x3 row-major matrix whose COLUMNS are the basis vectors (the reference's
+ *    Box(Mat basis, Arr<bool> periodic), box.hpp:103-120).
+ *  - Handles are single-owner and not thread-safe, like the reference objects they replace; use one
+ *    context per host thread (the reference's per-thread ThreadData, find.hpp:310-317).
+ */
+#ifndef FLYGPU_H
+#define FLYGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define FLYG_OK 0
+#define FLYG_INVALID_ARG 1  /* reference: fly::RuntimeError thrown by verify()/error() (core.hpp:89-138) */
+#define FLYG_SPLINE_OOB 2   /* reference: fly::Spline::OutOfBounds (spline.hpp:46-51, thrown at :141-143) */
+#define FLYG_DEVICE_ERROR 3 /* HIP runtime failure */
+
+/* Pointer location for `where` arguments. */
+#define FLYG_HOST 0
+#define FLYG_DEVICE 1
+
+/* Pair arithmetic precision of a potential handle. */
+#define FLYG_FP64 0  /* plain form: everything in FP64 */
+#define FLYG_MIXED 1 /* FP32 per-pair spline/force arithmetic, FP64 accumulation and FP64 energies */
+
+/* Dimer exit codes (reference Dimer::Exit, dimer.hpp:123-129). */
+#define FLYG_EXIT_SUCCESS 0
+#define FLYG_EXIT_CONVEX 1
+#define FLYG_EXIT_ITER_MAX 2
+#define FLYG_EXIT_COLLISION 3
+#define FLYG_EXIT_UNINIT 4
+
+typedef struct flyg_ctx_s* flyg_ctx;
+typedef struct flyg_eam_s* flyg_eam;
+typedef struct flyg_list_s* flyg_list;
+typedef struct flyg_steplbfgs_s* flyg_steplbfgs;
+typedef struct flyg_lbfgs_s* flyg_lbfgs;
+typedef struct flyg_dimer_s* flyg_dimer;
+typedef struct flyg_batch_s* flyg_batch;
+
+/* ---------------------------------------------------------------------------------------------------
+ * Context: a device, one HIP stream (default priority) and the last error message.
+ * No reference counterpart; stands in for the per-thread state of Master (find.hpp:310-317).
+ * ------------------------------------------------------------------------------------------------- */
+int flyg_ctx_create(int device, flyg_ctx* out);
+int flyg_ctx_destroy(flyg_ctx ctx);
+const char* flyg_ctx_last_error(flyg_ctx ctx);
+int flyg_ctx_synchronize(flyg_ctx ctx);
+/* The hipStream_t of the context, for callers that time or order work around it. */
+void* flyg_ctx_stream(flyg_ctx ctx);
+/* Device name and compute-unit count. */
+int flyg_ctx_device_info(flyg_ctx ctx, char* name, size_t len, int* compute_units);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Potential tables: replaces potential::DataEAM(Options, std::ifstream) (data.hpp:54, data.cpp:67-153)
+ * and the EAM(TypeMap, shared_ptr<DataEAM>) ctor (eam.hpp:50-63).  Parses a setfl/eam.fs file on the
+ * host (3 comment lines, species line, "Nrho drho Nr dr rcut", per-species "Z mass" + F + rho tables,
+ * then r*V pair tables for j <= i, five values per line), builds the natural cubic splines and
+ * uploads FP64 and FP32 coefficient tables.
+ * ------------------------------------------------------------------------------------------------- */
+int flyg_eam_load(flyg_ctx ctx, const char* path, flyg_eam* out);
+int flyg_eam_destroy(flyg_eam eam);
+/* potential::EAM::r_cut() (eam.hpp:70) / DataEAM::type_map(): names is ntypes*16 chars (NUL padded). */
+int flyg_eam_info(flyg_eam eam, int* ntypes, double* r_cut, double* masses, char* names);
+/* FLYG_FP64 (default) or FLYG_MIXED for gradient / HVP pair arithmetic.  Energies are always FP64. */
+int flyg_eam_set_precision(flyg_eam eam, int precision);
+/* Evaluates one table on the device at m host points: kind 0 = F_a, 1 = phi(a,b), 2 = V(a,b);
+ * deriv 0/1/2 = f/f'/f'' (DataEAM::f/phi/v + Spline::f/fp/fpp, data.hpp:73-116, spline.hpp:81-123). */
+int flyg_eam_table_eval(flyg_eam eam, int kind, int a, int b, const double* x, int64_t m, int deriv, double* out);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Neighbour list: replaces neigh::List (list.hpp:71-210, list.cpp:37-165).
+ * ------------------------------------------------------------------------------------------------- */
+/* List(Box const&, double r_cut) (list.hpp:86-91). */
+int flyg_list_create(flyg_ctx ctx, const double basis[9], const int periodic[3], double r_cut, flyg_list* out);
+int flyg_list_destroy(flyg_list nl);
+/* List::rebuild(SoA<Position const&>, int threads) (list.hpp:99, list.cpp:53-95). */
+int flyg_list_rebuild(flyg_list nl, const double* pos, int64_t n, int where);
+/* List::update(SoA<Delta const&>) (list.hpp:119, list.cpp:37-51): r <- r - delta for every atom. */
+int flyg_list_update(flyg_list nl, const double* delta, int64_t n, int where);
+/* List::size() (list.hpp:161). */
+int flyg_list_size(flyg_list nl, int64_t* n);
+/* Number of ghost images made by the last rebuild (List::m_num_plus_ghosts - size(), list.hpp:178). */
+int flyg_list_num_ghosts(flyg_list nl, int64_t* n_ghosts);
+/* Pairs with r < r_cut (List::for_neighbours, list.hpp:138-154): count, then export as CSR rows with
+ * the real neighbour index, the image shift code (s0+1)+3(s1+1)+9(s2+1) and dr = r_n - r_i. */
+int flyg_list_count(flyg_list nl, double r_cut, int64_t* n_pairs);
+int flyg_list_export(flyg_list nl, double r_cut, int64_t* offsets, int32_t* j, int32_t* shift, double* dr);
+/* Current internal positions (canonical images at the last rebuild, minus later updates). */
+int flyg_list_positions(flyg_list nl, double* pos, int where);
+
+/* ---------------------------------------------------------------------------------------------------
+ * EAM evaluation: replaces potential::EAM / potential::Generic energy, gradient, hessian
+ * (generic.hpp:79-136, eam.cpp:33-233).  `types` uint32 and `frozen` uint8 per atom.
+ * ------------------------------------------------------------------------------------------------- */
+/* EAM::energy (eam.cpp:33-51): sum over all atoms, frozen included. */
+int flyg_eam_energy(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
+                    double* energy);
+/* EAM::gradient (eam.cpp:53-120): two passes (density then force); frozen atoms get exactly 0. */
+int flyg_eam_gradient(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
+                      double* grad);
+/* H v with H the analytic Hessian of EAM::hessian (eam.cpp:122-233) restricted to non-frozen atoms,
+ * computed matrix-free in two passes (no 3N x 3N matrix). */
+int flyg_eam_hvp(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
+                 const double* v, double* hv, int where);
+
+/* ---------------------------------------------------------------------------------------------------
+ * StepLBFGS: replaces minimise::StepLBFGS (lbfgs_core.hpp:37-119, lbfgs_core.cpp:29-87).
+ * ------------------------------------------------------------------------------------------------- */
+int flyg_steplbfgs_create(flyg_ctx ctx, int m, flyg_steplbfgs* out);
+int flyg_steplbfgs_destroy(flyg_steplbfgs s);
+int flyg_steplbfgs_clear(flyg_steplbfgs s);
+/* newton_step(x, g): returns q ~ H^-1 g (the caller moves x <- x - q). x, g, q hold len doubles. */
+int flyg_steplbfgs_newton_step(flyg_steplbfgs s, const double* x, const double* g, int64_t len, double* q, int where);
+
+/* ---------------------------------------------------------------------------------------------------
+ * LBFGS minimiser: replaces minimise::LBFGS (lbfgs.hpp:53-118, lbfgs.cpp:29-115).
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct {
+  int n;               /* history length (10) */
+  int iter_max;        /* (2000) */
+  double f2norm;       /* (1e-6) */
+  double skin_frac;    /* (1.1) */
+  double proj_tol;     /* (0) */
+  double max_trust;    /* (0.5) */
+  double min_trust;    /* (0.01) */
+  double grow_trust;   /* (1.5) */
+  double shrink_trust; /* (0.5) */
+} flyg_lbfgs_options;
+
+void flyg_lbfgs_default_options(flyg_lbfgs_options* opt);
+/* LBFGS(Options, Box) */
+int flyg_lbfgs_create(flyg_ctx ctx, const flyg_lbfgs_options* opt, const double basis[9], const int periodic[3],
+                      flyg_lbfgs* out);
+int flyg_lbfgs_destroy(flyg_lbfgs m);
+/* LBFGS::minimise(out, in, pot, threads) -> bool: *failed = 1 when iter_max was reached (the
+ * reference returns true on failure, lbfgs.cpp:114).  pos is in/out, grad out. */
+int flyg_lbfgs_minimise(flyg_lbfgs m, flyg_eam eam, double* pos, double* grad, const uint32_t* types,
+                        const uint8_t* frozen, int64_t n, int where, int* failed, int* iterations);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Dimer saddle search: replaces saddle::Rotor + saddle::Dimer (rotor.hpp:42-139, dimer.hpp:51-170,
+ * rotor.cpp:49-164, dimer.cpp:28-171).
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct {
+  int n;               /* L-BFGS history (10) */
+  int max_steps;       /* (1000) */
+  int hist_check_freq; /* (5) */
+  double f2norm;       /* (1e-5) */
+  double skin_frac;    /* (1.1) */
+  double proj_tol;     /* (0) */
+  double max_trust;    /* (0.5) */
+  double min_trust;    /* (0.05) */
+  double grow_trust;   /* (1.5) */
+  double shrink_trust; /* (0.5) */
+  int convex_max;      /* (3) */
+  int use_history;     /* (1) */
+} flyg_dimer_options;
+
+typedef struct {
+  int n;               /* rotation L-BFGS history (6) */
+  int iter_max_rot;    /* (20) */
+  double delta_r;      /* dimer half-length (1e-3) */
+  double theta_tol;    /* (pi/360) */
+  int relax_in_convex; /* (1) */
+} flyg_rotor_options;
+
+typedef struct {
+  int exit_code;   /* FLYG_EXIT_* */
+  int steps;       /* translation steps taken */
+  int64_t grads;   /* gradient evaluations */
+  int rebuilds;    /* neighbour-list rebuilds */
+  double curv;     /* last curvature along the axis */
+  double energy;   /* FP64 energy at the returned positions (0 when not requested) */
+} flyg_search_stats;
+
+void flyg_dimer_default_options(flyg_dimer_options* opt);
+void flyg_rotor_default_options(flyg_rotor_options* opt);
+/* Dimer(Options, Rotor::Options, Box) */
+int flyg_dimer_create(flyg_ctx ctx, const flyg_dimer_options* opt, const flyg_rotor_options* ropt,
+                      const double basis[9], const int periodic[3], flyg_dimer* out);
+int flyg_dimer_destroy(flyg_dimer d);
+/* Dimer::find_sp(out, in, in_min, pot, hist_sp, theta_tol, count_frozen, threads) -> Exit.
+ * pos/axis are in/out (3n each); in_min 3n; hist_sp n_hist * 3n (may be NULL when n_hist == 0). */
+int flyg_dimer_find_sp(flyg_dimer d, flyg_eam eam, double* pos, double* axis, const uint32_t* types,
+                       const uint8_t* frozen, int64_t n, const double* in_min, const double* hist_sp, int n_hist,
+                       double theta_tol, int count_frozen, int where, flyg_search_stats* stats);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Batched saddle searches: the device-side counterpart of Master::find_batch's task-per-search loop
+ * (find.cpp:684-755, one OpenMP task per search, each running Master::find_sp -> Dimer::find_sp with
+ * threads = 1, find.cpp:983-1009).  One persistent kernel advances many searches at once, one
+ * workgroup per search, each on its active region (mobile atoms + frozen shells) held on the device.
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct {
+  int precision;        /* FLYG_MIXED (default) or FLYG_FP64 */
+  int block_threads;    /* workgroup size, 0 = autotuned */
+  int lanes_per_atom;   /* lanes cooperating on one atom's neighbours, 0 = autotuned */
+  double list_skin;     /* Verlet skin of the per-search local lists (A), 0 = default */
+  int max_concurrent;   /* resident searches per launch, 0 = 2 per compute unit */
+} flyg_batch_options;
+
+void flyg_batch_default_options(flyg_batch_options* opt);
+/* One base system (the relaxed minimum min0), resident on the device for the handle's lifetime. */
+int flyg_batch_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt, const flyg_rotor_options* ropt,
+                      const flyg_batch_options* bopt, const double basis[9], const int periodic[3],
+                      const double* min_pos, const uint32_t* types, int64_t n, flyg_batch* out);
+int flyg_batch_destroy(flyg_batch b);
+/* Queue searches.  For search k: every atom farther than radius[k] from atom centre[k] (minimum
+ * image, in min_pos) is frozen; start_pos / start_axis hold the full 3n start state of that search
+ * (a perturbed copy of min_pos and its dimer axis, as Master::perturb writes them). */
+int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const double* radius, const double* start_pos,
+                      const double* start_axis, int64_t* ids);
+/* Run every queued search to completion on the device. */
+int flyg_batch_run(flyg_batch b);
+/* Results of search `id`: exit code / stats (energy = FP64 E(sp) - E(min0), the barrier), and the full
+ * 3n saddle positions and axis (either may be NULL). */
+int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double* pos, double* axis);
+/* Mobile / region atom counts of search `id`. */
+int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLYGPU_H */

@@ -1,0 +1,216 @@
+// Shared host/device definitions of libflygpu (MI355X / gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/flygpu.h"
+
+namespace flyg {
+
+  // ----------------------------------------------------------------------------------------------
+  // Errors.  Host code throws Error(status, message); the C-ABI layer turns it into a status code
+  // plus a per-context message.  No exception crosses the C-ABI.
+  // ----------------------------------------------------------------------------------------------
+
+  struct Error : std::runtime_error {
+    int status;
+    Error(int s, std::string const& m) : std::runtime_error(m), status(s) {}
+  };
+
+  inline void hip_check(hipError_t e, char const* what) {
+    if (e != hipSuccess) throw Error(FLYG_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+  }
+#define HIPCHK(x) ::flyg::hip_check((x), #x)
+
+  inline void require(bool ok, std::string const& msg) {
+    if (!ok) throw Error(FLYG_INVALID_ARG, msg);
+  }
+
+  // Device-side error flags (bit set, read back by the host after a call).
+  enum : unsigned { ERR_SPLINE_OOB = 1u, ERR_LIST_OVERFLOW = 2u, ERR_REGION = 4u, ERR_NAN = 8u };
+
+  // ----------------------------------------------------------------------------------------------
+  // Growable device buffer (hipMalloc'd, never shrinks).
+  // ----------------------------------------------------------------------------------------------
+
+  template <typename T>
+  struct DBuf {
+    T* p = nullptr;
+    std::size_t cap = 0;
+    DBuf() = default;
+    DBuf(DBuf const&) = delete;
+    DBuf& operator=(DBuf const&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) {
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept {
+      std::swap(p, o.p);
+      std::swap(cap, o.cap);
+      return *this;
+    }
+    ~DBuf() {
+      if (p) (void)hipFree(p);
+    }
+    void reserve(std::size_t n) {
+      if (n <= cap) return;
+      if (p) HIPCHK(hipFree(p));
+      p = nullptr;
+      std::size_t c = n < 64 ? 64 : n;
+      HIPCHK(hipMalloc(&p, c * sizeof(T)));
+      cap = c;
+    }
+    T* get() const { return p; }
+  };
+
+  // ----------------------------------------------------------------------------------------------
+  // Periodic box and cell grid (host + device POD).  The grid follows the reference's HyperGrid:
+  // shape = 2 + floor(E / r_cut) per axis (one ghost layer each side), cell = E / floor(E / r_cut),
+  // x-index fastest; TriGrid uses the column sums of the basis as E.
+  // ----------------------------------------------------------------------------------------------
+
+  struct BoxDesc {
+    double B[3][3];      // B[row][col]: column c is basis vector c
+    double ext[3], inv_ext[3];
+    double Binv[3][3];
+    double hyper[3][3];  // column c = unit normal of the plane spanned by the other two basis vectors
+    int per[3];
+    int ortho;
+  };
+
+  struct GridDesc {
+    double rc;
+    double cell[3], inv_cell[3];
+    double gext[3];
+    int shape[3], prod[3];
+    int ncells;
+  };
+
+  BoxDesc make_box(double const* basis_rowmajor, int const* periodic);
+  GridDesc make_grid(BoxDesc const& box, double r_cut);
+  double box_min_width(BoxDesc const& b);
+
+  // ----------------------------------------------------------------------------------------------
+  // Spline tables.  One natural cubic spline = `count` segments of (a,b,c,d) on a uniform grid.
+  // FP64 coefficients for the plain path; an FP32 copy for the mixed-precision path.
+  // ----------------------------------------------------------------------------------------------
+
+  struct SplineDesc {
+    int off;        // first segment in the packed table
+    int count;      // number of segments
+    double dx, inv_dx;
+    float dxf, inv_dxf;
+  };
+
+  // Potential tables as the kernels see them.  Index rules follow DataEAM: F[a]; phi(a, b) at a + n*b
+  // (density at an atom of type b due to a neighbour of type a); V(a, b) symmetric at max + n*min.
+  constexpr int MAX_TYPES = 4;
+
+  struct EAMTables {
+    int ntypes;
+    double rc, rc2;
+    float rcf;
+    SplineDesc F[MAX_TYPES];
+    SplineDesc phi[MAX_TYPES * MAX_TYPES];
+    SplineDesc V[MAX_TYPES * MAX_TYPES];
+    double4 const* c64;   // packed FP64 segments
+    float4 const* c32;    // packed FP32 segments
+  };
+
+  // Packed neighbour entry: bits 0..25 real atom index, bits 26..30 image shift code
+  // (s0+1) + 3 (s1+1) + 9 (s2+1), 13 = no shift.
+  constexpr int NB_SHIFT_BITS = 26;
+  constexpr unsigned NB_IDX_MASK = (1u << NB_SHIFT_BITS) - 1u;
+  constexpr long MAX_ATOMS = 1l << NB_SHIFT_BITS;
+
+  __host__ __device__ inline int shift_code(int s0, int s1, int s2) { return (s0 + 1) + 3 * (s1 + 1) + 9 * (s2 + 1); }
+
+  // ----------------------------------------------------------------------------------------------
+  // Device spline evaluation (reference Spline::fetch/f/fp/fpp, spline.hpp:81-154): segment
+  // i = trunc(x / dx) via the stored reciprocal, t = x - i dx; out of range sets ERR_SPLINE_OOB.
+  // ----------------------------------------------------------------------------------------------
+
+  __device__ inline bool seg64(SplineDesc const& s, double x, int& i, double& t) {
+    double q = x * s.inv_dx;
+    if (!(q < static_cast<double>(s.count)) || q <= -1.0) return false;
+    i = static_cast<int>(q);
+    t = x - static_cast<double>(i) * s.dx;
+    return true;
+  }
+
+  template <int D>   // 0 = f, 1 = f', 2 = f''
+  __device__ inline double spl64(EAMTables const& T, SplineDesc const& s, double x, unsigned& err) {
+    int i;
+    double t;
+    if (!seg64(s, x, i, t)) {
+      err |= ERR_SPLINE_OOB;
+      return 0.0;
+    }
+    double4 c = T.c64[s.off + i];
+    if (D == 0) return c.x + t * (c.y + t * (c.z + t * c.w));
+    if (D == 1) return c.y + t * (2.0 * c.z + t * 3.0 * c.w);
+    return 2.0 * c.z + 6.0 * t * c.w;
+  }
+
+  // f and f' from one segment fetch.
+  __device__ inline void spl64_ffp(EAMTables const& T, SplineDesc const& s, double x, double& f, double& fp, unsigned& err) {
+    int i;
+    double t;
+    if (!seg64(s, x, i, t)) {
+      err |= ERR_SPLINE_OOB;
+      f = fp = 0.0;
+      return;
+    }
+    double4 c = T.c64[s.off + i];
+    f = c.x + t * (c.y + t * (c.z + t * c.w));
+    fp = c.y + t * (2.0 * c.z + t * 3.0 * c.w);
+  }
+
+  // FP32 pair arithmetic on FP32 coefficients (mixed-precision form).
+  template <int D>
+  __device__ inline float spl32f(EAMTables const& T, SplineDesc const& s, float x, unsigned& err) {
+    float q = x * s.inv_dxf;
+    if (!(q < static_cast<float>(s.count)) || q <= -1.0f) {
+      err |= ERR_SPLINE_OOB;
+      return 0.0f;
+    }
+    int i = static_cast<int>(q);
+    float t = x - static_cast<float>(i) * s.dxf;
+    float4 c = T.c32[s.off + i];
+    if (D == 0) return c.x + t * (c.y + t * (c.z + t * c.w));
+    if (D == 1) return c.y + t * (2.0f * c.z + t * 3.0f * c.w);
+    return 2.0f * c.z + 6.0f * t * c.w;
+  }
+
+  // ----------------------------------------------------------------------------------------------
+  // Wavefront helpers (64 lanes on CDNA).
+  // ----------------------------------------------------------------------------------------------
+
+  template <int L>
+  __device__ inline double group_sum(double v) {   // sum over aligned groups of L lanes
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  }
+
+  // Deterministic block reduction of one double (fixed tree order); result valid in thread 0.
+  template <int NT>
+  __device__ inline double block_sum(double v, double* sh) {
+    v = group_sum<64>(v);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < NT / 64; ++k) s += sh[k];
+    }
+    __syncthreads();
+    return s;
+  }
+
+}  // namespace flyg

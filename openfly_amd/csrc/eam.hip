@@ -1,0 +1,447 @@
+// EAM / Finnis-Sinclair energy, gradient and Hessian-vector product on gfx950
+// (replaces potential::EAM::energy / gradient / hessian, eam.cpp:33-233).
+//
+// Work mapping: L lanes of a wavefront cooperate on one atom's neighbour row (ELL layout, 32-bit
+// packed entries), each lane striding the row by L; partial sums are combined with cross-lane
+// shuffles in a fixed order, so results are bitwise reproducible run to run.  No atomics touch
+// floating-point data.  Positions are double4 (x, y, z, w): a neighbour gather is one 32-byte load,
+// and pass 1 writes (x, y, z, F'(rho)) so pass 2 gets the neighbour's position and F' in that load.
+//
+//   pass 1 (density):  rho_b = sum_a phi_{t_a->t_b}(r_ab);  F'_b = F_{t_b}'(rho_b)     (eam.cpp:62-86)
+//   pass 2 (force):    dE/dr_g = -sum_a [V'(r) + F'_g phi'_{a->g}(r) + F'_a phi'_{g->a}(r)] (r_a - r_g)/r
+//                      frozen atoms get exactly zero                                   (eam.cpp:90-120)
+//   energy:            E = 1/2 sum_a sum_b V(r_ab) + sum_a F(rho_a), frozen included   (eam.cpp:33-51)
+//   HVP (matrix-free directional derivative of the gradient along v, v masked to non-frozen atoms):
+//      pass 1: drho_b = sum_a phi'_{a->b}(r) u.(v_a - v_b);  dF'_b = F''(rho_b) drho_b
+//      pass 2: (Hv)_g = -sum_a [dm u + m (dv - u s)/r],  s = u.dv,  dv = v_a - v_g, u = (r_a - r_g)/r,
+//              dm = (V'' + F'_g phi''_{a->g} + F'_a phi''_{g->a}) s + dF'_g phi'_{a->g} + dF'_a phi'_{g->a}
+//      which is H v for the analytic Hessian of eam.cpp:122-233 (pinned against it in the tests).
+#include <cstring>
+
+#include "eam_host.hpp"
+
+namespace flyg {
+
+  namespace {
+
+    constexpr int NT = 256;
+
+    struct Shifts {
+      double3 v[27];
+    };
+
+    __device__ inline int vidx(EAMTables const& T, int a, int b) { return a > b ? a + T.ntypes * b : b + T.ntypes * a; }
+
+    template <int L, bool MULTI, bool ENERGY>
+    __global__ __launch_bounds__(NT) void k_density(EAMTables T, Shifts S, PairArgs A, double4* __restrict__ posf,
+                                                    double* __restrict__ eatom, unsigned* __restrict__ err) {
+      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long i = gid / L;
+      int lane = static_cast<int>(gid % L);
+      bool act = i < A.n;
+      double rho = 0, vs = 0;
+      unsigned e = 0;
+      double4 pi = make_double4(0, 0, 0, 0);
+      int ti = 0;
+      if (act) {
+        pi = A.pos[i];
+        ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int c = A.cnt[i];
+        unsigned const* row = A.nbr + i * static_cast<long>(A.K);
+        for (int k = lane; k < c; k += L) {
+          unsigned en = row[k];
+          unsigned j = en & NB_IDX_MASK;
+          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double4 pj = A.pos[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < T.rc2) {
+            double r = sqrt(r2);
+            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            rho += spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e);
+            if (ENERGY) vs += spl64<0>(T, T.V[vidx(T, ti, tj)], r, e);
+          }
+        }
+      }
+      rho = group_sum<L>(rho);
+      if (ENERGY) vs = group_sum<L>(vs);
+      if (act && lane == 0) {
+        double f, fp;
+        spl64_ffp(T, T.F[ti], rho, f, fp, e);
+        posf[i] = make_double4(pi.x, pi.y, pi.z, fp);
+        if (ENERGY) eatom[i] = 0.5 * vs + f;
+      }
+      if (e) atomicOr(err, e);
+    }
+
+    template <int L, bool MULTI, bool MIX>
+    __global__ __launch_bounds__(NT) void k_force(EAMTables T, Shifts S, PairArgs A, double4 const* __restrict__ posf,
+                                                  double* __restrict__ grad, unsigned* __restrict__ err) {
+      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long i = gid / L;
+      int lane = static_cast<int>(gid % L);
+      bool inr = i < A.n;
+      bool act = inr && !A.frozen[i];
+      double gx = 0, gy = 0, gz = 0;
+      unsigned e = 0;
+      if (act) {
+        double4 pi = posf[i];
+        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int c = A.cnt[i];
+        unsigned const* row = A.nbr + i * static_cast<long>(A.K);
+        for (int k = lane; k < c; k += L) {
+          unsigned en = row[k];
+          unsigned j = en & NB_IDX_MASK;
+          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double4 pj = posf[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < T.rc2) {
+            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            if (MIX) {
+              float rf = sqrtf(static_cast<float>(r2));
+              float vp = spl32f<1>(T, T.V[vidx(T, ti, tj)], rf, e);
+              float pa = spl32f<1>(T, T.phi[tj + T.ntypes * ti], rf, e);
+              float pb = MULTI ? spl32f<1>(T, T.phi[ti + T.ntypes * tj], rf, e) : pa;
+              float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(pj.w) * pb;
+              double cf = static_cast<double>(mag / rf);
+              gx -= cf * dx;
+              gy -= cf * dy;
+              gz -= cf * dz;
+            } else {
+              double r = sqrt(r2);
+              double vp = spl64<1>(T, T.V[vidx(T, ti, tj)], r, e);
+              double pa = spl64<1>(T, T.phi[tj + T.ntypes * ti], r, e);
+              double pb = MULTI ? spl64<1>(T, T.phi[ti + T.ntypes * tj], r, e) : pa;
+              double mag = vp + pi.w * pa + pj.w * pb;
+              double cf = mag / r;
+              gx -= cf * dx;
+              gy -= cf * dy;
+              gz -= cf * dz;
+            }
+          }
+        }
+      }
+      gx = group_sum<L>(gx);
+      gy = group_sum<L>(gy);
+      gz = group_sum<L>(gz);
+      if (L >= 3 && inr && lane < 3) {
+        double v = lane == 0 ? gx : (lane == 1 ? gy : gz);
+        grad[3 * i + lane] = act ? v : 0.0;
+      }
+      if (L < 3 && inr && lane == 0) {   // fewer than 3 lanes per atom: lane 0 writes all components
+        grad[3 * i] = act ? gx : 0.0;
+        grad[3 * i + 1] = act ? gy : 0.0;
+        grad[3 * i + 2] = act ? gz : 0.0;
+      }
+      if (e) atomicOr(err, e);
+    }
+
+    template <int L, bool MULTI>
+    __global__ __launch_bounds__(NT) void k_hvp_density(EAMTables T, Shifts S, PairArgs A, double const* __restrict__ v,
+                                                        double2* __restrict__ dfp, unsigned* __restrict__ err) {
+      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long i = gid / L;
+      int lane = static_cast<int>(gid % L);
+      bool act = i < A.n;
+      double rho = 0, drho = 0;
+      unsigned e = 0;
+      int ti = 0;
+      if (act) {
+        double4 pi = A.pos[i];
+        ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        bool fi = A.frozen[i];
+        double vix = fi ? 0 : v[3 * i], viy = fi ? 0 : v[3 * i + 1], viz = fi ? 0 : v[3 * i + 2];
+        int c = A.cnt[i];
+        unsigned const* row = A.nbr + i * static_cast<long>(A.K);
+        for (int k = lane; k < c; k += L) {
+          unsigned en = row[k];
+          unsigned j = en & NB_IDX_MASK;
+          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double4 pj = A.pos[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < T.rc2) {
+            double r = sqrt(r2);
+            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            double f, fp;
+            spl64_ffp(T, T.phi[tj + T.ntypes * ti], r, f, fp, e);
+            rho += f;
+            bool fj = A.frozen[j];
+            double vjx = fj ? 0 : v[3 * j], vjy = fj ? 0 : v[3 * j + 1], vjz = fj ? 0 : v[3 * j + 2];
+            double s = (dx * (vjx - vix) + dy * (vjy - viy) + dz * (vjz - viz)) / r;
+            drho += fp * s;
+          }
+        }
+      }
+      rho = group_sum<L>(rho);
+      drho = group_sum<L>(drho);
+      if (act && lane == 0) {
+        double fp = spl64<1>(T, T.F[ti], rho, e);
+        double fpp = spl64<2>(T, T.F[ti], rho, e);
+        dfp[i] = make_double2(fp, fpp * drho);
+      }
+      if (e) atomicOr(err, e);
+    }
+
+    template <int L, bool MULTI>
+    __global__ __launch_bounds__(NT) void k_hvp_force(EAMTables T, Shifts S, PairArgs A, double const* __restrict__ v,
+                                                      double2 const* __restrict__ dfp, double* __restrict__ hv,
+                                                      unsigned* __restrict__ err) {
+      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long i = gid / L;
+      int lane = static_cast<int>(gid % L);
+      bool inr = i < A.n;
+      bool act = inr && !A.frozen[i];
+      double hx = 0, hy = 0, hz = 0;
+      unsigned e = 0;
+      if (act) {
+        double4 pi = A.pos[i];
+        double2 di = dfp[i];
+        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        double vix = v[3 * i], viy = v[3 * i + 1], viz = v[3 * i + 2];
+        int c = A.cnt[i];
+        unsigned const* row = A.nbr + i * static_cast<long>(A.K);
+        for (int k = lane; k < c; k += L) {
+          unsigned en = row[k];
+          unsigned j = en & NB_IDX_MASK;
+          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double4 pj = A.pos[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < T.rc2) {
+            double r = sqrt(r2);
+            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            double2 dj = dfp[j];
+            bool fj = A.frozen[j];
+            double vjx = fj ? 0 : v[3 * j], vjy = fj ? 0 : v[3 * j + 1], vjz = fj ? 0 : v[3 * j + 2];
+            double ux = dx / r, uy = dy / r, uz = dz / r;
+            double dvx = vjx - vix, dvy = vjy - viy, dvz = vjz - viz;
+            double s = ux * dvx + uy * dvy + uz * dvz;
+            double vp = spl64<1>(T, T.V[vidx(T, ti, tj)], r, e);
+            double vpp = spl64<2>(T, T.V[vidx(T, ti, tj)], r, e);
+            double pa = spl64<1>(T, T.phi[tj + T.ntypes * ti], r, e);   // phi'_{a->g}
+            double paa = spl64<2>(T, T.phi[tj + T.ntypes * ti], r, e);
+            double pb = pa, pbb = paa;
+            if (MULTI) {
+              pb = spl64<1>(T, T.phi[ti + T.ntypes * tj], r, e);          // phi'_{g->a}
+              pbb = spl64<2>(T, T.phi[ti + T.ntypes * tj], r, e);
+            }
+            double m = vp + di.x * pa + dj.x * pb;
+            double dm = (vpp + di.x * paa + dj.x * pbb) * s + di.y * pa + dj.y * pb;
+            double q = m / r;
+            hx -= dm * ux + q * (dvx - ux * s);
+            hy -= dm * uy + q * (dvy - uy * s);
+            hz -= dm * uz + q * (dvz - uz * s);
+          }
+        }
+      }
+      hx = group_sum<L>(hx);
+      hy = group_sum<L>(hy);
+      hz = group_sum<L>(hz);
+      if (inr && lane == 0) {
+        hv[3 * i] = act ? hx : 0.0;
+        hv[3 * i + 1] = act ? hy : 0.0;
+        hv[3 * i + 2] = act ? hz : 0.0;
+      }
+      if (e) atomicOr(err, e);
+    }
+
+    __global__ void k_table_eval(EAMTables T, SplineDesc s, double const* __restrict__ x, long m, int deriv,
+                                 double* __restrict__ out, unsigned* __restrict__ err) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= m) return;
+      unsigned e = 0;
+      double v = deriv == 0 ? spl64<0>(T, s, x[i], e) : deriv == 1 ? spl64<1>(T, s, x[i], e) : spl64<2>(T, s, x[i], e);
+      out[i] = v;
+      if (e) atomicOr(err, e);
+    }
+
+    Shifts shifts_of(BoxDesc const& b) {
+      Shifts s;
+      shift_table(b, s.v);
+      return s;
+    }
+
+    PairArgs pair_args(flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
+      return PairArgs{nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, nl.n, types, frozen};
+    }
+
+    // Lanes per atom for the full-system kernels (a row of ~60 entries split 8 ways keeps ~8 pairs
+    // in flight per lane and the grid at >= 8 wavefronts per SIMD for >= 30k atoms).
+    constexpr int LANES = 8;
+
+    unsigned grid_for(long n, int L) { return static_cast<unsigned>((n * L + NT - 1) / NT); }
+
+  }  // namespace
+
+  void eam_setup_tables(flyg_eam_s& E) {
+    SetflData const& D = E.data;
+    int const n = D.ntypes;
+    std::vector<double4> all;
+    EAMTables T{};
+    T.ntypes = n;
+    T.rc = D.rcut;
+    T.rc2 = D.rcut * D.rcut;
+    T.rcf = static_cast<float>(D.rcut);
+    auto add = [&](std::vector<double> const& y, double dx) {
+      std::vector<double4> c = spline_coefficients(y, dx);
+      SplineDesc s{};
+      s.off = static_cast<int>(all.size());
+      s.count = static_cast<int>(c.size());
+      s.dx = dx;
+      s.inv_dx = 1 / dx;
+      s.dxf = static_cast<float>(dx);
+      s.inv_dxf = static_cast<float>(1 / dx);
+      all.insert(all.end(), c.begin(), c.end());
+      return s;
+    };
+    for (int a = 0; a < n; ++a) T.F[a] = add(D.F[a], D.drho);
+    for (int k = 0; k < n * n; ++k) T.phi[k] = add(D.phi[k], D.dr);
+    for (int a = 0; a < n; ++a)
+      for (int b = 0; b <= a; ++b) T.V[a + n * b] = add(D.V[a + n * b], D.dr);
+    std::vector<float4> all32(all.size());
+    for (std::size_t i = 0; i < all.size(); ++i)
+      all32[i] = make_float4(static_cast<float>(all[i].x), static_cast<float>(all[i].y), static_cast<float>(all[i].z),
+                             static_cast<float>(all[i].w));
+    E.c64.reserve(all.size());
+    E.c32.reserve(all.size());
+    HIPCHK(hipMemcpy(E.c64.get(), all.data(), all.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(E.c32.get(), all32.data(), all32.size() * sizeof(float4), hipMemcpyHostToDevice));
+    T.c64 = E.c64.get();
+    T.c32 = E.c32.get();
+    E.tab = T;
+  }
+
+  namespace {
+    template <int L>
+    void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy) {
+      hipStream_t st = E.ctx->stream;
+      unsigned* err = E.ctx->dflags.get();
+      bool multi = E.tab.ntypes > 1;
+      unsigned g = grid_for(A.n, L);
+      if (multi) {
+        if (energy)
+          k_density<L, true, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), E.eatom.get(), err);
+        else
+          k_density<L, true, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), nullptr, err);
+      } else {
+        if (energy)
+          k_density<L, false, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), E.eatom.get(), err);
+        else
+          k_density<L, false, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), nullptr, err);
+      }
+    }
+
+    template <int L>
+    void launch_force(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* grad, int precision) {
+      hipStream_t st = E.ctx->stream;
+      unsigned* err = E.ctx->dflags.get();
+      bool multi = E.tab.ntypes > 1;
+      bool mix = precision == FLYG_MIXED;
+      unsigned g = grid_for(A.n, L);
+      if (multi) {
+        if (mix)
+          k_force<L, true, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+        else
+          k_force<L, true, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+      } else {
+        if (mix)
+          k_force<L, false, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+        else
+          k_force<L, false, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+      }
+    }
+  }  // namespace
+
+  double eam_energy(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
+    require(nl.built, "List used before rebuild");
+    require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
+    E.posf.reserve(nl.n);
+    E.eatom.reserve(nl.n);
+    launch_density<LANES>(E, shifts_of(nl.box), pair_args(nl, types, frozen), true);
+    HIPCHK(hipGetLastError());
+    dev_sum(*E.ctx, E.eatom.get(), nl.n, E.ctx->dscal.get());
+    HIPCHK(hipMemcpyAsync(E.ctx->hscal, E.ctx->dscal.get(), sizeof(double), hipMemcpyDeviceToHost, E.ctx->stream));
+    unsigned fl = E.ctx->take_flags();
+    if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+    return E.ctx->hscal[0];
+  }
+
+  void eam_gradient(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen, double* d_grad,
+                    int precision) {
+    require(nl.built, "List used before rebuild");
+    require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
+    E.posf.reserve(nl.n);
+    Shifts S = shifts_of(nl.box);
+    PairArgs A = pair_args(nl, types, frozen);
+    launch_density<LANES>(E, S, A, false);
+    launch_force<LANES>(E, S, A, d_grad, precision);
+    HIPCHK(hipGetLastError());
+  }
+
+  void eam_hvp(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen, double const* d_v,
+               double* d_hv) {
+    require(nl.built, "List used before rebuild");
+    require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
+    E.dfp.reserve(nl.n);
+    Shifts S = shifts_of(nl.box);
+    PairArgs A = pair_args(nl, types, frozen);
+    hipStream_t st = E.ctx->stream;
+    unsigned* err = E.ctx->dflags.get();
+    unsigned g = grid_for(nl.n, LANES);
+    if (E.tab.ntypes > 1) {
+      k_hvp_density<LANES, true><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), err);
+      k_hvp_force<LANES, true><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), d_hv, err);
+    } else {
+      k_hvp_density<LANES, false><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), err);
+      k_hvp_force<LANES, false><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), d_hv, err);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  void eam_table_eval(flyg_eam_s& E, int kind, int a, int b, double const* x, long m, int deriv, double* out) {
+    int n = E.tab.ntypes;
+    require(a >= 0 && a < n && b >= 0 && b < n, "type index out of range");
+    require(deriv >= 0 && deriv <= 2, "deriv must be 0, 1 or 2");
+    SplineDesc s = kind == 0 ? E.tab.F[a] : kind == 1 ? E.tab.phi[a + n * b] : E.tab.V[a > b ? a + n * b : b + n * a];
+    DBuf<double> dx, dout;
+    dx.reserve(m);
+    dout.reserve(m);
+    hipStream_t st = E.ctx->stream;
+    HIPCHK(hipMemcpyAsync(dx.get(), x, m * sizeof(double), hipMemcpyHostToDevice, st));
+    k_table_eval<<<static_cast<unsigned>((m + 255) / 256), 256, 0, st>>>(E.tab, s, dx.get(), m, deriv, dout.get(),
+                                                                         E.ctx->dflags.get());
+    HIPCHK(hipMemcpyAsync(out, dout.get(), m * sizeof(double), hipMemcpyDeviceToHost, st));
+    unsigned fl = E.ctx->take_flags();
+    if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+  }
+
+  void stage_atoms(flyg_eam_s& E, uint32_t const* types, uint8_t const* frozen, long n, int where,
+                   uint32_t const** d_types, uint8_t const** d_frozen) {
+    if (where == FLYG_DEVICE) {
+      *d_types = types;
+      *d_frozen = frozen;
+      return;
+    }
+    E.types.reserve(n);
+    E.frozen.reserve(n);
+    bool same_t = static_cast<long>(E.last_types.size()) == n &&
+                  std::memcmp(E.last_types.data(), types, n * sizeof(uint32_t)) == 0;
+    bool same_f = static_cast<long>(E.last_frozen.size()) == n &&
+                  std::memcmp(E.last_frozen.data(), frozen, n * sizeof(uint8_t)) == 0;
+    if (!same_t) {
+      for (long i = 0; i < n; ++i) require(types[i] < static_cast<uint32_t>(E.tab.ntypes), "TypeID out of range");
+      E.last_types.assign(types, types + n);
+      HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, E.ctx->stream));
+    }
+    if (!same_f) {
+      E.last_frozen.assign(frozen, frozen + n);
+      HIPCHK(hipMemcpyAsync(E.frozen.get(), frozen, n * sizeof(uint8_t), hipMemcpyHostToDevice, E.ctx->stream));
+    }
+    if (!same_t || !same_f) HIPCHK(hipStreamSynchronize(E.ctx->stream));
+    *d_types = E.types.get();
+    *d_frozen = E.frozen.get();
+  }
+
+}  // namespace flyg

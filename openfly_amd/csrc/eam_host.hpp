@@ -1,0 +1,118 @@
+// Internal handle types of libflygpu (host side).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace flyg {
+
+  // Raw setfl contents (host, FP64).
+  struct SetflData {
+    int ntypes = 0;
+    std::vector<std::string> names;
+    std::vector<double> mass;
+    double drho = 0, dr = 0, rcut = 0;
+    std::vector<std::vector<double>> F, phi, V;   // phi[a + n*b], V[max + n*min]
+  };
+
+  SetflData read_setfl(std::string const& path);
+  std::vector<double4> spline_coefficients(std::vector<double> y, double dx);
+
+}  // namespace flyg
+
+// ------------------------------------------------------------------------------------------------
+// Handles (the opaque types of flygpu.h).
+// ------------------------------------------------------------------------------------------------
+
+struct flyg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cus = 0;
+  flyg::DBuf<unsigned> dflags;   // [0] error bits, [1..] scratch counters
+  flyg::DBuf<double> dscal;      // scalar scratch for reductions
+  flyg::DBuf<double> dpart;      // block partial sums
+  unsigned* hflags = nullptr;    // pinned mirror of small results
+  double* hscal = nullptr;
+
+  void set_device() const;
+  unsigned take_flags();         // sync, read and clear the error bits
+};
+
+struct flyg_eam_s {
+  flyg_ctx_s* ctx = nullptr;
+  flyg::SetflData data;
+  flyg::EAMTables tab{};   // device view
+  flyg::DBuf<double4> c64;
+  flyg::DBuf<float4> c32;
+  int precision = FLYG_FP64;
+  // scratch
+  flyg::DBuf<double4> posf;     // positions + F' (pass 1 -> pass 2)
+  flyg::DBuf<double2> dfp;      // HVP: (F', dF') per atom
+  flyg::DBuf<double> eatom;     // per-atom energies
+  flyg::DBuf<uint32_t> types;   // staged host inputs
+  flyg::DBuf<uint8_t> frozen;
+  flyg::DBuf<double> vec_in, vec_out;
+  std::vector<uint32_t> last_types;
+  std::vector<uint8_t> last_frozen;
+};
+
+struct flyg_list_s {
+  flyg_ctx_s* ctx = nullptr;
+  flyg::BoxDesc box{};
+  flyg::GridDesc grid{};
+  double rc = 0;
+  long n = 0;
+  long nghost = 0;
+  int K = 0;       // row capacity of the neighbour table
+  bool built = false;
+  flyg::DBuf<double4> pos;       // live positions (x, y, z, 0)
+  flyg::DBuf<double4> ent;       // reals + ghost images at build time (chained positions)
+  flyg::DBuf<unsigned> eid;      // packed (shift, real) of ent
+  flyg::DBuf<int> ecell;         // cell of ent
+  flyg::DBuf<int> ccount, cstart, ccursor;
+  flyg::DBuf<double4> spos;      // ent sorted by cell
+  flyg::DBuf<unsigned> sid;
+  flyg::DBuf<unsigned> nbr;      // n x K packed entries
+  flyg::DBuf<int> ncnt;          // entries per atom
+  flyg::DBuf<int> scan_tmp;
+  flyg::DBuf<double> stage;      // host staging of xyz vectors
+};
+
+namespace flyg {
+  // Shift vectors of the 27 image codes (sum of +-basis columns), for pair arithmetic.
+  void shift_table(BoxDesc const& b, double3* out27);
+
+  // neigh.hip
+  void list_rebuild(flyg_list_s& nl, double const* d_xyz, long n);
+  void list_update(flyg_list_s& nl, double const* d_delta, long n);
+  void exclusive_scan(flyg_ctx_s& ctx, int* data, int n, flyg::DBuf<int>& tmp);
+
+  // eam.hip
+  struct PairArgs {
+    double4 const* pos;
+    unsigned const* nbr;
+    int const* cnt;
+    int K;
+    long n;
+    uint32_t const* types;
+    uint8_t const* frozen;
+  };
+  void eam_setup_tables(flyg_eam_s& e);
+  double eam_energy(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen);
+  void eam_gradient(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen, double* d_grad,
+                    int precision);
+  void eam_hvp(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen, double const* d_v,
+               double* d_hv);
+  void eam_table_eval(flyg_eam_s& e, int kind, int a, int b, double const* x, long m, int deriv, double* out);
+
+  // Stage host per-atom inputs into the potential handle (re-uploads only when they changed).
+  void stage_atoms(flyg_eam_s& e, uint32_t const* types, uint8_t const* frozen, long n, int where,
+                   uint32_t const** d_types, uint8_t const** d_frozen);
+
+  // Deterministic device reductions (vecops.hip).
+  void dev_sum(flyg_ctx_s& ctx, double const* x, long n, double* d_out);
+}  // namespace flyg

@@ -1,0 +1,489 @@
+// Neighbour-list build and update on gfx950 (replaces neigh::List, list.cpp:37-165).
+//
+// Rebuild pipeline (all on the device, one HIP stream):
+//   1. canonicalise real positions                       (List::rebuild copy-in, list.cpp:68-72)
+//   2. ghost images, axis by axis, chained               (List::make_ghosts, list.cpp:100-132)
+//   3. counting-sort binning of reals + ghosts into the HyperGrid cells (atomic histogram, exclusive
+//      scan, scatter, then each cell sorted by packed id so the result does not depend on atomics)
+//   4. one thread per real atom scans its 27 cells and appends every (real, image) with d^2 < r_cut^2
+//      (strict, list.cpp:146,159) to its row of an ELL table: n rows x K packed 32-bit entries.
+// The distance tests, canonical images and ghost chains round exactly like the reference's FP64
+// expressions (no FMA contraction), so the pair sets equal the reference's at rebuild time.
+// Ghosts are not kept after the build: an entry stores (shift code, real index) and the pair kernels
+// add the shift vector to the live real position, so update() touches only the N real atoms.
+#include <algorithm>
+#include <numeric>
+
+#include "eam_host.hpp"
+
+namespace flyg {
+
+  namespace {
+
+    constexpr int TPB = 256;
+
+    __device__ inline void canon_image(BoxDesc const& b, double* x) {
+#pragma clang fp contract(off)
+      if (b.ortho) {
+        for (int k = 0; k < 3; ++k) x[k] = x[k] - b.ext[k] * floor(x[k] * b.inv_ext[k]);
+        return;
+      }
+      double f[3];
+      for (int r = 0; r < 3; ++r) f[r] = (b.Binv[r][0] * x[0] + b.Binv[r][1] * x[1]) + b.Binv[r][2] * x[2];
+      for (int r = 0; r < 3; ++r) f[r] = f[r] - floor(f[r]);
+      for (int r = 0; r < 3; ++r) x[r] = (b.B[r][0] * f[0] + b.B[r][1] * f[1]) + b.B[r][2] * f[2];
+    }
+
+    // OrthoGrid::gen_image (orthorhombic.hpp:54-74) / TriGrid::gen_image (triclinic.hpp:47-71).
+    __device__ inline bool gen_image(BoxDesc const& b, double rc, int sign, double* x, int ax) {
+#pragma clang fp contract(off)
+      if (b.ortho) {
+        if (sign > 0) {
+          if (x[ax] < rc) {
+            x[ax] += b.ext[ax];
+            return true;
+          }
+        } else if (x[ax] > b.ext[ax] - rc) {
+          x[ax] -= b.ext[ax];
+          return true;
+        }
+        return false;
+      }
+      if (sign > 0) {
+        double d = (x[0] * b.hyper[0][ax] + x[1] * b.hyper[1][ax]) + x[2] * b.hyper[2][ax];
+        if (d < rc) {
+          for (int r = 0; r < 3; ++r) x[r] = x[r] + b.B[r][ax];
+          return true;
+        }
+      } else {
+        double d = ((b.B[0][ax] - x[0]) * b.hyper[0][ax] + (b.B[1][ax] - x[1]) * b.hyper[1][ax])
+                   + (b.B[2][ax] - x[2]) * b.hyper[2][ax];
+        if (d < rc) {
+          for (int r = 0; r < 3; ++r) x[r] = x[r] - b.B[r][ax];
+          return true;
+        }
+      }
+      return false;
+    }
+
+    // Position of the image with shift code s of a real atom at x: basis columns added axis by axis
+    // (the chain make_ghosts produces).
+    __device__ inline void image_of(BoxDesc const& b, unsigned s, double* x) {
+#pragma clang fp contract(off)
+      int sa[3] = {static_cast<int>(s % 3) - 1, static_cast<int>((s / 3) % 3) - 1, static_cast<int>(s / 9) - 1};
+      for (int ax = 0; ax < 3; ++ax) {
+        if (sa[ax] > 0) {
+          for (int r = 0; r < 3; ++r) x[r] = x[r] + b.B[r][ax];
+        } else if (sa[ax] < 0) {
+          for (int r = 0; r < 3; ++r) x[r] = x[r] - b.B[r][ax];
+        }
+      }
+    }
+
+    // HyperGrid::cell_idx (hypergrid.hpp:74-99): clamp(int((x + cell) / cell)) per axis, x fastest.
+    __device__ inline int cell_idx(GridDesc const& g, double const* x, int* c3) {
+#pragma clang fp contract(off)
+      int idx = 0;
+      for (int k = 0; k < 3; ++k) {
+        int i = static_cast<int>((x[k] + g.cell[k]) * g.inv_cell[k]);
+        i = i < 0 ? 0 : (i > g.shape[k] - 1 ? g.shape[k] - 1 : i);
+        c3[k] = i;
+        idx += i * g.prod[k];
+      }
+      return idx;
+    }
+
+    __device__ inline double dist2(double const* a, double const* b) {
+#pragma clang fp contract(off)
+      double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+      return (dx * dx + dy * dy) + dz * dz;
+    }
+
+    __global__ void k_canon(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos,
+                            double4* __restrict__ ent, unsigned* __restrict__ eid) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+      canon_image(b, x);
+      double4 p = make_double4(x[0], x[1], x[2], 0.0);
+      pos[i] = p;
+      ent[i] = p;
+      eid[i] = (13u << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+    }
+
+    // make_ghosts for one real atom: the tree of chained images over the periodic axes.
+    template <bool WRITE>
+    __global__ void k_ghosts(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
+                             unsigned* __restrict__ eid, unsigned* __restrict__ counter, long cap) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double cx[27][3];
+      int cs[27];
+      int m = 1;
+      double4 p = pos[i];
+      cx[0][0] = p.x;
+      cx[0][1] = p.y;
+      cx[0][2] = p.z;
+      cs[0] = 13;
+      int const pow3[3] = {1, 3, 9};
+      for (int ax = 0; ax < 3; ++ax) {
+        if (!b.per[ax]) continue;
+        int const end = m;
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+          for (int c = 0; c < end; ++c) {
+            double im[3] = {cx[c][0], cx[c][1], cx[c][2]};
+            if (gen_image(b, rc, sgn, im, ax)) {
+              cx[m][0] = im[0];
+              cx[m][1] = im[1];
+              cx[m][2] = im[2];
+              cs[m] = cs[c] + sgn * pow3[ax];
+              ++m;
+            }
+          }
+        }
+      }
+      unsigned g = static_cast<unsigned>(m - 1);
+      if (g == 0) return;
+      unsigned base = atomicAdd(counter, g);
+      if (!WRITE) return;
+      for (int k = 1; k < m; ++k) {
+        long slot = n + static_cast<long>(base) + (k - 1);
+        if (slot >= cap) return;   // capacity is sized from the counting pass; never taken
+        ent[slot] = make_double4(cx[k][0], cx[k][1], cx[k][2], 0.0);
+        eid[slot] = (static_cast<unsigned>(cs[k]) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+      }
+    }
+
+    __global__ void k_cell_count(GridDesc g, double4 const* __restrict__ ent, long m, int* __restrict__ ecell,
+                                 int* __restrict__ ccount) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= m) return;
+      double4 p = ent[e];
+      double x[3] = {p.x, p.y, p.z};
+      int c3[3];
+      int c = cell_idx(g, x, c3);
+      ecell[e] = c;
+      atomicAdd(&ccount[c], 1);
+    }
+
+    __global__ void k_scatter(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
+                              int const* __restrict__ ecell, long m, int const* __restrict__ cstart,
+                              int* __restrict__ ccursor, double4* __restrict__ spos, unsigned* __restrict__ sid) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= m) return;
+      int c = ecell[e];
+      int slot = cstart[c] + atomicAdd(&ccursor[c], 1);
+      spos[slot] = ent[e];
+      sid[slot] = eid[e];
+    }
+
+    // Orders each cell by packed id so that neighbour rows are independent of atomic ordering.
+    __global__ void k_cell_sort(int ncells, int const* __restrict__ cstart, int const* __restrict__ ccount,
+                                double4* __restrict__ spos, unsigned* __restrict__ sid) {
+      int c = blockIdx.x * blockDim.x + threadIdx.x;
+      if (c >= ncells) return;
+      int b = cstart[c], e = b + ccount[c];
+      for (int i = b + 1; i < e; ++i) {
+        unsigned key = sid[i];
+        double4 v = spos[i];
+        int j = i - 1;
+        while (j >= b && sid[j] > key) {
+          sid[j + 1] = sid[j];
+          spos[j + 1] = spos[j];
+          --j;
+        }
+        sid[j + 1] = key;
+        spos[j + 1] = v;
+      }
+    }
+
+    // List::build_neigh_list (list.cpp:134-165) for every real atom: own cell and its neighbours.
+    __global__ void k_build(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid, long m,
+                            int const* __restrict__ cstart, int const* __restrict__ ccount, unsigned* __restrict__ nbr,
+                            int* __restrict__ ncnt, int K, unsigned* __restrict__ maxcnt) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= m) return;
+      unsigned id = sid[e];
+      if ((id >> NB_SHIFT_BITS) != 13u) return;   // ghosts have no rows
+      long i = id & NB_IDX_MASK;
+      double4 p = spos[e];
+      double xi[3] = {p.x, p.y, p.z};
+      int c3[3];
+      cell_idx(g, xi, c3);
+      double const rc2 = g.rc * g.rc;
+      int cnt = 0;
+      unsigned* row = nbr + i * static_cast<long>(K);
+      for (int z = max(c3[2] - 1, 0); z < min(c3[2] + 2, g.shape[2]); ++z)
+        for (int y = max(c3[1] - 1, 0); y < min(c3[1] + 2, g.shape[1]); ++y)
+          for (int x = max(c3[0] - 1, 0); x < min(c3[0] + 2, g.shape[0]); ++x) {
+            int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
+            int b = cstart[cc], end = b + ccount[cc];
+            for (int s = b; s < end; ++s) {
+              if (s == e) continue;
+              double4 q = spos[s];
+              double xn[3] = {q.x, q.y, q.z};
+              if (dist2(xi, xn) < rc2) {
+                if (cnt < K) row[cnt] = sid[s];
+                ++cnt;
+              }
+            }
+          }
+      ncnt[i] = cnt;
+      atomicMax(maxcnt, static_cast<unsigned>(cnt));
+    }
+
+    __global__ void k_update(double4* __restrict__ pos, double const* __restrict__ d, long n) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double4 p = pos[i];
+      p.x -= d[3 * i];
+      p.y -= d[3 * i + 1];
+      p.z -= d[3 * i + 2];
+      pos[i] = p;
+    }
+
+    __global__ void k_positions(double4 const* __restrict__ pos, long n, double* __restrict__ out) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double4 p = pos[i];
+      out[3 * i] = p.x;
+      out[3 * i + 1] = p.y;
+      out[3 * i + 2] = p.z;
+    }
+
+    // for_neighbours(i, r_cut) with the image chain: count, then write (j, shift, dr) rows.
+    template <bool WRITE>
+    __global__ void k_export(BoxDesc b, double rc, double4 const* __restrict__ pos, unsigned const* __restrict__ nbr,
+                             int const* __restrict__ ncnt, int K, long n, long long const* __restrict__ off,
+                             int* __restrict__ counts, int* __restrict__ oj, int* __restrict__ os, double* __restrict__ odr) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double4 p = pos[i];
+      double xi[3] = {p.x, p.y, p.z};
+      int c = 0;
+      long long o = WRITE ? off[i] : 0;
+      for (int k = 0; k < ncnt[i]; ++k) {
+        unsigned e = nbr[i * static_cast<long>(K) + k];
+        unsigned j = e & NB_IDX_MASK, s = e >> NB_SHIFT_BITS;
+        double4 q = pos[j];
+        double xn[3] = {q.x, q.y, q.z};
+        image_of(b, s, xn);
+        double dr[3], r2;
+        {
+#pragma clang fp contract(off)
+          for (int r = 0; r < 3; ++r) dr[r] = xn[r] - xi[r];
+          r2 = (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+        }
+        if (r2 < rc * rc) {
+          if (WRITE) {
+            oj[o + c] = static_cast<int>(j);
+            os[o + c] = static_cast<int>(s);
+            for (int r = 0; r < 3; ++r) odr[3 * (o + c) + r] = dr[r];
+          }
+          ++c;
+        }
+      }
+      if (!WRITE) counts[i] = c;
+    }
+
+    // ---- exclusive scan (int), two levels per recursion --------------------------------------
+    constexpr int SCAN_B = 1024;
+
+    __global__ void k_scan_block(int* __restrict__ data, int n, int* __restrict__ sums) {
+      __shared__ int wsum[SCAN_B / 64];
+      int gi = blockIdx.x * SCAN_B + threadIdx.x;
+      int v = gi < n ? data[gi] : 0;
+      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      int inc = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      if (lane == 63) wsum[w] = inc;
+      __syncthreads();
+      if (w == 0) {
+        int s = lane < SCAN_B / 64 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          int t = __shfl_up(s, o, 64);
+          if (lane >= o) s += t;
+        }
+        if (lane < SCAN_B / 64) wsum[lane] = s;
+      }
+      __syncthreads();
+      int before = (w > 0 ? wsum[w - 1] : 0) + inc - v;
+      if (gi < n) data[gi] = before;
+      if (threadIdx.x == SCAN_B - 1) sums[blockIdx.x] = wsum[SCAN_B / 64 - 1];
+    }
+
+    __global__ void k_scan_add(int* __restrict__ data, int n, int const* __restrict__ sums) {
+      int gi = blockIdx.x * SCAN_B + threadIdx.x;
+      if (gi < n) data[gi] += sums[blockIdx.x];
+    }
+
+    inline unsigned nblk(long n, int t) { return static_cast<unsigned>((n + t - 1) / t); }
+
+  }  // namespace
+
+  void exclusive_scan(flyg_ctx_s& ctx, int* data, int n, DBuf<int>& tmp) {
+    if (n <= 0) return;
+    int nb = static_cast<int>((n + SCAN_B - 1) / SCAN_B);
+    // Workspace layout: [sums of this level | workspace of the next level]
+    std::vector<int> offs;
+    std::vector<int> sizes;
+    int total = 0, cur = nb;
+    while (true) {
+      offs.push_back(total);
+      sizes.push_back(cur);
+      total += cur;
+      if (cur <= 1) break;
+      cur = (cur + SCAN_B - 1) / SCAN_B;
+    }
+    tmp.reserve(static_cast<std::size_t>(total) + 1);
+    // Down-sweep of block sums.
+    int* level_data = data;
+    int level_n = n;
+    for (std::size_t l = 0; l < offs.size(); ++l) {
+      int blocks = (level_n + SCAN_B - 1) / SCAN_B;
+      k_scan_block<<<blocks, SCAN_B, 0, ctx.stream>>>(level_data, level_n, tmp.get() + offs[l]);
+      if (blocks == 1) break;
+      level_data = tmp.get() + offs[l];
+      level_n = blocks;
+    }
+    // Up-sweep: add scanned block sums back, deepest level first.
+    for (std::size_t l = offs.size(); l-- > 0;) {
+      int ln = l == 0 ? n : sizes[l - 1];
+      int* ld = l == 0 ? data : tmp.get() + offs[l - 1];
+      int blocks = (ln + SCAN_B - 1) / SCAN_B;
+      if (blocks == 1) continue;
+      k_scan_add<<<blocks, SCAN_B, 0, ctx.stream>>>(ld, ln, tmp.get() + offs[l]);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  void shift_table(BoxDesc const& b, double3* out) {
+    for (int s = 0; s < 27; ++s) {
+      int sa[3] = {s % 3 - 1, (s / 3) % 3 - 1, s / 9 - 1};
+      double v[3] = {0, 0, 0};
+      for (int ax = 0; ax < 3; ++ax)
+        for (int r = 0; r < 3; ++r) v[r] += sa[ax] * b.B[r][ax];
+      out[s] = make_double3(v[0], v[1], v[2]);
+    }
+  }
+
+  void list_rebuild(flyg_list_s& nl, double const* d_xyz, long n) {
+    require(n > 0, "List::rebuild with no atoms");
+    require(n < MAX_ATOMS, "too many atoms for the packed neighbour format");
+    flyg_ctx_s& ctx = *nl.ctx;
+    hipStream_t st = ctx.stream;
+    nl.n = n;
+    nl.pos.reserve(n);
+    nl.ent.reserve(n);
+    nl.eid.reserve(n);
+    k_canon<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get());
+
+    // Ghost count, then ghost images.
+    unsigned* ctr = ctx.dflags.get() + 1;
+    HIPCHK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
+    k_ghosts<false><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nullptr, nullptr, ctr, 0);
+    HIPCHK(hipMemcpyAsync(ctx.hflags, ctr, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    long G = ctx.hflags[0];
+    long m = n + G;
+    nl.nghost = G;
+    nl.ent.reserve(m);
+    nl.eid.reserve(m);
+    HIPCHK(hipMemsetAsync(ctr, 0, sizeof(unsigned), st));
+    k_ghosts<true><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), ctr, m);
+
+    // Counting sort into cells.
+    int nc = nl.grid.ncells;
+    nl.ccount.reserve(nc);
+    nl.cstart.reserve(nc);
+    nl.ccursor.reserve(nc);
+    nl.ecell.reserve(m);
+    nl.spos.reserve(m);
+    nl.sid.reserve(m);
+    HIPCHK(hipMemsetAsync(nl.ccount.get(), 0, nc * sizeof(int), st));
+    HIPCHK(hipMemsetAsync(nl.ccursor.get(), 0, nc * sizeof(int), st));
+    k_cell_count<<<nblk(m, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), m, nl.ecell.get(), nl.ccount.get());
+    HIPCHK(hipMemcpyAsync(nl.cstart.get(), nl.ccount.get(), nc * sizeof(int), hipMemcpyDeviceToDevice, st));
+    exclusive_scan(ctx, nl.cstart.get(), nc, nl.scan_tmp);
+    k_scatter<<<nblk(m, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), m, nl.cstart.get(),
+                                           nl.ccursor.get(), nl.spos.get(), nl.sid.get());
+    k_cell_sort<<<nblk(nc, TPB), TPB, 0, st>>>(nc, nl.cstart.get(), nl.ccount.get(), nl.spos.get(), nl.sid.get());
+
+    // Neighbour rows; grow K and redo the last step if any row overflowed.
+    if (nl.K == 0) nl.K = 64;
+    nl.ncnt.reserve(n);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
+      unsigned* mx = ctx.dflags.get() + 2;
+      HIPCHK(hipMemsetAsync(mx, 0, sizeof(unsigned), st));
+      k_build<<<nblk(m, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), m, nl.cstart.get(), nl.ccount.get(),
+                                           nl.nbr.get(), nl.ncnt.get(), nl.K, mx);
+      HIPCHK(hipMemcpyAsync(ctx.hflags + 1, mx, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      int maxc = static_cast<int>(ctx.hflags[1]);
+      if (maxc <= nl.K) break;
+      nl.K = (maxc + 15) / 16 * 16 + 16;
+    }
+    HIPCHK(hipGetLastError());
+    nl.built = true;
+  }
+
+  void list_update(flyg_list_s& nl, double const* d_delta, long n) {
+    require(nl.built, "List::update before rebuild");
+    require(n == nl.n, "Input to update has the wrong number of atoms");
+    k_update<<<nblk(n, TPB), TPB, 0, nl.ctx->stream>>>(nl.pos.get(), d_delta, n);
+    HIPCHK(hipGetLastError());
+  }
+
+  // Host-facing helpers used by the C-ABI.
+  void list_positions(flyg_list_s& nl, double* d_out) {
+    k_positions<<<nblk(nl.n, TPB), TPB, 0, nl.ctx->stream>>>(nl.pos.get(), nl.n, d_out);
+    HIPCHK(hipGetLastError());
+  }
+
+  long list_count(flyg_list_s& nl, double rc, std::vector<int>& counts) {
+    require(nl.built, "List used before rebuild");
+    require(rc <= nl.rc, "r_cut is bigger than the internal r_cut");
+    flyg_ctx_s& ctx = *nl.ctx;
+    DBuf<int> dc;
+    dc.reserve(nl.n);
+    k_export<false><<<nblk(nl.n, TPB), TPB, 0, ctx.stream>>>(nl.box, rc, nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K,
+                                                            nl.n, nullptr, dc.get(), nullptr, nullptr, nullptr);
+    counts.resize(nl.n);
+    HIPCHK(hipMemcpyAsync(counts.data(), dc.get(), nl.n * sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    long t = 0;
+    for (int c : counts) t += c;
+    return t;
+  }
+
+  void list_export(flyg_list_s& nl, double rc, int64_t* offsets, int32_t* j, int32_t* shift, double* dr) {
+    std::vector<int> counts;
+    long total = list_count(nl, rc, counts);
+    flyg_ctx_s& ctx = *nl.ctx;
+    std::vector<long long> off(nl.n + 1);
+    off[0] = 0;
+    for (long i = 0; i < nl.n; ++i) off[i + 1] = off[i] + counts[i];
+    for (long i = 0; i <= nl.n; ++i) offsets[i] = off[i];
+    DBuf<long long> doff;
+    DBuf<int> dj, ds;
+    DBuf<double> ddr;
+    doff.reserve(nl.n + 1);
+    dj.reserve(total + 1);
+    ds.reserve(total + 1);
+    ddr.reserve(3 * total + 3);
+    HIPCHK(hipMemcpyAsync(doff.get(), off.data(), (nl.n + 1) * sizeof(long long), hipMemcpyHostToDevice, ctx.stream));
+    k_export<true><<<nblk(nl.n, TPB), TPB, 0, ctx.stream>>>(nl.box, rc, nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K,
+                                                           nl.n, doff.get(), nullptr, dj.get(), ds.get(), ddr.get());
+    HIPCHK(hipMemcpyAsync(j, dj.get(), total * sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipMemcpyAsync(shift, ds.get(), total * sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipMemcpyAsync(dr, ddr.get(), 3 * total * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+  }
+
+}  // namespace flyg

@@ -1,0 +1,481 @@
+"""Python mirror of the reference's hot-path API over the libflygpu C-ABI (include/flygpu.h).
+
+Names and argument meanings follow the reference classes:
+  system::Box (box.hpp:103-120)            -> Box
+  neigh::List (list.hpp:71-210)            -> List
+  potential::DataEAM + EAM (data.hpp, eam.hpp) -> EAM
+  minimise::StepLBFGS / LBFGS (lbfgs_core.hpp, lbfgs.hpp) -> StepLBFGS / LBFGS
+  saddle::Rotor + Dimer (rotor.hpp, dimer.hpp) -> Dimer
+  saddle::Master::find_batch (find.cpp:684-755) on the device -> Batch
+Errors map to FlyError (fly::RuntimeError) and SplineOutOfBounds (fly::Spline::OutOfBounds).
+
+Arrays: numpy arrays are host data; torch tensors on a GPU are passed as device pointers (the caller
+orders torch's stream against the context's stream, e.g. with torch.cuda.synchronize()).
+There is no CPU fallback: without the built library every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflygpu.so")
+
+FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR = 0, 1, 2, 3
+HOST, DEVICE = 0, 1
+FP64, MIXED = 0, 1
+EXIT_NAMES = {0: "success", 1: "convex", 2: "iter_max", 3: "collision", 4: "uninit"}
+
+
+class FlyError(RuntimeError):
+    """fly::RuntimeError."""
+
+
+class SplineOutOfBounds(FlyError):
+    """fly::Spline::OutOfBounds."""
+
+
+class DeviceError(FlyError):
+    pass
+
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+_u32 = C.POINTER(C.c_uint32)
+_u8 = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+_lib = None
+
+
+class LBFGSOptions(C.Structure):
+    _fields_ = [("n", C.c_int), ("iter_max", C.c_int), ("f2norm", C.c_double), ("skin_frac", C.c_double),
+                ("proj_tol", C.c_double), ("max_trust", C.c_double), ("min_trust", C.c_double),
+                ("grow_trust", C.c_double), ("shrink_trust", C.c_double)]
+
+
+class DimerOptions(C.Structure):
+    _fields_ = [("n", C.c_int), ("max_steps", C.c_int), ("hist_check_freq", C.c_int), ("f2norm", C.c_double),
+                ("skin_frac", C.c_double), ("proj_tol", C.c_double), ("max_trust", C.c_double),
+                ("min_trust", C.c_double), ("grow_trust", C.c_double), ("shrink_trust", C.c_double),
+                ("convex_max", C.c_int), ("use_history", C.c_int)]
+
+
+class RotorOptions(C.Structure):
+    _fields_ = [("n", C.c_int), ("iter_max_rot", C.c_int), ("delta_r", C.c_double), ("theta_tol", C.c_double),
+                ("relax_in_convex", C.c_int)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("exit_code", C.c_int), ("steps", C.c_int), ("grads", C.c_int64), ("rebuilds", C.c_int),
+                ("curv", C.c_double), ("energy", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("block_threads", C.c_int), ("lanes_per_atom", C.c_int),
+                ("list_skin", C.c_double), ("max_concurrent", C.c_int)]
+
+
+_SIGS = {
+    "flyg_ctx_create": [C.c_int, C.POINTER(_vp)],
+    "flyg_ctx_destroy": [_vp],
+    "flyg_ctx_last_error": [_vp],
+    "flyg_ctx_synchronize": [_vp],
+    "flyg_ctx_stream": [_vp],
+    "flyg_ctx_device_info": [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)],
+    "flyg_eam_load": [_vp, C.c_char_p, C.POINTER(_vp)],
+    "flyg_eam_destroy": [_vp],
+    "flyg_eam_info": [_vp, C.POINTER(C.c_int), _d, _d, C.c_char_p],
+    "flyg_eam_set_precision": [_vp, C.c_int],
+    "flyg_eam_table_eval": [_vp, C.c_int, C.c_int, C.c_int, _d, C.c_int64, C.c_int, _d],
+    "flyg_list_create": [_vp, _d, C.POINTER(C.c_int), C.c_double, C.POINTER(_vp)],
+    "flyg_list_destroy": [_vp],
+    "flyg_list_rebuild": [_vp, _vp, C.c_int64, C.c_int],
+    "flyg_list_update": [_vp, _vp, C.c_int64, C.c_int],
+    "flyg_list_size": [_vp, _i64],
+    "flyg_list_num_ghosts": [_vp, _i64],
+    "flyg_list_count": [_vp, C.c_double, _i64],
+    "flyg_list_export": [_vp, C.c_double, _i64, _i32, _i32, _d],
+    "flyg_list_positions": [_vp, _vp, C.c_int],
+    "flyg_eam_energy": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _d],
+    "flyg_eam_gradient": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _vp],
+    "flyg_eam_hvp": [_vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int],
+    "flyg_steplbfgs_create": [_vp, C.c_int, C.POINTER(_vp)],
+    "flyg_steplbfgs_destroy": [_vp],
+    "flyg_steplbfgs_clear": [_vp],
+    "flyg_steplbfgs_newton_step": [_vp, _vp, _vp, C.c_int64, _vp, C.c_int],
+    "flyg_lbfgs_default_options": [C.POINTER(LBFGSOptions)],
+    "flyg_lbfgs_create": [_vp, C.POINTER(LBFGSOptions), _d, C.POINTER(C.c_int), C.POINTER(_vp)],
+    "flyg_lbfgs_destroy": [_vp],
+    "flyg_lbfgs_minimise": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_int, C.POINTER(C.c_int),
+                            C.POINTER(C.c_int)],
+    "flyg_dimer_default_options": [C.POINTER(DimerOptions)],
+    "flyg_rotor_default_options": [C.POINTER(RotorOptions)],
+    "flyg_dimer_create": [_vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), _d, C.POINTER(C.c_int),
+                          C.POINTER(_vp)],
+    "flyg_dimer_destroy": [_vp],
+    "flyg_dimer_find_sp": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int, C.c_double, C.c_int,
+                           C.c_int, C.POINTER(SearchStats)],
+    "flyg_batch_default_options": [C.POINTER(BatchOptions)],
+    "flyg_batch_create": [_vp, _vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), C.POINTER(BatchOptions),
+                          _d, C.POINTER(C.c_int), _d, _u32, C.c_int64, C.POINTER(_vp)],
+    "flyg_batch_destroy": [_vp],
+    "flyg_batch_submit": [_vp, C.c_int, _i64, _d, _d, _d, _i64],
+    "flyg_batch_run": [_vp],
+    "flyg_batch_result": [_vp, C.c_int64, C.POINTER(SearchStats), _d, _d],
+    "flyg_batch_region": [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+}
+_VOID = {"flyg_lbfgs_default_options", "flyg_dimer_default_options", "flyg_rotor_default_options",
+         "flyg_batch_default_options"}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlyError(f"{LIB_PATH} is not built (run openfly_amd/build.py); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            if name in _VOID:
+                f.restype = None
+            elif name == "flyg_ctx_last_error":
+                f.restype = C.c_char_p
+            elif name == "flyg_ctx_stream":
+                f.restype = _vp
+            else:
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def _raise(ctx_h, rc):
+    msg = (lib().flyg_ctx_last_error(ctx_h) or b"").decode()
+    if rc == FLYG_SPLINE_OOB:
+        raise SplineOutOfBounds(msg)
+    if rc == FLYG_DEVICE_ERROR:
+        raise DeviceError(msg)
+    raise FlyError(msg)
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, dtype):
+    """(pointer, where, keepalive) of a numpy array or a torch tensor."""
+    if _is_torch(a):
+        if a.is_cuda:
+            return C.c_void_p(a.data_ptr()), DEVICE, a
+        a = a.numpy()
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return C.c_void_p(arr.ctypes.data), HOST, arr
+
+
+class Context:
+    """A device, one HIP stream and the last error (one per host thread)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = lib().flyg_ctx_create(device, C.byref(h))
+        if rc:
+            _raise(None, rc)
+        self.h = h
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_ctx_destroy(self.h)
+            self.h = None
+
+    def check(self, rc):
+        if rc:
+            _raise(self.h, rc)
+
+    def synchronize(self):
+        self.check(lib().flyg_ctx_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return lib().flyg_ctx_stream(self.h)
+
+    def device_info(self):
+        buf = C.create_string_buffer(64)
+        cu = C.c_int()
+        self.check(lib().flyg_ctx_device_info(self.h, buf, 64, C.byref(cu)))
+        return buf.value.decode(), cu.value
+
+
+class Box:
+    """system::Box(basis, periodic): basis columns are the box vectors."""
+
+    def __init__(self, basis, periodic=(True, True, True)):
+        self.basis = np.ascontiguousarray(basis, dtype=np.float64).reshape(3, 3)
+        self.periodic = np.ascontiguousarray(np.asarray(periodic, dtype=np.int32).reshape(3))
+
+    def args(self):
+        return self.basis.ctypes.data_as(_d), self.periodic.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class EAM:
+    """potential::DataEAM(file) + potential::EAM: setfl tables resident on the device."""
+
+    def __init__(self, ctx: Context, path):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(lib().flyg_eam_load(ctx.h, os.fsencode(str(path)), C.byref(h)))
+        self.h = h
+        nt = C.c_int()
+        rc = C.c_double()
+        ctx.check(lib().flyg_eam_info(h, C.byref(nt), C.byref(rc), None, None))
+        self.ntypes = nt.value
+        self._r_cut = rc.value
+        self.masses = np.zeros(self.ntypes)
+        names = C.create_string_buffer(16 * self.ntypes)
+        ctx.check(lib().flyg_eam_info(h, None, None, self.masses.ctypes.data_as(_d), names))
+        self.names = [names.raw[16 * i:16 * (i + 1)].split(b"\0")[0].decode() for i in range(self.ntypes)]
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_eam_destroy(self.h)
+            self.h = None
+
+    def r_cut(self):
+        return self._r_cut
+
+    def set_precision(self, precision):
+        self.ctx.check(lib().flyg_eam_set_precision(self.h, precision))
+
+    def table(self, kind, a, b, x, deriv=0):
+        x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+        out = np.zeros_like(x)
+        k = {"F": 0, "phi": 1, "V": 2}[kind]
+        self.ctx.check(lib().flyg_eam_table_eval(self.h, k, a, b, x.ctypes.data_as(_d), x.size, deriv,
+                                                 out.ctypes.data_as(_d)))
+        return out
+
+    def energy(self, types, frozen, nl: "List"):
+        tp, w1, k1 = _ptr(types, np.uint32)
+        fp, w2, k2 = _ptr(frozen, np.uint8)
+        if w1 != w2:
+            raise FlyError("types and frozen must both be host or both be device arrays")
+        E = C.c_double()
+        self.ctx.check(lib().flyg_eam_energy(self.h, nl.h, tp, fp, nl.n, w1, C.byref(E)))
+        return E.value
+
+    def gradient(self, types, frozen, nl: "List", out=None):
+        tp, w1, k1 = _ptr(types, np.uint32)
+        fp, w2, k2 = _ptr(frozen, np.uint8)
+        if out is None:
+            out = np.zeros((nl.n, 3))
+        op, w3, k3 = _ptr(out, np.float64)
+        if not (w1 == w2 == w3):
+            raise FlyError("all arrays must be host or all device")
+        self.ctx.check(lib().flyg_eam_gradient(self.h, nl.h, tp, fp, nl.n, w1, op))
+        return out if w3 == DEVICE else k3.reshape(-1, 3)
+
+    def hvp(self, types, frozen, nl: "List", v, out=None):
+        tp, w1, k1 = _ptr(types, np.uint32)
+        fp, w2, k2 = _ptr(frozen, np.uint8)
+        vp, w3, k3 = _ptr(v, np.float64)
+        if out is None:
+            out = np.zeros((nl.n, 3))
+        op, w4, k4 = _ptr(out, np.float64)
+        if not (w1 == w2 == w3 == w4):
+            raise FlyError("all arrays must be host or all device")
+        self.ctx.check(lib().flyg_eam_hvp(self.h, nl.h, tp, fp, nl.n, vp, op, w1))
+        return out if w4 == DEVICE else k4.reshape(-1, 3)
+
+
+DataEAM = EAM
+
+
+class List:
+    """neigh::List(box, r_cut)."""
+
+    def __init__(self, ctx: Context, box: Box, r_cut: float):
+        self.ctx = ctx
+        self.box = box
+        self.r_cut = float(r_cut)
+        h = _vp()
+        b, p = box.args()
+        ctx.check(lib().flyg_list_create(ctx.h, b, p, self.r_cut, C.byref(h)))
+        self.h = h
+        self.n = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_list_destroy(self.h)
+            self.h = None
+
+    def rebuild(self, positions, num_threads=1):
+        p, w, keep = _ptr(positions, np.float64)
+        n = (positions.numel() if _is_torch(positions) else np.size(positions)) // 3
+        self.ctx.check(lib().flyg_list_rebuild(self.h, p, n, w))
+        self.n = n
+
+    def update(self, delta):
+        p, w, keep = _ptr(delta, np.float64)
+        self.ctx.check(lib().flyg_list_update(self.h, p, self.n, w))
+
+    def size(self):
+        return self.n
+
+    def num_ghosts(self):
+        g = C.c_int64()
+        self.ctx.check(lib().flyg_list_num_ghosts(self.h, C.byref(g)))
+        return g.value
+
+    def pairs(self, r_cut=None):
+        """(offsets, j, shift, dr) of every neighbour with r < r_cut (List::for_neighbours)."""
+        rc = self.r_cut if r_cut is None else float(r_cut)
+        m = C.c_int64()
+        self.ctx.check(lib().flyg_list_count(self.h, rc, C.byref(m)))
+        off = np.zeros(self.n + 1, np.int64)
+        j = np.zeros(max(m.value, 1), np.int32)
+        s = np.zeros(max(m.value, 1), np.int32)
+        dr = np.zeros((max(m.value, 1), 3))
+        self.ctx.check(lib().flyg_list_export(self.h, rc, off.ctypes.data_as(_i64), j.ctypes.data_as(_i32),
+                                              s.ctypes.data_as(_i32), dr.ctypes.data_as(_d)))
+        return off, j[:m.value], s[:m.value], dr[:m.value]
+
+    def for_neighbours(self, i, r_cut, f):
+        off, j, s, dr = self.pairs(r_cut)
+        for k in range(off[i], off[i + 1]):
+            f(int(j[k]), float(np.sqrt(dr[k] @ dr[k])), dr[k])
+
+    def positions(self):
+        out = np.zeros((self.n, 3))
+        self.ctx.check(lib().flyg_list_positions(self.h, C.c_void_p(out.ctypes.data), HOST))
+        return out
+
+
+class StepLBFGS:
+    """minimise::StepLBFGS(n)."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx = ctx
+        h = _vp()
+        ctx.check(lib().flyg_steplbfgs_create(ctx.h, n, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_steplbfgs_destroy(self.h)
+            self.h = None
+
+    def clear(self):
+        self.ctx.check(lib().flyg_steplbfgs_clear(self.h))
+
+    def newton_step(self, x, g):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        g = np.ascontiguousarray(g, dtype=np.float64).ravel()
+        q = np.zeros_like(x)
+        self.ctx.check(lib().flyg_steplbfgs_newton_step(self.h, C.c_void_p(x.ctypes.data), C.c_void_p(g.ctypes.data),
+                                                        x.size, C.c_void_p(q.ctypes.data), HOST))
+        return q
+
+
+def _fill(struct, defaults_fn, over):
+    o = struct()
+    defaults_fn(C.byref(o))
+    for k, v in (over or {}).items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def lbfgs_options(**kw):
+    return _fill(LBFGSOptions, lib().flyg_lbfgs_default_options, kw)
+
+
+def dimer_options(**kw):
+    return _fill(DimerOptions, lib().flyg_dimer_default_options, kw)
+
+
+def rotor_options(**kw):
+    return _fill(RotorOptions, lib().flyg_rotor_default_options, kw)
+
+
+def batch_options(**kw):
+    return _fill(BatchOptions, lib().flyg_batch_default_options, kw)
+
+
+class LBFGS:
+    """minimise::LBFGS(Options, Box)."""
+
+    def __init__(self, ctx: Context, box: Box, **opts):
+        self.ctx = ctx
+        self.opt = lbfgs_options(**opts)
+        h = _vp()
+        b, p = box.args()
+        ctx.check(lib().flyg_lbfgs_create(ctx.h, C.byref(self.opt), b, p, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_lbfgs_destroy(self.h)
+            self.h = None
+
+    def minimise(self, positions, types, frozen, pot: EAM):
+        """Returns (positions, gradient, failed, iterations); failed mirrors the reference's bool."""
+        pos = np.array(positions, dtype=np.float64, copy=True).reshape(-1, 3)
+        n = len(pos)
+        grad = np.zeros_like(pos)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        f = np.ascontiguousarray(frozen, dtype=np.uint8)
+        failed = C.c_int()
+        it = C.c_int()
+        self.ctx.check(lib().flyg_lbfgs_minimise(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(grad.ctypes.data),
+                                                 C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n, HOST,
+                                                 C.byref(failed), C.byref(it)))
+        return pos, grad, bool(failed.value), it.value
+
+
+class Dimer:
+    """saddle::Dimer(Options, Rotor::Options, Box)."""
+
+    def __init__(self, ctx: Context, box: Box, dimer=None, rotor=None):
+        self.ctx = ctx
+        self.opt = dimer_options(**(dimer or {}))
+        self.ropt = rotor_options(**(rotor or {}))
+        h = _vp()
+        b, p = box.args()
+        ctx.check(lib().flyg_dimer_create(ctx.h, C.byref(self.opt), C.byref(self.ropt), b, p, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_dimer_destroy(self.h)
+            self.h = None
+
+    def find_sp(self, positions, axis, types, frozen, in_min, pot: EAM, hist_sp=(), theta_tol=0.0,
+                count_frozen=None):
+        pos = np.array(positions, dtype=np.float64, copy=True).reshape(-1, 3)
+        ax = np.array(axis, dtype=np.float64, copy=True).reshape(-1, 3)
+        n = len(pos)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        f = np.ascontiguousarray(frozen, dtype=np.uint8)
+        mn = np.ascontiguousarray(in_min, dtype=np.float64)
+        hs = np.ascontiguousarray(np.stack([np.ravel(h) for h in hist_sp]) if len(hist_sp) else np.zeros(1))
+        if count_frozen is None:
+            count_frozen = int(f.sum())
+        st = SearchStats()
+        self.ctx.check(lib().flyg_dimer_find_sp(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(ax.ctypes.data),
+                                                C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n,
+                                                C.c_void_p(mn.ctypes.data), C.c_void_p(hs.ctypes.data), len(hist_sp),
+                                                float(theta_tol), int(count_frozen), HOST, C.byref(st)))
+        out = st.as_dict()
+        out.update(pos=pos, axis=ax, exit=st.exit_code)
+        return out

@@ -1,0 +1,244 @@
+"""Synthetic inputs of the saddle-search workloads: bcc lattices with defects and tabulated
+Finnis-Sinclair potentials written as setfl (eam.fs) files.
+
+Recorded choices (they fix the synthetic potential; the setfl file is the single source of truth read by
+both the GPU path and the CPU oracle):
+  * pair term      V(r)   = D [exp(-2 a (r - r0)) - 2 exp(-a (r - r0))] S(r)   (Morse, a = 1.4 / A)
+  * density        rho(r) = exp(-2 (r - rd)) S(r), with rd = 2.48 A for Fe and rd = r0 for species X
+  * embedding      F(rho) = -sqrt(rho)
+  * smooth cutoff  S(r) = 1 - 10 t^3 + 15 t^4 - 6 t^5, t = (r - (rc - 0.5)) / 0.5 on [rc - 0.5, rc], 0 beyond
+                   (C2 quintic taper, applied to V and to rho), rc = 5.3 A
+  * grids          nr = nrho = 5000 knots, dr = rc / (nr - 1), drho = 200 / (nrho - 1)
+  * Fe: D = 0.4 eV, r0 = 2.5 A.  X (second species): D = 0.1 eV, r0 = 1.8 A.  Cross pair Fe-X uses X's
+    Morse parameters; the density an atom receives depends on the neighbour's species only.
+The setfl layout follows what the reference's DataEAM reads (data.cpp:67-153): three comment lines,
+species line, "Nrho drho Nr dr rcut", per species "Z mass a lattice" + F + n density tables, then r*V
+for j <= i, five values per line.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+A_FE = 2.855
+R_CUT = 5.3
+
+
+def smooth_cutoff(r, rc=R_CUT, width=0.5):
+    r = np.asarray(r, dtype=np.float64)
+    t = np.clip((r - (rc - width)) / width, 0.0, 1.0)
+    s = 1.0 - 10.0 * t**3 + 15.0 * t**4 - 6.0 * t**5
+    return np.where(r >= rc, 0.0, s)
+
+
+def morse(r, D, r0, alpha=1.4):
+    return D * (np.exp(-2.0 * alpha * (r - r0)) - 2.0 * np.exp(-alpha * (r - r0)))
+
+
+SPECIES = {
+    "Fe": dict(Z=26, mass=55.847, D=0.4, r0=2.5, rd=2.48, lattice=A_FE),
+    "X": dict(Z=1, mass=1.008, D=0.1, r0=1.8, rd=1.8, lattice=A_FE),
+}
+
+
+def _fmt_block(vals):
+    lines = []
+    for k in range(0, len(vals), 5):
+        lines.append(" ".join(f"{v: .16e}" for v in vals[k:k + 5]))
+    return "\n".join(lines)
+
+
+def write_setfl(path, species=("Fe",), nr=5000, nrho=5000, rc=R_CUT, rho_max=200.0):
+    """Write the synthetic Finnis-Sinclair potential for `species` to `path` (setfl, eam/fs flavour)."""
+    dr = rc / (nr - 1)
+    drho = rho_max / (nrho - 1)
+    r = np.arange(nr) * dr
+    rho = np.arange(nrho) * drho
+    S = smooth_cutoff(r, rc)
+    n = len(species)
+    out = [
+        "synthetic Finnis-Sinclair EAM: Morse pair x C2 taper, rho = exp(-2(r - rd)) x taper, F = -sqrt(rho)",
+        f"species {' '.join(species)}; rc = {rc}; nr = {nr}; nrho = {nrho}",
+        "written by openfly_amd.systems.write_setfl",
+        f"{n} " + " ".join(species),
+        f"{nrho} {drho:.16e} {nr} {dr:.16e} {rc:.16e}",
+    ]
+    for sp in species:
+        p = SPECIES[sp]
+        out.append(f"{p['Z']} {p['mass']} {p['lattice']} bcc")
+        out.append(_fmt_block(-np.sqrt(rho)))
+        for _ in species:   # density received from a neighbour of this species
+            out.append(_fmt_block(np.exp(-2.0 * (r - p["rd"])) * S))
+    for i, a in enumerate(species):
+        for j in range(i + 1):
+            b = species[j]
+            pa, pb = SPECIES[a], SPECIES[b]
+            # Cross pairs use the parameters of the later (non-Fe) species.
+            p = pa if (a != "Fe") else pb
+            V = morse(r, p["D"], p["r0"]) * S
+            out.append(_fmt_block(r * V))
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        f.write("\n".join(out) + "\n")
+    os.replace(tmp, path)
+    return path
+
+
+def potential_path(species=("Fe",), directory=None):
+    """Path of the synthetic potential for `species`, written on first use."""
+    directory = directory or os.path.join(os.path.dirname(os.path.abspath(__file__)), "_potentials")
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, "fs_" + "_".join(species) + ".eam.fs")
+    if not os.path.exists(path):
+        write_setfl(path, species)
+    return path
+
+
+# ---------------------------------------------------------------------------------------------------
+# Lattices
+# ---------------------------------------------------------------------------------------------------
+
+def bcc_sites(a, cells):
+    """bcc sites of a cells^3 supercell; x index fastest, motif innermost (z, y, x, motif order)."""
+    c = np.arange(cells, dtype=np.float64)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    base = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    pos = np.empty((base.shape[0], 2, 3))
+    pos[:, 0] = base
+    pos[:, 1] = base + 0.5
+    return pos.reshape(-1, 3) * a
+
+
+def tetra_sites(a, cells):
+    """The 12 tetrahedral interstitial sites per bcc cell, a*(1/2, 1/4, 0) and permutations."""
+    offs = []
+    for k in range(3):   # the cube face normal to axis k (coordinate k = 0)
+        o = [i for i in range(3) if i != k]
+        for u, v in ((0.5, 0.25), (0.5, 0.75), (0.25, 0.5), (0.75, 0.5)):
+            x = np.zeros(3)
+            x[o[0]] = u
+            x[o[1]] = v
+            offs.append(x)
+    offs = np.array(offs)
+    c = np.arange(cells, dtype=np.float64)
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    base = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    return ((base[:, None, :] + offs[None, :, :]).reshape(-1, 3)) * a
+
+
+def cubic_box(L):
+    return np.diag([L, L, L]).astype(np.float64)
+
+
+class System:
+    """Positions (n,3), types (n,) uint32, frozen (n,) uint8 and a box basis (3x3, columns = vectors)."""
+
+    def __init__(self, pos, types, basis, periodic=(1, 1, 1), species=("Fe",), note=""):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float64)
+        self.types = np.ascontiguousarray(types, dtype=np.uint32)
+        self.frozen = np.zeros(len(self.pos), np.uint8)
+        self.basis = np.ascontiguousarray(basis, dtype=np.float64)
+        self.periodic = np.asarray(periodic, dtype=np.int32)
+        self.species = tuple(species)
+        self.note = note
+
+    @property
+    def n(self):
+        return len(self.pos)
+
+    def min_image(self, d):
+        L = np.diag(self.basis)
+        return d - L * np.round(d / L)
+
+
+def config1(seed=1):
+    """bcc Fe a=2.855 A, 16^3 cells (8192 sites) minus one random site, Gaussian sigma=0.05 A noise."""
+    rng = np.random.default_rng(seed)
+    pos = bcc_sites(A_FE, 16)
+    vac = int(rng.integers(len(pos)))
+    pos = np.delete(pos, vac, axis=0)
+    pos = pos + rng.normal(0.0, 0.05, pos.shape)
+    s = System(pos, np.zeros(len(pos)), cubic_box(16 * A_FE), note=f"vacancy site {vac}")
+    s.vacancy = bcc_sites(A_FE, 16)[vac]
+    return s
+
+
+def config2(seed=2, cells=30):
+    """bcc Fe 30^3 cells (54000 sites), 1% vacancies, species X at 2 per 100 Fe on random tetrahedral
+    sites, thermal Gaussian sigma=0.08 A noise."""
+    rng = np.random.default_rng(seed)
+    sites = bcc_sites(A_FE, cells)
+    nvac = int(round(0.01 * len(sites)))
+    keep = np.ones(len(sites), bool)
+    keep[rng.choice(len(sites), nvac, replace=False)] = False
+    fe = sites[keep]
+    tet = tetra_sites(A_FE, cells)
+    nx = int(round(0.02 * len(fe)))
+    xs = tet[rng.choice(len(tet), nx, replace=False)]
+    pos = np.concatenate([fe, xs])
+    types = np.concatenate([np.zeros(len(fe)), np.ones(len(xs))])
+    pos = pos + rng.normal(0.0, 0.08, pos.shape)
+    return System(pos, types, cubic_box(cells * A_FE), species=("Fe", "X"),
+                  note=f"{nvac} vacancies, {nx} X interstitials")
+
+
+def defect_system(cells, n_vac, n_int, seed, clusters=False):
+    """bcc Fe cells^3 with n_vac vacancies and n_int Fe interstitials on tetrahedral sites."""
+    rng = np.random.default_rng(seed)
+    sites = bcc_sites(A_FE, cells)
+    keep = np.ones(len(sites), bool)
+    if clusters:
+        L = cells * A_FE
+        removed = 0
+        while removed < n_vac:
+            size = int(min(rng.integers(1, 7), n_vac - removed))
+            c = sites[rng.integers(len(sites))]
+            d = sites - c
+            d -= L * np.round(d / L)
+            order = np.argsort(np.einsum("ij,ij->i", d, d))
+            picked = [k for k in order if keep[k]][:size]
+            keep[picked] = False
+            removed += len(picked)
+    else:
+        keep[rng.choice(len(sites), n_vac, replace=False)] = False
+    vac_sites = sites[~keep]
+    fe = sites[keep]
+    tet = tetra_sites(A_FE, cells)
+    ints = tet[rng.choice(len(tet), n_int, replace=False)] if n_int else np.zeros((0, 3))
+    pos = np.concatenate([fe, ints])
+    s = System(pos, np.zeros(len(pos)), cubic_box(cells * A_FE), note=f"{n_vac} vacancies, {n_int} interstitials")
+    s.vacancies = vac_sites
+    s.interstitial_ids = np.arange(len(fe), len(pos))
+    return s
+
+
+def config3(seed=3):
+    """bcc Fe 20^3 cells (16000 sites) with 20 vacancies and 40 interstitials."""
+    return defect_system(20, 20, 40, seed)
+
+
+def config5(seed=5):
+    """bcc Fe 80^3 cells (1,024,000 sites), 0.1% vacancies in clusters of 1-6, 0.5% interstitials."""
+    n_sites = 2 * 80**3
+    return defect_system(80, int(round(0.001 * n_sites)), int(round(0.005 * n_sites)), seed, clusters=True)
+
+
+def defect_adjacent_atoms(sys: System, r=3.0):
+    """Atoms within r of a vacancy site or of an interstitial (the centres of saddle searches)."""
+    L = np.diag(sys.basis)
+    pts = []
+    if getattr(sys, "vacancies", None) is not None and len(sys.vacancies):
+        pts.append(np.asarray(sys.vacancies))
+    if getattr(sys, "interstitial_ids", None) is not None and len(sys.interstitial_ids):
+        pts.append(sys.pos[sys.interstitial_ids])
+    if not pts:
+        return np.arange(sys.n)
+    pts = np.concatenate(pts)
+    out = set()
+    for p in pts:
+        d = sys.pos - p
+        d -= L * np.round(d / L)
+        out.update(np.nonzero(np.einsum("ij,ij->i", d, d) < r * r)[0].tolist())
+    return np.array(sorted(out), dtype=np.int64)

@@ -1,0 +1,184 @@
+"""GPU EAM energy / gradient / HVP vs the reference's known answers and the FP64 oracle.
+
+Tolerances: FP64 path within 1e-10 relative of the oracle (energies; gradients relative to the largest
+component); mixed precision (FP32 pair arithmetic, FP64 accumulation) within 2e-6 relative on forces.
+"""
+import numpy as np
+import pytest
+
+import openfly_amd as fa
+from openfly_amd import systems
+from oracle import pyoracle as po
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wen_gpu(ctx):
+    return fa.EAM(ctx, H.WEN)
+
+
+@pytest.fixture(scope="module")
+def wen_orc():
+    return po.DataEAM(H.WEN)
+
+
+def gpu_eval(ctx, pot, sys_pos, basis, types, frozen, rc=None, periodic=(1, 1, 1)):
+    nl = fa.List(ctx, fa.Box(basis, periodic), rc or pot.r_cut())
+    nl.rebuild(sys_pos)
+    return nl, pot.energy(types, frozen, nl), pot.gradient(types, frozen, nl)
+
+
+def orc_eval(data, sys_pos, basis, types, frozen, rc=None, periodic=(1, 1, 1)):
+    nl = po.List(po.Box(basis, periodic), rc or data.r_cut)
+    nl.rebuild(sys_pos, 4)
+    return nl, po.energy(data, nl, types, frozen, 4), po.gradient(data, nl, types, frozen, 4)
+
+
+def test_parser_kat(wen_gpu):
+    assert wen_gpu.ntypes == 2 and wen_gpu.names == ["Fe", "H"]
+    F = wen_gpu.table("F", 0, 0, [0.0, 1.9999999999999999e-02])
+    assert abs(F[0] - H.KAT_F0[0]) < 1e-14 and abs(F[1] - H.KAT_F0[1]) < 1e-14
+
+
+def test_tables_match_oracle(wen_gpu, wen_orc):
+    rng = np.random.default_rng(0)
+    r = rng.uniform(0, 5.3, 2000)
+    rho = rng.uniform(0, 150, 2000)
+    for kind, a, b, x in (("F", 0, 0, rho), ("F", 1, 1, rho), ("phi", 0, 1, r), ("phi", 1, 0, r), ("V", 1, 0, r),
+                          ("V", 0, 0, r)):
+        for d in (0, 1, 2):
+            g = wen_gpu.table(kind, a, b, x, d)
+            o = wen_orc.eval(kind, a, b, x, d)
+            assert np.allclose(g, o, rtol=1e-12, atol=1e-12 * np.abs(o).max()), (kind, a, b, d)
+
+
+def test_energy_gradient_kat(ctx, wen_gpu):
+    pos, basis = H.kat_lattice()
+    n = len(pos)
+    t, f = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
+    print(f"case KAT 7^3 n={n}", flush=True)
+    _, E, g = gpu_eval(ctx, wen_gpu, pos, basis, t, f)
+    assert abs(E - H.KAT_E0) <= 1e-10 * abs(H.KAT_E0)
+    assert np.linalg.norm(g[0] - H.KAT_G0) < 1e-10
+
+
+def test_hvp_kat_block(ctx, wen_gpu):
+    # H e_k for atom 0's three unit vectors reproduces the reference's H(0,0) block.
+    pos, basis = H.kat_lattice()
+    n = len(pos)
+    t, f = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut())
+    nl.rebuild(pos)
+    for k in range(3):
+        v = np.zeros((n, 3))
+        v[0, k] = 1.0
+        hv = wen_gpu.hvp(t, f, nl, v)
+        assert np.linalg.norm(hv[0] - H.KAT_H00[:, k]) < 1e-10
+
+
+@pytest.mark.parametrize("frozen_every", [0, 4])
+def test_two_species_vs_oracle(ctx, wen_gpu, wen_orc, frozen_every):
+    rng = np.random.default_rng(9)
+    pos, basis = H.kat_lattice()
+    pos = pos + rng.normal(0, 0.06, pos.shape)
+    n = len(pos)
+    t = (rng.random(n) < 0.15).astype(np.uint32)
+    f = np.zeros(n, np.uint8)
+    if frozen_every:
+        f[::frozen_every] = 1
+    _, E, g = gpu_eval(ctx, wen_gpu, pos, basis, t, f)
+    _, Eo, go = orc_eval(wen_orc, pos, basis, t, f)
+    assert abs(E - Eo) <= 1e-10 * abs(Eo)
+    assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+    assert np.all(g[f == 1] == 0.0)
+
+
+def test_hvp_vs_oracle_dense_hessian(ctx, wen_gpu, wen_orc):
+    rng = np.random.default_rng(21)
+    pos, basis = H.kat_lattice()
+    pos = pos + rng.normal(0, 0.05, pos.shape)
+    n = len(pos)
+    t = (rng.random(n) < 0.1).astype(np.uint32)
+    f = np.zeros(n, np.uint8)
+    f[rng.random(n) < 0.2] = 1
+    v = rng.normal(size=(n, 3))
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut())
+    nl.rebuild(pos)
+    hv = wen_gpu.hvp(t, f, nl, v)
+    onl = po.List(po.Box(basis), wen_orc.r_cut)
+    onl.rebuild(pos, 4)
+    ho = po.hvp(wen_orc, onl, t, f, v, 4)
+    assert np.max(np.abs(hv - ho)) <= 1e-10 * np.max(np.abs(ho))
+
+
+def test_synthetic_fs_config1_vs_oracle(ctx):
+    path = systems.potential_path(("Fe",))
+    pot = fa.EAM(ctx, path)
+    data = po.DataEAM(path)
+    s = systems.config1(seed=1)
+    print(f"case config1 n={s.n}", flush=True)
+    rc = 5.3 + 2e-3 + (1.1 ** (1 / 3) - 1) * (5.3 + 2e-3)
+    _, E, g = gpu_eval(ctx, pot, s.pos, s.basis, s.types, s.frozen, rc)
+    _, Eo, go = orc_eval(data, s.pos, s.basis, s.types, s.frozen, rc)
+    assert abs(E - Eo) <= 1e-10 * abs(Eo)
+    assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+    # Mixed precision: FP32 pair arithmetic, FP64 accumulation.
+    pot.set_precision(fa.MIXED)
+    nl = fa.List(ctx, fa.Box(s.basis), rc)
+    nl.rebuild(s.pos)
+    gm = pot.gradient(s.types, s.frozen, nl)
+    err = np.max(np.abs(gm - go)) / np.max(np.abs(go))
+    print(f"mixed precision max rel force error {err:.3e}")
+    assert err < 2e-6
+    assert pot.energy(s.types, s.frozen, nl) == E   # energies stay FP64
+    pot.set_precision(fa.FP64)
+
+
+def test_config2_two_species_scale(ctx):
+    path = systems.potential_path(("Fe", "X"))
+    pot = fa.EAM(ctx, path)
+    data = po.DataEAM(path)
+    s = systems.config2(seed=2, cells=16)
+    print(f"case config2(16^3) n={s.n}", flush=True)
+    _, E, g = gpu_eval(ctx, pot, s.pos, s.basis, s.types, s.frozen)
+    _, Eo, go = orc_eval(data, s.pos, s.basis, s.types, s.frozen)
+    assert abs(E - Eo) <= 1e-10 * abs(Eo)
+    assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+    assert np.linalg.norm(g.sum(axis=0)) < 1e-8 * np.abs(g).max() * s.n
+
+
+def test_gradient_deterministic(ctx):
+    pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
+    s = systems.config1(seed=4)
+    nl = fa.List(ctx, fa.Box(s.basis), 5.3)
+    nl.rebuild(s.pos)
+    g1 = pot.gradient(s.types, s.frozen, nl)
+    nl.rebuild(s.pos)
+    g2 = pot.gradient(s.types, s.frozen, nl)
+    assert np.array_equal(g1, g2)
+
+
+def test_spline_out_of_bounds_raises(ctx, wen_gpu):
+    # 40 atoms within 2 A: the density exceeds the F table (rho > (Nrho + 4) drho ~ 200), which must raise
+    # Spline::OutOfBounds (spline.hpp:141-143); phi_Fe(0.05 A) ~ 185 in wen.eam.fs.
+    basis = np.diag([30.0, 30.0, 30.0])
+    pos = np.array([[10.0, 10.0, 10.0]] + [[10.0 + 0.05 * k, 10.0, 10.0] for k in range(1, 40)])
+    n = len(pos)
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut())
+    nl.rebuild(pos)
+    with pytest.raises(fa.SplineOutOfBounds):
+        wen_gpu.gradient(np.zeros(n, np.uint32), np.zeros(n, np.uint8), nl)
+    # The context stays usable after the error.
+    pos2, basis2 = H.kat_lattice()
+    _, E, _ = gpu_eval(ctx, wen_gpu, pos2, basis2, np.zeros(len(pos2), np.uint32), np.zeros(len(pos2), np.uint8))
+    assert abs(E - H.KAT_E0) <= 1e-10 * abs(H.KAT_E0)
+
+
+def test_list_cutoff_smaller_than_potential_raises(ctx, wen_gpu):
+    pos, basis = H.kat_lattice()
+    nl = fa.List(ctx, fa.Box(basis), 5.0)
+    nl.rebuild(pos)
+    with pytest.raises(fa.FlyError):
+        wen_gpu.energy(np.zeros(len(pos), np.uint32), np.zeros(len(pos), np.uint8), nl)
