@@ -75,6 +75,8 @@ int flyg_ctx_device_info(flyg_ctx ctx, char* name, size_t len, int* compute_unit
  * uploads FP64 and FP32 coefficient tables.
  * ------------------------------------------------------------------------------------------------- */
 int flyg_eam_load(flyg_ctx ctx, const char* path, flyg_eam* out);
+/* The same from the file's text (for DataEAM(Options, std::ifstream)). */
+int flyg_eam_load_text(flyg_ctx ctx, const char* text, size_t len, flyg_eam* out);
 int flyg_eam_destroy(flyg_eam eam);
 /* potential::EAM::r_cut() (eam.hpp:70) / DataEAM::type_map(): names is ntypes*16 chars (NUL padded). */
 int flyg_eam_info(flyg_eam eam, int* ntypes, double* r_cut, double* masses, char* names);

@@ -1,0 +1,656 @@
+// Header-only C++ drop-in for the reference's saddle-search hot-path API (namespaces, class names and
+// member signatures of fly::neigh::List, fly::potential::{DataEAM, EAM, Generic},
+// fly::minimise::{StepLBFGS, LBFGS}, fly::saddle::{Rotor, Dimer} and the SoA/property/Box types they
+// take), implemented over the C-ABI of libflygpu (include/flygpu.h).  No Eigen: the small value types
+// (Vec, Mat, Arr) and the SoA container are re-implemented for the subset the hot path uses.
+//
+// Errors: status codes from the C-ABI are re-thrown as fly::RuntimeError or fly::Spline::OutOfBounds,
+// so reference-style callers (Master::find_batch's retry on OutOfBounds, find.cpp:706-723) behave the
+// same.  The `threads` arguments are accepted and ignored (the work runs on the GPU).
+// Each host thread uses its own device context (the reference keeps per-thread state in Master's
+// ThreadData, find.hpp:310-317).
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <fstream>
+#include <iterator>
+#include <memory>
+#include <optional>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../../flygpu.h"
+
+namespace fly {
+
+  // ---- errors (core.hpp:89-171, spline.hpp:46-51) ------------------------------------------------
+  struct RuntimeError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+  };
+
+  struct Spline {
+    struct OutOfBounds : RuntimeError {
+      using RuntimeError::RuntimeError;
+    };
+  };
+
+  namespace gpu {
+    // Per-thread device context.
+    inline flyg_ctx context() {
+      struct Holder {
+        flyg_ctx h = nullptr;
+        ~Holder() {
+          if (h) flyg_ctx_destroy(h);
+        }
+      };
+      thread_local Holder holder;
+      if (!holder.h) {
+        int dev = 0;
+        if (char const* e = std::getenv("FLYG_DEVICE")) dev = std::atoi(e);
+        if (flyg_ctx_create(dev, &holder.h) != FLYG_OK) throw RuntimeError(flyg_ctx_last_error(nullptr));
+      }
+      return holder.h;
+    }
+
+    inline void check(int rc, flyg_ctx ctx = nullptr) {
+      if (rc == FLYG_OK) return;
+      std::string msg = flyg_ctx_last_error(ctx ? ctx : context());
+      if (rc == FLYG_SPLINE_OOB) throw Spline::OutOfBounds(msg);
+      throw RuntimeError(msg);
+    }
+  }  // namespace gpu
+
+  // ---- small value types ---------------------------------------------------------------------------
+  constexpr int spatial_dims = 3;
+
+  struct Vec {
+    std::array<double, 3> v{0, 0, 0};
+    Vec() = default;
+    Vec(double x, double y, double z) : v{x, y, z} {}
+    static Vec Zero() { return {}; }
+    static Vec Constant(double c) { return {c, c, c}; }
+    double& operator[](int i) { return v[static_cast<std::size_t>(i)]; }
+    double operator[](int i) const { return v[static_cast<std::size_t>(i)]; }
+    double& operator()(int i) { return v[static_cast<std::size_t>(i)]; }
+    double operator()(int i) const { return v[static_cast<std::size_t>(i)]; }
+    friend Vec operator+(Vec a, Vec const& b) {
+      for (int k = 0; k < 3; ++k) a[k] += b[k];
+      return a;
+    }
+    friend Vec operator-(Vec a, Vec const& b) {
+      for (int k = 0; k < 3; ++k) a[k] -= b[k];
+      return a;
+    }
+    friend Vec operator*(double s, Vec a) {
+      for (int k = 0; k < 3; ++k) a[k] *= s;
+      return a;
+    }
+    friend Vec operator*(Vec a, double s) { return s * a; }
+  };
+
+  inline double gdot(Vec const& a, Vec const& b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+  inline double gnorm_sq(Vec const& a) { return gdot(a, a); }
+  inline double gnorm(Vec const& a) { return std::sqrt(gnorm_sq(a)); }
+
+  struct Mat {   // 3x3, m(r, c); columns are basis vectors when used as a box basis
+    double m[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    static Mat Zero() { return {}; }
+    double& operator()(int r, int c) { return m[r][c]; }
+    double operator()(int r, int c) const { return m[r][c]; }
+  };
+
+  template <typename T>
+  struct Arr {
+    std::array<T, 3> a{};
+    Arr() = default;
+    Arr(T x, T y, T z) : a{x, y, z} {}
+    static Arr Constant(T c) { return {c, c, c}; }
+    T& operator[](int i) { return a[static_cast<std::size_t>(i)]; }
+    T operator[](int i) const { return a[static_cast<std::size_t>(i)]; }
+  };
+
+  // ---- properties (property.hpp:53-312) ------------------------------------------------------------
+  namespace system {
+    template <typename Scalar, int Rows = 1>
+    struct Property {
+      using scalar_t = Scalar;
+      static constexpr int size() { return Rows; }
+    };
+  }  // namespace system
+
+  struct Position : system::Property<double, 3> {};
+  struct Axis : system::Property<double, 3> {};
+  struct PotentialGradient : system::Property<double, 3> {};
+  struct Delta : system::Property<double, 3> {};
+  struct TypeID : system::Property<std::uint32_t> {};
+  struct Frozen : system::Property<bool> {};
+  struct Index : system::Property<std::int64_t> {};
+
+  inline constexpr Position r_{};
+  inline constexpr Axis ax_{};
+  inline constexpr PotentialGradient g_{};
+  inline constexpr Delta del_{};
+  inline constexpr TypeID id_{};
+  inline constexpr Frozen fzn_{};
+
+  namespace system {
+
+    // ---- SoA (SoA.hpp:74-347): one contiguous array per property, 3-vectors interleaved per atom.
+    // Owning (SoA<Position>) and reference (SoA<Position&>, SoA<Position const&>) flavours share their
+    // storage when converted, as the reference's views alias the caller's arrays.
+    namespace detail {
+      template <typename P>
+      using base_t = std::remove_cv_t<std::remove_reference_t<P>>;
+
+      template <typename Tag>
+      struct Slot {
+        using elem = std::conditional_t<std::is_same_v<typename Tag::scalar_t, bool>, std::uint8_t, typename Tag::scalar_t>;
+        std::shared_ptr<std::vector<elem>> data = std::make_shared<std::vector<elem>>();
+      };
+
+      template <typename Tag, typename... Ps>
+      inline constexpr bool has = (std::is_same_v<Tag, base_t<Ps>> || ...);
+    }  // namespace detail
+
+    // A 3-vector stored inside an SoA array.
+    struct VecRef {
+      double* p;
+      operator Vec() const { return {p[0], p[1], p[2]}; }
+      VecRef& operator=(Vec const& v) {
+        p[0] = v[0];
+        p[1] = v[1];
+        p[2] = v[2];
+        return *this;
+      }
+      VecRef& operator+=(Vec const& v) {
+        for (int k = 0; k < 3; ++k) p[k] += v[k];
+        return *this;
+      }
+      VecRef& operator-=(Vec const& v) {
+        for (int k = 0; k < 3; ++k) p[k] -= v[k];
+        return *this;
+      }
+      double& operator[](int i) const { return p[i]; }
+    };
+
+    // Flat view of one property array (the reference's array_ref_t).
+    template <typename T>
+    struct FlatView {
+      T* p;
+      std::size_t n;
+      T* data() const { return p; }
+      std::size_t size() const { return n; }
+      T& operator[](std::size_t i) const { return p[i]; }
+    };
+
+    template <typename... Ps>
+    class SoA : private detail::Slot<detail::base_t<Ps>>... {
+      template <typename... Q>
+      friend class SoA;
+
+      template <typename Tag>
+      detail::Slot<Tag>& slot() {
+        return static_cast<detail::Slot<Tag>&>(*this);
+      }
+      template <typename Tag>
+      detail::Slot<Tag> const& slot() const {
+        return static_cast<detail::Slot<Tag> const&>(*this);
+      }
+
+      std::int64_t m_size = 0;
+
+    public:
+      explicit SoA(std::int64_t n = 0) { destructive_resize(n); }
+
+      // View / copy-free conversion from any SoA that holds every property of this one.
+      template <typename... Q, typename = std::enable_if_t<(detail::has<detail::base_t<Ps>, Q...> && ...)>>
+      SoA(SoA<Q...> const& other) : m_size(other.size()) {
+        ((slot<detail::base_t<Ps>>().data = other.template slot<detail::base_t<Ps>>().data), ...);
+      }
+
+      std::int64_t size() const { return m_size; }
+
+      void destructive_resize(std::int64_t n) {
+        m_size = n;
+        ((slot<detail::base_t<Ps>>().data->resize(static_cast<std::size_t>(n * detail::base_t<Ps>::size()))), ...);
+      }
+
+      // Point property Tag at other's storage (SoA::rebind, SoA.hpp).
+      template <typename Tag, typename... Q>
+      void rebind(Tag, SoA<Q...> const& other) {
+        slot<Tag>().data = other.template slot<Tag>().data;
+      }
+
+      template <typename Tag>
+      decltype(auto) operator()(Tag, std::int64_t i) const {
+        auto* base = slot<Tag>().data->data();
+        if constexpr (Tag::size() == 3) {
+          return VecRef{base + 3 * i};
+        } else {
+          return (base[i]);
+        }
+      }
+
+      template <typename Tag>
+      auto operator[](Tag) const {
+        auto& v = *slot<Tag>().data;
+        using E = typename detail::Slot<Tag>::elem;
+        return FlatView<E>{v.data(), v.size()};
+      }
+    };
+
+    // ---- TypeMap (typemap.hpp:67-214), names only ------------------------------------------------
+    template <typename... Extra>
+    class TypeMap {
+    public:
+      explicit TypeMap(int n = 0) : m_names(static_cast<std::size_t>(n)) {}
+      int num_types() const { return static_cast<int>(m_names.size()); }
+      void set(int i, std::string name) { m_names.at(static_cast<std::size_t>(i)) = std::move(name); }
+      std::string const& get(int i) const { return m_names.at(static_cast<std::size_t>(i)); }
+
+    private:
+      std::vector<std::string> m_names;
+    };
+
+    // ---- Box (box.hpp:103-120) -------------------------------------------------------------------
+    class Box {
+    public:
+      Box(Mat const& basis, Arr<bool> const& periodic) {
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) m_basis[3 * r + c] = basis(r, c);
+        for (int k = 0; k < 3; ++k) m_periodic[k] = periodic[k] ? 1 : 0;
+      }
+      double const* basis_data() const { return m_basis; }
+      int const* periodic_data() const { return m_periodic; }
+      bool periodic(int i) const { return m_periodic[i] != 0; }
+      Mat basis() const {
+        Mat m;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) m(r, c) = m_basis[3 * r + c];
+        return m;
+      }
+
+    private:
+      double m_basis[9];
+      int m_periodic[3];
+    };
+
+    // ---- Hessian (hessian.hpp:42-148): dense 3N x 3N, filled column by column from HVPs ----------
+    class Hessian {
+    public:
+      void zero_for(std::int64_t n) {
+        m_n = n;
+        m_h.assign(static_cast<std::size_t>(9 * n * n), 0.0);
+      }
+      // Block (i, j) entry (r, c).
+      double operator()(std::int64_t i, std::int64_t j, int r, int c) const {
+        return m_h[static_cast<std::size_t>((3 * i + r) * 3 * m_n + 3 * j + c)];
+      }
+      double* data() { return m_h.data(); }
+      std::int64_t size() const { return m_n; }
+
+    private:
+      std::int64_t m_n = 0;
+      std::vector<double> m_h;
+    };
+
+  }  // namespace system
+
+  // ---- neigh::List (list.hpp:71-210) -------------------------------------------------------------
+  namespace neigh {
+    class List {
+    public:
+      List(system::Box const& box, double r_cut) : m_ctx(gpu::context()), m_r_cut(r_cut) {
+        gpu::check(flyg_list_create(m_ctx, box.basis_data(), box.periodic_data(), r_cut, &m_h), m_ctx);
+      }
+      List(List const&) = delete;
+      List& operator=(List const&) = delete;
+      List(List&& o) noexcept { swap(o); }
+      List& operator=(List&& o) noexcept {
+        swap(o);
+        return *this;
+      }
+      ~List() {
+        if (m_h) flyg_list_destroy(m_h);
+      }
+
+      auto rebuild(system::SoA<Position const&> positions, int = 1) -> void {
+        m_n = positions.size();
+        gpu::check(flyg_list_rebuild(m_h, positions[r_].data(), m_n, FLYG_HOST), m_ctx);
+        m_cache_rc = -1;
+      }
+
+      auto update(system::SoA<Delta const&> x) -> void {
+        if (x.size() != m_n) throw RuntimeError("Input to update has the wrong number of atoms");
+        gpu::check(flyg_list_update(m_h, x[del_].data(), m_n, FLYG_HOST), m_ctx);
+        m_cache_rc = -1;
+      }
+
+      // Calls f(n, r, dr) for every neighbour n of atom i within r_cut (list.hpp:138-154).
+      template <typename F>
+      auto for_neighbours(std::int64_t i, double r_cut, F&& f) const -> void {
+        if (r_cut > m_r_cut) throw RuntimeError("r_cut is bigger than the internal r_cut");
+        refresh(r_cut);
+        for (std::int64_t k = m_off[static_cast<std::size_t>(i)]; k < m_off[static_cast<std::size_t>(i) + 1]; ++k) {
+          Vec dr{m_dr[3 * k], m_dr[3 * k + 1], m_dr[3 * k + 2]};
+          f(static_cast<std::int64_t>(m_j[static_cast<std::size_t>(k)]), std::sqrt(gnorm_sq(dr)), dr);
+        }
+      }
+
+      auto size() const noexcept -> std::int64_t { return m_n; }
+      double r_cut() const { return m_r_cut; }
+      flyg_list handle() const { return m_h; }
+
+    private:
+      flyg_ctx m_ctx = nullptr;
+      flyg_list m_h = nullptr;
+      double m_r_cut = 0;
+      std::int64_t m_n = 0;
+      mutable double m_cache_rc = -1;
+      mutable std::vector<std::int64_t> m_off;
+      mutable std::vector<std::int32_t> m_j, m_s;
+      mutable std::vector<double> m_dr;
+
+      void swap(List& o) {
+        std::swap(m_ctx, o.m_ctx);
+        std::swap(m_h, o.m_h);
+        std::swap(m_r_cut, o.m_r_cut);
+        std::swap(m_n, o.m_n);
+        std::swap(m_cache_rc, o.m_cache_rc);
+        m_off.swap(o.m_off);
+        m_j.swap(o.m_j);
+        m_s.swap(o.m_s);
+        m_dr.swap(o.m_dr);
+      }
+
+      void refresh(double rc) const {
+        if (m_cache_rc == rc) return;
+        std::int64_t np = 0;
+        gpu::check(flyg_list_count(m_h, rc, &np), m_ctx);
+        m_off.resize(static_cast<std::size_t>(m_n + 1));
+        m_j.resize(static_cast<std::size_t>(np + 1));
+        m_s.resize(static_cast<std::size_t>(np + 1));
+        m_dr.resize(static_cast<std::size_t>(3 * np + 3));
+        gpu::check(flyg_list_export(m_h, rc, m_off.data(), m_j.data(), m_s.data(), m_dr.data()), m_ctx);
+        m_cache_rc = rc;
+      }
+    };
+  }  // namespace neigh
+
+  // ---- potentials (data.hpp, eam.hpp, generic.hpp) ------------------------------------------------
+  namespace potential {
+
+    class DataEAM {
+    public:
+      struct Options {
+        bool debug = false;
+        bool symmetric = false;
+      };
+
+      // DataEAM(Options, std::ifstream) (data.hpp:54): the file is parsed and tabulated on the device.
+      DataEAM(Options const&, std::ifstream in) : m_ctx(gpu::context()) {
+        if (!in.good()) throw RuntimeError("Could not open eam in");
+        std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        gpu::check(flyg_eam_load_text(m_ctx, text.data(), text.size(), &m_h), m_ctx);
+        int nt = 0;
+        gpu::check(flyg_eam_info(m_h, &nt, &m_rcut, nullptr, nullptr), m_ctx);
+        std::vector<char> names(static_cast<std::size_t>(16 * nt));
+        gpu::check(flyg_eam_info(m_h, nullptr, nullptr, nullptr, names.data()), m_ctx);
+        m_map = system::TypeMap<>(nt);
+        for (int i = 0; i < nt; ++i) m_map.set(i, std::string(names.data() + 16 * i));
+      }
+      DataEAM(DataEAM const&) = delete;
+      ~DataEAM() {
+        if (m_h) flyg_eam_destroy(m_h);
+      }
+
+      auto r_cut() const noexcept -> double { return m_rcut; }
+      auto type_map() const noexcept -> system::TypeMap<> const& { return m_map; }
+      flyg_eam handle() const { return m_h; }
+
+    private:
+      flyg_ctx m_ctx;
+      flyg_eam m_h = nullptr;
+      double m_rcut = 0;
+      system::TypeMap<> m_map;
+    };
+
+    class EAM {
+    public:
+      EAM(system::TypeMap<> const& map, std::shared_ptr<DataEAM const> data) : m_data(std::move(data)) {
+        if (map.num_types() < m_data->type_map().num_types()) throw RuntimeError("Not enough types in eam data file");
+        for (int i = 0; i < m_data->type_map().num_types(); ++i)
+          if (map.get(i) != m_data->type_map().get(i)) throw RuntimeError("TypeID mismatch");
+      }
+
+      auto r_cut() const noexcept -> double { return m_data->r_cut(); }
+
+      auto energy(system::SoA<TypeID const&, Frozen const&> in, neigh::List const& nl, int = 1) -> double {
+        double E = 0;
+        gpu::check(flyg_eam_energy(m_data->handle(), nl.handle(), in[id_].data(), in[fzn_].data(), in.size(), FLYG_HOST, &E));
+        return E;
+      }
+
+      auto gradient(system::SoA<PotentialGradient&> out, system::SoA<TypeID const&, Frozen const&> in,
+                    neigh::List const& nl, int = 1) -> void {
+        if (in.size() != out.size()) throw RuntimeError("EAM gradient size mismatch");
+        gpu::check(flyg_eam_gradient(m_data->handle(), nl.handle(), in[id_].data(), in[fzn_].data(), in.size(),
+                                     FLYG_HOST, out[g_].data()));
+      }
+
+      // Dense Hessian (eam.cpp:122-233), assembled column by column from device Hessian-vector products.
+      auto hessian(system::Hessian& out, system::SoA<TypeID const&, Frozen const&> in, neigh::List const& nl, int = 1)
+          -> void {
+        std::int64_t n = in.size();
+        out.zero_for(n);
+        std::vector<double> v(static_cast<std::size_t>(3 * n), 0.0), hv(static_cast<std::size_t>(3 * n));
+        for (std::int64_t c = 0; c < 3 * n; ++c) {
+          v[static_cast<std::size_t>(c)] = 1.0;
+          gpu::check(flyg_eam_hvp(m_data->handle(), nl.handle(), in[id_].data(), in[fzn_].data(), n, v.data(),
+                                  hv.data(), FLYG_HOST));
+          v[static_cast<std::size_t>(c)] = 0.0;
+          for (std::int64_t r = 0; r < 3 * n; ++r)
+            out.data()[static_cast<std::size_t>(r * 3 * n + c)] = hv[static_cast<std::size_t>(r)];
+        }
+      }
+
+      DataEAM const& data() const { return *m_data; }
+
+    private:
+      std::shared_ptr<DataEAM const> m_data;
+    };
+
+    // potential::Generic (generic.hpp:50-140): the EAM alternative of the reference's variant.
+    class Generic {
+    public:
+      explicit Generic(EAM eam) : m_eam(std::move(eam)) {}
+      auto r_cut() const noexcept -> double { return m_eam.r_cut(); }
+      auto energy(system::SoA<TypeID const&, Frozen const&> in, neigh::List const& nl, int threads = 1) -> double {
+        return m_eam.energy(in, nl, threads);
+      }
+      auto gradient(system::SoA<PotentialGradient&> out, system::SoA<TypeID const&, Frozen const&> in,
+                    neigh::List const& nl, int threads = 1) -> void {
+        m_eam.gradient(out, in, nl, threads);
+      }
+      auto hessian(system::Hessian& out, system::SoA<TypeID const&, Frozen const&> in, neigh::List const& nl,
+                   int threads = 1) -> void {
+        m_eam.hessian(out, in, nl, threads);
+      }
+      flyg_eam handle() const { return m_eam.data().handle(); }
+
+    private:
+      EAM m_eam;
+    };
+
+  }  // namespace potential
+
+  // ---- minimisers (lbfgs_core.hpp, lbfgs.hpp) -----------------------------------------------------
+  namespace minimise {
+
+    class StepLBFGS {
+    public:
+      explicit StepLBFGS(int n) : m_ctx(gpu::context()) { gpu::check(flyg_steplbfgs_create(m_ctx, n, &m_h), m_ctx); }
+      StepLBFGS(StepLBFGS const&) = delete;
+      ~StepLBFGS() {
+        if (m_h) flyg_steplbfgs_destroy(m_h);
+      }
+      auto clear() -> void { gpu::check(flyg_steplbfgs_clear(m_h), m_ctx); }
+
+      template <typename T1, typename T2>
+      auto newton_step(system::SoA<T1 const&> x, system::SoA<T2 const&> g) -> system::SoA<Delta>& {
+        auto xv = x[T1{}];
+        auto gv = g[T2{}];
+        if (xv.size() != gv.size()) throw RuntimeError("LBFGS core inputs size mismatch");
+        m_r.destructive_resize(x.size());
+        gpu::check(flyg_steplbfgs_newton_step(m_h, xv.data(), gv.data(), static_cast<std::int64_t>(xv.size()),
+                                              m_r[del_].data(), FLYG_HOST),
+                   m_ctx);
+        return m_r;
+      }
+
+    private:
+      flyg_ctx m_ctx;
+      flyg_steplbfgs m_h = nullptr;
+      system::SoA<Delta> m_r;
+    };
+
+    class LBFGS {
+    public:
+      struct Options {
+        int n = 10;
+        int iter_max = 2000;
+        double f2norm = 1e-6;
+        double skin_frac = 1.1;
+        double proj_tol = 0;
+        double max_trust = 0.50;
+        double min_trust = 0.01;
+        double grow_trust = 1.5;
+        double shrink_trust = 0.5;
+        bool debug = false;
+      };
+
+      LBFGS(Options const& opt, system::Box const& box) : m_ctx(gpu::context()) {
+        flyg_lbfgs_options o{opt.n, opt.iter_max, opt.f2norm, opt.skin_frac, opt.proj_tol,
+                             opt.max_trust, opt.min_trust, opt.grow_trust, opt.shrink_trust};
+        gpu::check(flyg_lbfgs_create(m_ctx, &o, box.basis_data(), box.periodic_data(), &m_h), m_ctx);
+      }
+      LBFGS(LBFGS const&) = delete;
+      ~LBFGS() {
+        if (m_h) flyg_lbfgs_destroy(m_h);
+      }
+
+      // Returns true on FAILURE (lbfgs.cpp:114).
+      auto minimise(system::SoA<Position&, PotentialGradient&> out, system::SoA<Position const&, TypeID const&, Frozen const&> in,
+                    potential::Generic& pot, int = 1) -> bool {
+        auto o = out[r_];
+        auto i = in[r_];
+        for (std::size_t k = 0; k < o.size(); ++k) o[k] = i[k];
+        int failed = 0, it = 0;
+        gpu::check(flyg_lbfgs_minimise(m_h, pot.handle(), o.data(), out[g_].data(), in[id_].data(), in[fzn_].data(),
+                                       in.size(), FLYG_HOST, &failed, &it),
+                   m_ctx);
+        return failed != 0;
+      }
+
+    private:
+      flyg_ctx m_ctx;
+      flyg_lbfgs m_h = nullptr;
+    };
+  }  // namespace minimise
+
+  // ---- saddle (rotor.hpp, dimer.hpp) -------------------------------------------------------------
+  namespace saddle {
+
+    class Rotor {
+    public:
+      struct Options {
+        int n = 6;
+        int iter_max_rot = 20;
+        double delta_r = 0.001;
+        double theta_tol = 1 * M_PI / 360.;
+        bool relax_in_convex = true;
+        bool debug = false;
+      };
+      explicit Rotor(Options const& opt) : m_opt(opt) {}
+      auto r_cut(potential::Generic const& pot) const noexcept { return pot.r_cut() + m_opt.delta_r * 2; }
+      Options const& options() const { return m_opt; }
+
+    private:
+      Options m_opt;
+    };
+
+    class Dimer {
+    public:
+      struct Options {
+        int n = 10;
+        int max_steps = 1000;
+        int hist_check_freq = 5;
+        double f2norm = 1e-5;
+        double skin_frac = 1.1;
+        double proj_tol = 0;
+        double max_trust = 0.5;
+        double min_trust = 0.05;
+        double grow_trust = 1.5;
+        double shrink_trust = 0.5;
+        int convex_max = 3;
+        bool use_history = true;
+        bool debug = false;
+      };
+
+      enum Exit : int { success = 0, convex = 1, iter_max = 2, collision = 3, uninit = 4 };
+
+      Dimer(Options const& opt, Rotor::Options const& ropt, system::Box const& box) : m_ctx(gpu::context()) {
+        flyg_dimer_options o{opt.n, opt.max_steps, opt.hist_check_freq, opt.f2norm, opt.skin_frac, opt.proj_tol,
+                             opt.max_trust, opt.min_trust, opt.grow_trust, opt.shrink_trust, opt.convex_max,
+                             opt.use_history ? 1 : 0};
+        flyg_rotor_options r{ropt.n, ropt.iter_max_rot, ropt.delta_r, ropt.theta_tol, ropt.relax_in_convex ? 1 : 0};
+        gpu::check(flyg_dimer_create(m_ctx, &o, &r, box.basis_data(), box.periodic_data(), &m_h), m_ctx);
+      }
+      Dimer(Dimer const&) = delete;
+      ~Dimer() {
+        if (m_h) flyg_dimer_destroy(m_h);
+      }
+
+      auto find_sp(system::SoA<Position&, Axis&> out, system::SoA<Position const&, Axis const&, TypeID const&, Frozen const&> in,
+                   system::SoA<Position const&> in_min, potential::Generic& pot,
+                   std::vector<system::SoA<Position>> const& hist_sp, double theta_tol, int count_frozen, int = 1) -> Exit {
+        std::int64_t n = in.size();
+        auto ro = out[r_];
+        auto ao = out[ax_];
+        auto ri = in[r_];
+        auto ai = in[ax_];
+        for (std::size_t k = 0; k < ro.size(); ++k) {
+          ro[k] = ri[k];
+          ao[k] = ai[k];
+        }
+        std::vector<double> hist;
+        for (auto const& sp : hist_sp) {
+          auto v = sp[r_];
+          hist.insert(hist.end(), v.data(), v.data() + v.size());
+        }
+        flyg_search_stats st{};
+        gpu::check(flyg_dimer_find_sp(m_h, pot.handle(), ro.data(), ao.data(), in[id_].data(), in[fzn_].data(), n,
+                                      in_min[r_].data(), hist.empty() ? nullptr : hist.data(),
+                                      static_cast<int>(hist_sp.size()), theta_tol, count_frozen, FLYG_HOST, &st),
+                   m_ctx);
+        m_last = st;
+        return static_cast<Exit>(st.exit_code);
+      }
+
+      flyg_search_stats const& last_stats() const { return m_last; }
+
+    private:
+      flyg_ctx m_ctx;
+      flyg_dimer m_h = nullptr;
+      flyg_search_stats m_last{};
+    };
+
+  }  // namespace saddle
+}  // namespace fly

@@ -2,6 +2,7 @@
 // exceptions and returns a status code; the message is kept on the context.
 #include <cstring>
 #include <memory>
+#include <sstream>
 
 #include "plain.hpp"
 
@@ -14,6 +15,16 @@ namespace flyg {
 using namespace flyg;
 
 void flyg_ctx_s::set_device() const { HIPCHK(hipSetDevice(device)); }
+
+void flyg_ctx_s::release() {
+  if (--refs > 0) return;
+  (void)hipSetDevice(device);
+  (void)hipStreamSynchronize(stream);
+  if (hflags) (void)hipHostFree(hflags);
+  if (hscal) (void)hipHostFree(hscal);
+  (void)hipStreamDestroy(stream);
+  delete this;   // device buffers are freed by their destructors
+}
 
 unsigned flyg_ctx_s::take_flags() {
   HIPCHK(hipMemcpyAsync(hflags, dflags.get(), sizeof(unsigned), hipMemcpyDeviceToHost, stream));
@@ -87,13 +98,7 @@ int flyg_ctx_create(int device, flyg_ctx* out) {
 }
 
 int flyg_ctx_destroy(flyg_ctx ctx) {
-  if (!ctx) return FLYG_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->hflags) (void)hipHostFree(ctx->hflags);
-  if (ctx->hscal) (void)hipHostFree(ctx->hscal);
-  (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  if (ctx) ctx->release();
   return FLYG_OK;
 }
 
@@ -128,15 +133,31 @@ int flyg_eam_load(flyg_ctx ctx, const char* path, flyg_eam* out) {
     e->ctx = ctx;
     e->data = read_setfl(path);
     eam_setup_tables(*e);
+    ctx->retain();
+    *out = e.release();
+  });
+}
+
+int flyg_eam_load_text(flyg_ctx ctx, const char* text, size_t len, flyg_eam* out) {
+  return guard(ctx, [&] {
+    require(ctx && text && out, "null argument");
+    auto e = std::make_unique<flyg_eam_s>();
+    e->ctx = ctx;
+    std::istringstream in(std::string(text, len));
+    e->data = read_setfl_stream(in);
+    eam_setup_tables(*e);
+    ctx->retain();
     *out = e.release();
   });
 }
 
 int flyg_eam_destroy(flyg_eam eam) {
   if (!eam) return FLYG_OK;
-  (void)hipSetDevice(eam->ctx->device);
-  (void)hipStreamSynchronize(eam->ctx->stream);
+  flyg_ctx_s* ctx = eam->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete eam;
+  ctx->release();
   return FLYG_OK;
 }
 
@@ -178,14 +199,17 @@ int flyg_list_create(flyg_ctx ctx, const double basis[9], const int periodic[3],
   return guard(ctx, [&] {
     require(ctx && basis && periodic && out, "null argument");
     *out = make_list(ctx, basis, periodic, r_cut).release();
+    ctx->retain();
   });
 }
 
 int flyg_list_destroy(flyg_list nl) {
   if (!nl) return FLYG_OK;
-  (void)hipSetDevice(nl->ctx->device);
-  (void)hipStreamSynchronize(nl->ctx->stream);
+  flyg_ctx_s* ctx = nl->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete nl;
+  ctx->release();
   return FLYG_OK;
 }
 
@@ -303,15 +327,18 @@ int flyg_steplbfgs_create(flyg_ctx ctx, int m, flyg_steplbfgs* out) {
     auto s = std::make_unique<flyg_steplbfgs_s>();
     s->ctx = ctx;
     s->core = std::make_unique<DevLBFGS>(ctx, m);
+    ctx->retain();
     *out = s.release();
   });
 }
 
 int flyg_steplbfgs_destroy(flyg_steplbfgs s) {
   if (!s) return FLYG_OK;
-  (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
+  flyg_ctx_s* ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete s;
+  ctx->release();
   return FLYG_OK;
 }
 
@@ -360,15 +387,18 @@ int flyg_lbfgs_create(flyg_ctx ctx, const flyg_lbfgs_options* opt, const double 
     m->box = make_box(basis, periodic);
     std::memcpy(m->basis, basis, sizeof(m->basis));
     std::memcpy(m->periodic, periodic, sizeof(m->periodic));
+    ctx->retain();
     *out = m.release();
   });
 }
 
 int flyg_lbfgs_destroy(flyg_lbfgs m) {
   if (!m) return FLYG_OK;
-  (void)hipSetDevice(m->ctx->device);
-  (void)hipStreamSynchronize(m->ctx->stream);
+  flyg_ctx_s* ctx = m->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete m;
+  ctx->release();
   return FLYG_OK;
 }
 
@@ -431,15 +461,18 @@ int flyg_dimer_create(flyg_ctx ctx, const flyg_dimer_options* opt, const flyg_ro
     (void)make_box(basis, periodic);
     std::memcpy(d->basis, basis, sizeof(d->basis));
     std::memcpy(d->periodic, periodic, sizeof(d->periodic));
+    ctx->retain();
     *out = d.release();
   });
 }
 
 int flyg_dimer_destroy(flyg_dimer d) {
   if (!d) return FLYG_OK;
-  (void)hipSetDevice(d->ctx->device);
-  (void)hipStreamSynchronize(d->ctx->stream);
+  flyg_ctx_s* ctx = d->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete d;
+  ctx->release();
   return FLYG_OK;
 }
 

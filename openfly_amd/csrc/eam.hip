@@ -46,7 +46,7 @@ namespace flyg {
       if (act) {
         pi = A.pos[i];
         ti = MULTI ? static_cast<int>(A.types[i]) : 0;
-        int c = A.cnt[i];
+        int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
@@ -87,7 +87,7 @@ namespace flyg {
       if (act) {
         double4 pi = posf[i];
         int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
-        int c = A.cnt[i];
+        int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
@@ -152,7 +152,7 @@ namespace flyg {
         ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         bool fi = A.frozen[i];
         double vix = fi ? 0 : v[3 * i], viy = fi ? 0 : v[3 * i + 1], viz = fi ? 0 : v[3 * i + 2];
-        int c = A.cnt[i];
+        int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
@@ -200,7 +200,7 @@ namespace flyg {
         double2 di = dfp[i];
         int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         double vix = v[3 * i], viy = v[3 * i + 1], viz = v[3 * i + 2];
-        int c = A.cnt[i];
+        int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
@@ -244,6 +244,39 @@ namespace flyg {
         hv[3 * i + 1] = act ? hy : 0.0;
         hv[3 * i + 2] = act ? hz : 0.0;
       }
+      if (e) atomicOr(err, e);
+    }
+
+    // FP64 densities only (base state of batched searches).
+    template <int L, bool MULTI>
+    __global__ __launch_bounds__(NT) void k_rho(EAMTables T, Shifts S, PairArgs A, double* __restrict__ rho_out,
+                                                unsigned* __restrict__ err) {
+      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long i = gid / L;
+      int lane = static_cast<int>(gid % L);
+      bool act = i < A.n;
+      double rho = 0;
+      unsigned e = 0;
+      if (act) {
+        double4 pi = A.pos[i];
+        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int const c = min(A.cnt[i], A.K);
+        unsigned const* row = A.nbr + i * static_cast<long>(A.K);
+        for (int k = lane; k < c; k += L) {
+          unsigned en = row[k];
+          unsigned j = en & NB_IDX_MASK;
+          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double4 pj = A.pos[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < T.rc2) {
+            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            rho += spl64<0>(T, T.phi[tj + T.ntypes * ti], sqrt(r2), e);
+          }
+        }
+      }
+      rho = group_sum<L>(rho);
+      if (act && lane == 0) rho_out[i] = rho;
       if (e) atomicOr(err, e);
     }
 
@@ -396,6 +429,19 @@ namespace flyg {
     } else {
       k_hvp_density<LANES, false><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), err);
       k_hvp_force<LANES, false><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), d_hv, err);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  void eam_density(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* d_types, double* d_rho) {
+    require(nl.built, "List used before rebuild");
+    Shifts S = shifts_of(nl.box);
+    PairArgs A = pair_args(nl, d_types, nullptr);
+    unsigned g = grid_for(nl.n, LANES);
+    if (E.tab.ntypes > 1) {
+      k_rho<LANES, true><<<g, NT, 0, E.ctx->stream>>>(E.tab, S, A, d_rho, E.ctx->dflags.get());
+    } else {
+      k_rho<LANES, false><<<g, NT, 0, E.ctx->stream>>>(E.tab, S, A, d_rho, E.ctx->dflags.get());
     }
     HIPCHK(hipGetLastError());
   }

@@ -1,6 +1,7 @@
 // Internal handle types of libflygpu (host side).
 #pragma once
 
+#include <istream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -19,6 +20,7 @@ namespace flyg {
   };
 
   SetflData read_setfl(std::string const& path);
+  SetflData read_setfl_stream(std::istream& in);
   std::vector<double4> spline_coefficients(std::vector<double> y, double dx);
 
 }  // namespace flyg
@@ -38,8 +40,12 @@ struct flyg_ctx_s {
   unsigned* hflags = nullptr;    // pinned mirror of small results
   double* hscal = nullptr;
 
+  int refs = 1;                  // the context lives until its creator and every handle released it
+
   void set_device() const;
   unsigned take_flags();         // sync, read and clear the error bits
+  void retain() { ++refs; }
+  void release();
 };
 
 struct flyg_eam_s {

@@ -99,15 +99,18 @@ namespace flyg {
       return (dx * dx + dy * dy) + dz * dz;
     }
 
-    __global__ void k_canon(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos,
-                            double4* __restrict__ ent, unsigned* __restrict__ eid) {
+    __global__ void k_canon(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (i >= n) return;
       double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
       canon_image(b, x);
-      double4 p = make_double4(x[0], x[1], x[2], 0.0);
-      pos[i] = p;
-      ent[i] = p;
+      pos[i] = make_double4(x[0], x[1], x[2], 0.0);
+    }
+
+    __global__ void k_reals(double4 const* __restrict__ pos, long n, double4* __restrict__ ent, unsigned* __restrict__ eid) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      ent[i] = pos[i];
       eid[i] = (13u << NB_SHIFT_BITS) | static_cast<unsigned>(i);
     }
 
@@ -262,7 +265,8 @@ namespace flyg {
       double xi[3] = {p.x, p.y, p.z};
       int c = 0;
       long long o = WRITE ? off[i] : 0;
-      for (int k = 0; k < ncnt[i]; ++k) {
+      int const cnt = min(ncnt[i], K);
+      for (int k = 0; k < cnt; ++k) {
         unsigned e = nbr[i * static_cast<long>(K) + k];
         unsigned j = e & NB_IDX_MASK, s = e >> NB_SHIFT_BITS;
         double4 q = pos[j];
@@ -378,10 +382,9 @@ namespace flyg {
     flyg_ctx_s& ctx = *nl.ctx;
     hipStream_t st = ctx.stream;
     nl.n = n;
+    nl.built = false;
     nl.pos.reserve(n);
-    nl.ent.reserve(n);
-    nl.eid.reserve(n);
-    k_canon<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get());
+    k_canon<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get());
 
     // Ghost count, then ghost images.
     unsigned* ctr = ctx.dflags.get() + 1;
@@ -392,8 +395,10 @@ namespace flyg {
     long G = ctx.hflags[0];
     long m = n + G;
     nl.nghost = G;
+    // (DBuf::reserve does not preserve contents: size the entry arrays before anything is written.)
     nl.ent.reserve(m);
     nl.eid.reserve(m);
+    k_reals<<<nblk(n, TPB), TPB, 0, st>>>(nl.pos.get(), n, nl.ent.get(), nl.eid.get());
     HIPCHK(hipMemsetAsync(ctr, 0, sizeof(unsigned), st));
     k_ghosts<true><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), ctr, m);
 
@@ -417,6 +422,7 @@ namespace flyg {
     // Neighbour rows; grow K and redo the last step if any row overflowed.
     if (nl.K == 0) nl.K = 64;
     nl.ncnt.reserve(n);
+    HIPCHK(hipMemsetAsync(nl.ncnt.get(), 0, n * sizeof(int), st));
     for (int attempt = 0; attempt < 2; ++attempt) {
       nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
       unsigned* mx = ctx.dflags.get() + 2;
@@ -427,6 +433,7 @@ namespace flyg {
       HIPCHK(hipStreamSynchronize(st));
       int maxc = static_cast<int>(ctx.hflags[1]);
       if (maxc <= nl.K) break;
+      require(attempt == 0, "neighbour rows overflowed twice");
       nl.K = (maxc + 15) / 16 * 16 + 16;
     }
     HIPCHK(hipGetLastError());

@@ -45,12 +45,12 @@ namespace flyg {
   }
 
   namespace {
-    std::istringstream line_of(std::ifstream& in) {
+    std::istringstream line_of(std::istream& in) {
       std::string s;
       if (!std::getline(in, s)) throw Error(FLYG_INVALID_ARG, "File terminated too soon");
       return std::istringstream(s);
     }
-    std::vector<double> values(std::ifstream& in, std::size_t count) {
+    std::vector<double> values(std::istream& in, std::size_t count) {
       std::vector<double> v;
       v.reserve(count);
       std::istringstream ln;
@@ -67,6 +67,10 @@ namespace flyg {
   SetflData read_setfl(std::string const& path) {
     std::ifstream in(path);
     require(in.good(), "Could not open eam file " + path);
+    return read_setfl_stream(in);
+  }
+
+  SetflData read_setfl_stream(std::istream& in) {
     SetflData D;
     for (int i = 0; i < 3; ++i) line_of(in);
     {

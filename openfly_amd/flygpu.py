@@ -89,6 +89,7 @@ _SIGS = {
     "flyg_ctx_stream": [_vp],
     "flyg_ctx_device_info": [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)],
     "flyg_eam_load": [_vp, C.c_char_p, C.POINTER(_vp)],
+    "flyg_eam_load_text": [_vp, C.c_char_p, C.c_size_t, C.POINTER(_vp)],
     "flyg_eam_destroy": [_vp],
     "flyg_eam_info": [_vp, C.POINTER(C.c_int), _d, _d, C.c_char_p],
     "flyg_eam_set_precision": [_vp, C.c_int],
@@ -478,4 +479,63 @@ class Dimer:
                                                 float(theta_tol), int(count_frozen), HOST, C.byref(st)))
         out = st.as_dict()
         out.update(pos=pos, axis=ax, exit=st.exit_code)
+        return out
+
+
+class Batch:
+    """Batched dimer searches on one resident base system (Master::find_batch on the device)."""
+
+    def __init__(self, ctx: Context, pot: EAM, box: Box, min_pos, types, dimer=None, rotor=None, **batch):
+        self.ctx = ctx
+        self.pot = pot
+        self.opt = dimer_options(**(dimer or {}))
+        self.ropt = rotor_options(**(rotor or {}))
+        self.bopt = batch_options(**batch)
+        self.base = np.ascontiguousarray(min_pos, dtype=np.float64).reshape(-1, 3)
+        self.n = len(self.base)
+        self.types = np.ascontiguousarray(types, dtype=np.uint32)
+        h = _vp()
+        b, p = box.args()
+        ctx.check(lib().flyg_batch_create(ctx.h, pot.h, C.byref(self.opt), C.byref(self.ropt), C.byref(self.bopt), b, p,
+                                          self.base.ctypes.data_as(_d), self.types.ctypes.data_as(_u32), self.n,
+                                          C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_batch_destroy(self.h)
+            self.h = None
+
+    def submit(self, centres, radii, start_pos, start_axis):
+        c = np.ascontiguousarray(np.atleast_1d(centres), dtype=np.int64)
+        r = np.ascontiguousarray(np.atleast_1d(radii), dtype=np.float64)
+        k = len(c)
+        sp = np.ascontiguousarray(start_pos, dtype=np.float64).reshape(k, self.n, 3)
+        sa = np.ascontiguousarray(start_axis, dtype=np.float64).reshape(k, self.n, 3)
+        ids = np.zeros(k, np.int64)
+        self.ctx.check(lib().flyg_batch_submit(self.h, k, c.ctypes.data_as(_i64), r.ctypes.data_as(_d),
+                                               sp.ctypes.data_as(_d), sa.ctypes.data_as(_d), ids.ctypes.data_as(_i64)))
+        return ids
+
+    def run(self):
+        self.ctx.check(lib().flyg_batch_run(self.h))
+
+    def result(self, sid, positions=False):
+        st = SearchStats()
+        pos = np.zeros((self.n, 3)) if positions else None
+        ax = np.zeros((self.n, 3)) if positions else None
+        self.ctx.check(lib().flyg_batch_result(self.h, int(sid), C.byref(st),
+                                               pos.ctypes.data_as(_d) if positions else None,
+                                               ax.ctypes.data_as(_d) if positions else None))
+        out = st.as_dict()
+        out["barrier"] = st.energy
+        out["exit"] = st.exit_code
+        if positions:
+            out["pos"] = pos
+            out["axis"] = ax
+        nm = C.c_int()
+        nr = C.c_int()
+        self.ctx.check(lib().flyg_batch_region(self.h, int(sid), C.byref(nm), C.byref(nr)))
+        out["n_mobile"] = nm.value
+        out["n_region"] = nr.value
         return out
