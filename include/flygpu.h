@@ -189,7 +189,9 @@ typedef struct {
   int64_t grads;   /* gradient evaluations */
   int rebuilds;    /* neighbour-list rebuilds */
   double curv;     /* last curvature along the axis */
-  double energy;   /* FP64 energy at the returned positions (0 when not requested) */
+  double energy;   /* batched searches: FP64 barrier E(saddle) - E(base); plain dimer: 0 */
+  double max_disp; /* batched searches: largest displacement of a mobile atom from its base position */
+  int launches;    /* batched searches: kernel launches used (a search that outgrew its region is re-run) */
 } flyg_search_stats;
 
 void flyg_dimer_default_options(flyg_dimer_options* opt);

@@ -9,8 +9,8 @@
 //   S1 = frozen atoms within r_L = r_cut + skin of a mobile atom (their density, hence F', can change).
 // Neighbour rows are built once from the base positions with cut-off r_L (a Verlet skin): mobile rows
 // hold every region neighbour, S1 rows only the mobile ones, and the frozen part of an S1 density is the
-// base density minus the mobile part.  The rows stay complete while every mobile atom is within skin/2
-// of its base position; a search that leaves that ball is re-run with a larger skin.  Every 3N vector of
+// base density minus the mobile part.  The rows stay complete while d1 + d2 < skin for the two largest
+// mobile displacements from the base; a search that breaks this is re-run with a larger skin.  Every 3N vector of
 // the reference is zero on frozen atoms, so the device keeps only the 3*|M| mobile components.
 // Barrier = E_loc(saddle) - E_loc(base) in FP64, where E_loc collects every term of the EAM energy that
 // depends on mobile positions (sum over M u S1 of F(rho) plus the pairs with a mobile member).
@@ -29,15 +29,17 @@ namespace flyg {
 
     constexpr int EXIT_REGION = 5;
     constexpr int EXIT_OOB = 6;
-    constexpr int NVEC = 48;   // workspace vectors per search (each 3*nm doubles)
+    // Workspace vectors per search (each 3*nm doubles): 10 named vectors, then the translation core
+    // (m s-vectors, m y-vectors, x_prev, g_prev, q) and the rotation core (the same with its own m).
+    inline int nvec(int m, int rm) { return 16 + 2 * (m + rm); }
 
     struct SearchHdr {
       int nm, nr;          // mobile atoms, region atoms (M u S1)
       long reg_off;        // first region atom in the packed region arrays
       long row_off;        // first row pointer (nr + 1 entries)
       long v3_off;         // first element of this search's 3*nm start/out vectors
-      long ws_off;         // first element of this search's workspace (NVEC * 3*nm)
-      double half_skin;    // allowed displacement of a mobile atom from its base position
+      long ws_off;         // first element of this search's workspace (nvec * 3*nm)
+      double skin;         // Verlet skin of the rows (r_L = r_c + skin)
     };
 
     struct SearchOut {
@@ -399,8 +401,11 @@ namespace flyg {
       double* dgp = vecp(7);
       double* axp = vecp(8);
       double* xb = vecp(9);
-      Core<T> tc{vecp(10), vecp(10 + o.n), vecp(30), vecp(31), vecp(32), lds_core, lds_core + 32, o.n, 0, n3};
-      Core<T> rc{vecp(33), vecp(33 + o.rn), vecp(45), vecp(46), vecp(47), lds_core + 64, lds_core + 96, o.rn, 0, n3};
+      int const t0 = 10, r0v = 13 + 2 * o.n;
+      Core<T> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
+                 lds_core, lds_core + 32, o.n, 0, n3};
+      Core<T> rc{vecp(r0v), vecp(r0v + o.rn), vecp(r0v + 2 * o.rn), vecp(r0v + 2 * o.rn + 1), vecp(r0v + 2 * o.rn + 2),
+                 lds_core + 64, lds_core + 96, o.rn, 0, n3};
       double const dr = o.delta_r;
       unsigned err = 0;
       long grads = 0;
@@ -424,21 +429,42 @@ namespace flyg {
       double curv = 0;
       double maxd = 0;
 
+      // Rows built at the base positions with cut-off r_c + skin stay complete while every pair that is
+      // missing from them stays beyond r_c: a mobile-frozen pair needs d_i < skin and a mobile-mobile
+      // pair d_i + d_j < skin, i.e. d1 + d2 < skin for the two largest mobile displacements d1 >= d2.
       auto displacement_ok = [&]() {
-        double m = 0;
+        double m1 = 0, m2 = 0;
         for (int a = tm.tid; a < v.nm; a += T) {
           double dx = x[3 * a] - xb[3 * a], dy = x[3 * a + 1] - xb[3 * a + 1], dz = x[3 * a + 2] - xb[3 * a + 2];
-          m = fmax(m, dx * dx + dy * dy + dz * dz);
+          double d = sqrt(dx * dx + dy * dy + dz * dz);
+          if (!(d <= m1)) {   // also takes NaN, which then fails the test below
+            m2 = m1;
+            m1 = d;
+          } else if (d > m2) {
+            m2 = d;
+          }
         }
-        // max over the workgroup (fixed order)
-        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-        if ((tm.tid & 63) == 0) tm.red[4 * (tm.tid >> 6)] = m;
+        for (int off = 32; off > 0; off >>= 1) {
+          double o1 = __shfl_xor(m1, off, 64), o2 = __shfl_xor(m2, off, 64);
+          double hi = fmax(m1, o1), lo = fmax(fmin(m1, o1), fmax(m2, o2));
+          m1 = hi;
+          m2 = lo;
+        }
+        if ((tm.tid & 63) == 0) {
+          tm.red[4 * (tm.tid >> 6)] = m1;
+          tm.red[4 * (tm.tid >> 6) + 1] = m2;
+        }
         __syncthreads();
-        double mm = 0;
-        for (int w = 0; w < T / 64; ++w) mm = fmax(mm, tm.red[4 * w]);
+        double a1 = 0, a2 = 0;
+        for (int w = 0; w < T / 64; ++w) {
+          double b1 = tm.red[4 * w], b2 = tm.red[4 * w + 1];
+          double hi = fmax(a1, b1), lo = fmax(fmin(a1, b1), fmax(a2, b2));
+          a1 = hi;
+          a2 = lo;
+        }
         __syncthreads();
-        maxd = fmax(maxd, sqrt(mm));
-        return sqrt(mm) <= h.half_skin;
+        maxd = fmax(maxd, a1);
+        return a1 + a2 < h.skin;
       };
 
       // Rotor::eff_gradient (rotor.cpp:49-164); returns false on a spline error.
@@ -680,6 +706,7 @@ namespace flyg {
 
     struct Pending {
       int64_t id;
+      int launches = 0;
       int64_t centre;
       double radius;
       double skin;
@@ -748,8 +775,8 @@ namespace flyg {
       std::vector<int> s1;
       for (int i = 0; i < nm; ++i)
         B.grid.query(&B.base[3 * P.mobile[i]], rL, [&](int j, int, double r2) {
-          if (r2 < rL * rL && local_of[j] < 0) {
-            local_of[j] = -2;   // marked
+          if (r2 < rL * rL && local_of[j] == -1) {
+            local_of[j] = -2;   // collected into S1
             s1.push_back(j);
           }
         });
@@ -860,12 +887,12 @@ namespace flyg {
         hdr[k].row_off = nrow;
         hdr[k].v3_off = n3;
         hdr[k].ws_off = nws;
-        hdr[k].half_skin = 0.5 * work[k]->skin;
+        hdr[k].skin = work[k]->skin;
         nreg += g.nr;
         nrow += g.nr + 1;
         nent += static_cast<long>(g.ent.size());
         n3 += 3L * g.nm;
-        nws += static_cast<long>(NVEC) * 3L * g.nm;
+        nws += static_cast<long>(nvec(dopt.n, dopt.rn)) * 3L * g.nm;
       }
       std::vector<double4> hbase(nreg);
       std::vector<uint8_t> htype(nreg);
@@ -979,9 +1006,10 @@ namespace flyg {
         Pending& P = *work[k];
         SearchOut const& o = outs[k];
         int nm = regions[k].nm;
-        if (o.exit_code == EXIT_REGION && P.skin * 1.5 <= skin_cap) {
+        ++P.launches;
+        if (o.exit_code == EXIT_REGION && P.skin < skin_cap) {
           Pending q = P;
-          q.skin = std::min(skin_cap, std::max(P.skin * 1.5, 2.2 * o.max_disp));
+          q.skin = std::min(skin_cap, std::max(P.skin * 1.5, 2.0 * o.max_disp + 0.5));
           requeue.push_back(std::move(q));
           off += 3L * nm;
           continue;
@@ -994,6 +1022,8 @@ namespace flyg {
         R.st.rebuilds = 0;
         R.st.curv = o.curv;
         R.st.energy = o.barrier;
+        R.st.max_disp = o.max_disp;
+        R.st.launches = P.launches;
         R.mobile = P.mobile;
         R.x.assign(ox.begin() + off, ox.begin() + off + 3L * nm);
         R.ax.assign(oax.begin() + off, oax.begin() + off + 3L * nm);
@@ -1086,8 +1116,8 @@ int flyg_batch_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt,
       flyg_batch_default_options(&b->bopt);
     }
     if (b->bopt.list_skin <= 0) b->bopt.list_skin = 2.0;
-    require(b->opt.n > 0 && b->opt.n <= 20 && b->ropt.n > 0 && b->ropt.n <= 12,
-            "batched searches support L-BFGS histories up to 20 (translation) and 12 (rotation)");
+    require(b->opt.n > 0 && b->opt.n <= 32 && b->ropt.n > 0 && b->ropt.n <= 32,
+            "batched searches support L-BFGS histories up to 32");
     require(b->bopt.precision == FLYG_MIXED || b->bopt.precision == FLYG_FP64, "bad precision");
     std::memcpy(b->basis, basis, sizeof(b->basis));
     std::memcpy(b->periodic, periodic, sizeof(b->periodic));
@@ -1100,8 +1130,13 @@ int flyg_batch_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt,
     for (long i = 0; i < n; ++i) {
       require(types[i] < static_cast<uint32_t>(eam->tab.ntypes), "TypeID out of range");
       for (int k = 0; k < 3; ++k) {
+        // Strictly inside [0, L): the canonical image of a coordinate just below 0 rounds to L, and the
+        // cell grid and the image shifts must agree on which side of the box an atom is.
         double x = min_pos[3 * i + k];
-        b->base[3 * i + k] = x - b->box.ext[k] * std::floor(x * b->box.inv_ext[k]);
+        double w = x - b->box.ext[k] * std::floor(x * b->box.inv_ext[k]);
+        if (w >= b->box.ext[k]) w -= b->box.ext[k];
+        if (w < 0) w = 0;
+        b->base[3 * i + k] = w;
       }
     }
     double rl = eam->tab.rc + b->bopt.list_skin;
@@ -1192,6 +1227,7 @@ int flyg_batch_run(flyg_batch b) {
       auto& R = b->results[P.id];
       R.done = 1;
       R.st.exit_code = EXIT_REGION;
+      R.st.launches = P.launches;
       R.mobile = P.mobile;
     }
   });

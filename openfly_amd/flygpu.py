@@ -70,7 +70,7 @@ class RotorOptions(C.Structure):
 
 class SearchStats(C.Structure):
     _fields_ = [("exit_code", C.c_int), ("steps", C.c_int), ("grads", C.c_int64), ("rebuilds", C.c_int),
-                ("curv", C.c_double), ("energy", C.c_double)]
+                ("curv", C.c_double), ("energy", C.c_double), ("max_disp", C.c_double), ("launches", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -82,7 +82,8 @@ def test_fp64_batch_matches_oracle(ctx, setup):
     for (centre, f, x, ax), r in zip(st, res):
         o = po.find_sp(S["box"], S["data"], x, ax, S["t"], f, S["mn"], threads=8)
         print(f"  centre {centre} mobile {r['n_mobile']} region {r['n_region']}: gpu exit {r['exit']} steps "
-              f"{r['steps']} barrier {r['barrier']:.6f} | oracle exit {o['exit']} steps {o['steps']}")
+              f"{r['steps']} barrier {r['barrier']:.6f} max_disp {r['max_disp']:.3f} launches {r['launches']} | "
+              f"oracle exit {o['exit']} steps {o['steps']} max_disp {np.abs(o['pos'] - S['mn']).max():.3f}")
         assert r["n_mobile"] == int((f == 0).sum())
         assert r["exit"] == o["exit"]
         if o["exit"] == 0:
