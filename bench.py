@@ -56,6 +56,119 @@ def cpu_baseline(sys_, path, target_s=12.0):
                       f"one rebuild, {el:.1f} s, OpenMP static schedule over {threads} threads"}
 
 
+def perturbed_starts(sys_, centres, radii, seed, stddev=0.6, r_pert=4.0):
+    """Master::perturb-style starts (find.cpp:1147-1214): every mobile atom within r_pert of the centre gets
+    a Gaussian kick scaled by (1 - r/r_pert) and a standard-normal axis component, the centre a full kick;
+    the axis is normalised.  Atoms farther than the search radius are frozen."""
+    rng = np.random.default_rng(seed)
+    L = np.diag(sys_.basis)
+    starts = np.repeat(sys_.pos[None], len(centres), axis=0)
+    axes = np.zeros_like(starts)
+    for k, (c, R) in enumerate(zip(centres, radii)):
+        d = sys_.pos - sys_.pos[c]
+        d -= L * np.round(d / L)
+        r = np.sqrt(np.einsum("ij,ij->i", d, d))
+        sd = -1.0
+        while sd < 0:
+            sd = rng.normal(stddev, stddev / 3.0)
+        near = np.nonzero((r < r_pert) & (r <= R))[0]
+        for j in near:
+            scale = 1.0 if j == c else (1.0 - r[j] / r_pert)
+            starts[k, j] += rng.normal(0.0, sd, 3) * scale
+            axes[k, j] += rng.normal(0.0, 1.0, 3)
+        axes[k] /= np.linalg.norm(axes[k])
+    return starts, axes
+
+
+def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
+    """Config 3: 256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials,
+    active radius U(4, 9) A around defect-adjacent atoms, kick 0.6 A, |g| < 1e-3 eV/A."""
+    import torch  # noqa: F401  (device already initialised by the caller)
+    s = systems.config3(3)
+    path = systems.potential_path(("Fe",))
+    pot = fa.EAM(ctx, path)
+    rng = np.random.default_rng(33)
+    cand = systems.defect_adjacent_atoms(s)
+    centres = rng.choice(cand, n_search, replace=True)
+    radii = rng.uniform(4.0, 9.0, n_search)
+    starts, axes = perturbed_starts(s, centres, radii, seed=34)
+    box = fa.Box(s.basis)
+    dimer = dict(f2norm=1e-3)
+    out = {"workload": "bcc Fe 20^3 + 20 vacancies + 40 interstitials (config 3), 256 searches, radius U(4,9) A, "
+                       "kick 0.6 A, |g| < 1e-3 eV/A, start from the unrelaxed defect lattice"}
+    res = {}
+    for name, prec in (("batched_mixed", fa.MIXED), ("batched_fp64", fa.FP64)):
+        b = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
+        ids = b.submit(centres, radii, starts, axes)
+        b.run()   # first run includes the shape autotuner
+        b2 = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
+        ids2 = b2.submit(centres, radii, starts, axes)
+        t0 = time.perf_counter()
+        b2.run()
+        el = time.perf_counter() - t0
+        r = [b2.result(i) for i in ids2]
+        res[name] = r
+        ok = sum(1 for x in r if x["exit"] == 0)
+        out[name] = {"seconds": el, "converged": ok, "searches_per_s": n_search / el,
+                     "converged_per_s": ok / el, "mean_grads": float(np.mean([x["grads"] for x in r])),
+                     "mean_mobile": float(np.mean([x["n_mobile"] for x in r])),
+                     "relaunched": int(sum(1 for x in r if x["launches"] > 1))}
+    # Agreement of the mixed form with the FP64 form from the same starts (the FP64 batched path matches
+    # the FP64 oracle's exits, steps and barriers in tests/test_gpu_batch.py).
+    m, f = res["batched_mixed"], res["batched_fp64"]
+    agree = [abs(a["barrier"] - b["barrier"]) for a, b in zip(m, f) if a["exit"] == 0 and b["exit"] == 0]
+    good = sum(1 for a, b in zip(m, f) if a["exit"] == 0 and b["exit"] == 0 and abs(a["barrier"] - b["barrier"]) <= 1e-3)
+    el = out["batched_mixed"]["seconds"]
+    out["batched_mixed"]["matching_fp64_within_1e-3eV"] = good
+    out["batched_mixed"]["searches_per_s_dE_1e-3"] = good / el
+    out["batched_mixed"]["max_abs_dE_vs_fp64"] = float(max(agree)) if agree else None
+    # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
+    frozen_all = []
+    dim = fa.Dimer(ctx, box, dimer=dimer)
+    L = np.diag(s.basis)
+    t0 = time.perf_counter()
+    plain = []
+    for k in range(plain_sample):
+        d = s.pos - s.pos[centres[k]]
+        d -= L * np.round(d / L)
+        fz = (np.einsum("ij,ij->i", d, d) > radii[k] ** 2).astype(np.uint8)
+        frozen_all.append(fz)
+        plain.append(dim.find_sp(starts[k], axes[k], s.types, fz, s.pos, pot))
+    el = time.perf_counter() - t0
+    out["plain_fp64_gpu"] = {"sample": plain_sample, "seconds": el, "searches_per_s": plain_sample / el,
+                             "exits_match_batched_fp64": int(sum(1 for k in range(plain_sample)
+                                                                 if plain[k]["exit"] == f[k]["exit"]))}
+    return out, (s, path, centres, radii, starts, axes, dimer)
+
+
+def cpu_search_baseline(data_args, cpu_sample=16):
+    """FP64 oracle, one search per thread over all host cores (find.cpp:697-732 with threads = 1)."""
+    from oracle import pyoracle as po
+    s, path, centres, radii, starts, axes, dimer = data_args
+    threads = os.cpu_count() or 1
+    k = min(cpu_sample, threads)
+    data = po.DataEAM(path)
+    L = np.diag(s.basis)
+    # frozen masks differ per search; run them one batch per distinct mask through find_sp_many
+    t0 = time.perf_counter()
+    exits = []
+    import concurrent.futures as cf
+
+    def one(i):
+        d = s.pos - s.pos[centres[i]]
+        d -= L * np.round(d / L)
+        fz = (np.einsum("ij,ij->i", d, d) > radii[i] ** 2).astype(np.uint8)
+        return po.find_sp(po.Box(s.basis), data, starts[i], axes[i], s.types, fz, s.pos, threads=1, dimer=dimer)
+
+    with cf.ThreadPoolExecutor(max_workers=k) as ex:
+        for r in ex.map(one, range(k)):
+            exits.append(r["exit"])
+    el = time.perf_counter() - t0
+    return {"value": k / el, "unit": "searches/s", "cores": k, "kind": "port",
+            "sample": f"FP64 oracle Dimer::find_sp on the first {k} of the same starts, one search per thread, "
+                      f"{el:.1f} s, {sum(1 for e in exits if e == 0)} converged"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +177,7 @@ def main():
     ap.add_argument("--cells", type=int, default=30)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-search", action="store_true", help="skip the batched saddle-search leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box of this build has a "
@@ -182,6 +296,11 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_grad,
         },
     }
+    if not args.no_search:
+        sr, sargs = search_bench(ctx, fa, systems)
+        if not args.no_cpu:
+            sr["cpu_baseline"] = cpu_search_baseline(sargs)
+        out["searches"] = sr
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, path)
     print(json.dumps(out), flush=True)

@@ -82,6 +82,9 @@ int flyg_eam_destroy(flyg_eam eam);
 int flyg_eam_info(flyg_eam eam, int* ntypes, double* r_cut, double* masses, char* names);
 /* FLYG_FP64 (default) or FLYG_MIXED for gradient / HVP pair arithmetic.  Energies are always FP64. */
 int flyg_eam_set_precision(flyg_eam eam, int precision);
+/* Lanes of a wavefront that share one atom's neighbour row in the gradient kernels: 4, 8 or 16, or 0
+ * (default) to time the three once per system size.  Results are bitwise identical for every choice. */
+int flyg_eam_set_lanes(flyg_eam eam, int lanes);
 /* Evaluates one table on the device at m host points: kind 0 = F_a, 1 = phi(a,b), 2 = V(a,b);
  * deriv 0/1/2 = f/f'/f'' (DataEAM::f/phi/v + Spline::f/fp/fpp, data.hpp:73-116, spline.hpp:81-123). */
 int flyg_eam_table_eval(flyg_eam eam, int kind, int a, int b, const double* x, int64_t m, int deriv, double* out);

@@ -23,6 +23,10 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Wall",
+    # Contract a*b+c into FMA only inside one source expression (hipcc's default also fuses across
+    # statements, where the choice depends on unrolling and use counts, so sums would round differently
+    # for different lanes-per-atom / block shapes).
+    "-ffp-contract=on",
     # Packed FP32 VALU ops stay off: the compiler emits them from plain float code otherwise.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

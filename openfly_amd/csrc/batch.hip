@@ -85,7 +85,7 @@ namespace flyg {
 
     template <int T>
     struct Team {
-      double* red;   // LDS, 4 * (T / 64)
+      double* red;   // LDS, 4 * T
       int tid;
 
       __device__ double wave(double v) const {
@@ -94,41 +94,71 @@ namespace flyg {
         return v;
       }
 
-      // Sums up to four values over the workgroup; every thread gets the totals.
+      // Sums up to four values over the workgroup with a tree that does not depend on T: element i of a
+      // vector belongs to virtual slot i mod 512, each slot adds in increasing i, and the 512 slots meet
+      // in the xor tree 256, 128, ..., 1 (levels >= T in registers, 64..T/2 through LDS, < 64 by
+      // shuffles).  v[q][k] is the slot tid + q T of value k; every thread gets the totals.
+      static constexpr int VS = 512;
+      static constexpr int NQ = VS / T;
+
       template <int K>
-      __device__ void sum(double* v) const {
+      __device__ void tree(double (&v)[NQ][4]) const {
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = wave(v[k]);
-        if ((tid & 63) == 0)
+        for (int bit = VS / 2; bit >= T; bit >>= 1)
 #pragma unroll
-          for (int k = 0; k < K; ++k) red[4 * (tid >> 6) + k] = v[k];
-        __syncthreads();
+          for (int q = 0; q < NQ; ++q)
+            if (((q * T) & bit) == 0 && q + bit / T < NQ)
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          double s = 0;
-          for (int w = 0; w < T / 64; ++w) s += red[4 * w + k];
-          v[k] = s;
+              for (int k = 0; k < K; ++k) v[q][k] = __dadd_rn(v[q][k], v[q + bit / T][k]);
+        for (int bit = T / 2; bit >= 64; bit >>= 1) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) red[4 * tid + k] = v[0][k];
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < K; ++k) v[0][k] = __dadd_rn(v[0][k], red[4 * (tid ^ bit) + k]);
+          __syncthreads();
         }
-        __syncthreads();
+#pragma unroll
+        for (int bit = 32; bit >= 1; bit >>= 1)
+#pragma unroll
+          for (int k = 0; k < K; ++k) v[0][k] = __dadd_rn(v[0][k], __shfl_xor(v[0][k], bit, 64));
+      }
+
+      template <int K>
+      __device__ void sum(double* out, double (&v)[NQ][4]) const {
+        tree<K>(v);
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[k] = v[0][k];
       }
 
       __device__ double dot(double const* a, double const* b, int n) const {
-        double v[1] = {0};
-        for (int i = tid; i < n; i += T) v[0] += a[i] * b[i];
-        sum<1>(v);
-        return v[0];
+        double v[NQ][4] = {};
+        for (int i = tid, q = 0; i < n; i += T, q = (q + 1) % NQ) v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
+        double o[1];
+        sum<1>(o, v);
+        return o[0];
       }
 
       __device__ void dot2(double const* a, double const* b, double const* c, double const* d, int n, double& x,
                            double& y) const {
-        double v[2] = {0, 0};
-        for (int i = tid; i < n; i += T) {
-          v[0] += a[i] * b[i];
-          v[1] += c[i] * d[i];
+        double v[NQ][4] = {};
+        for (int i = tid, q = 0; i < n; i += T, q = (q + 1) % NQ) {
+          v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
+          v[q][1] = __dadd_rn(v[q][1], c[i] * d[i]);
         }
-        sum<2>(v);
-        x = v[0];
-        y = v[1];
+        double o[2];
+        sum<2>(o, v);
+        x = o[0];
+        y = o[1];
+      }
+
+      // Workgroup sum of per-thread values that are already in a fixed per-row form (region energy).
+      __device__ double sum1_rows(double x) const {
+        double v[NQ][4] = {};
+        v[0][0] = x;
+        double o[1];
+        sum<1>(o, v);
+        return o[0];
       }
 
       // o = ca a + cb b + cc c (b, c may be null); o may alias a.
@@ -183,9 +213,9 @@ namespace flyg {
       for (int r0 = tm.tid / L; r0 < v.nr; r0 += T / L) {
         double4 pi = v.cur[r0];
         int ti = A.multi ? v.type[r0] : 0;
-        double rho = 0;
+        Acc16<L> rho_acc;
         int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
-        for (int k = b + lane; k < c; k += L) {
+        for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
           double3 sh = A.S[en >> NB_SHIFT_BITS];
@@ -195,13 +225,13 @@ namespace flyg {
           if (r2 < Tb.rc2) {
             int tj = A.multi ? v.type[j] : 0;
             if (MIX) {
-              rho += static_cast<double>(spl32f<0>(Tb, Tb.phi[tj + nt * ti], sqrtf(static_cast<float>(r2)), e));
+              rho_acc.add(m, static_cast<double>(spl32f<0>(Tb, Tb.phi[tj + nt * ti], sqrtf(static_cast<float>(r2)), e)));
             } else {
-              rho += spl64<0>(Tb, Tb.phi[tj + nt * ti], sqrt(r2), e);
+              rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], sqrt(r2), e));
             }
           }
         }
-        rho = group_sum<L>(rho);
+        double rho = rho_acc.total();
         if (lane == 0) {
           if (r0 >= v.nm) rho += v.rhof[r0];
           v.fp[r0] = spl64<1>(Tb, Tb.F[ti], rho, e);
@@ -212,9 +242,9 @@ namespace flyg {
         double4 pi = v.cur[r0];
         double fpi = v.fp[r0];
         int ti = A.multi ? v.type[r0] : 0;
-        double gx = 0, gy = 0, gz = 0;
+        Acc16<L> gx, gy, gz;
         int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
-        for (int k = b + lane; k < c; k += L) {
+        for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
           double3 sh = A.S[en >> NB_SHIFT_BITS];
@@ -239,18 +269,16 @@ namespace flyg {
               double pb = A.multi ? spl64<1>(Tb, Tb.phi[ti + nt * tj], r, e) : pa;
               cf = (vp + fpi * pa + fpj * pb) / r;
             }
-            gx -= cf * dx;
-            gy -= cf * dy;
-            gz -= cf * dz;
+            gx.add(m, -cf * dx);
+            gy.add(m, -cf * dy);
+            gz.add(m, -cf * dz);
           }
         }
-        gx = group_sum<L>(gx);
-        gy = group_sum<L>(gy);
-        gz = group_sum<L>(gz);
+        double tx = gx.total(), ty = gy.total(), tz = gz.total();
         if (lane == 0) {
-          g[3 * r0] = gx;
-          g[3 * r0 + 1] = gy;
-          g[3 * r0 + 2] = gz;
+          g[3 * r0] = tx;
+          g[3 * r0 + 1] = ty;
+          g[3 * r0 + 2] = tz;
         }
       }
       __syncthreads();
@@ -265,14 +293,14 @@ namespace flyg {
       int const nt = Tb.ntypes;
       unsigned e = 0;
       int const lane = tm.tid % L;
-      double acc = 0;
+      // Per-row energies go to the fp scratch (overwritten by the next gradient), then a fixed-order sum.
       for (int r0 = tm.tid / L; r0 < v.nr; r0 += T / L) {
         double4 pi = v.cur[r0];
         int ti = A.multi ? v.type[r0] : 0;
-        double rho = 0, pair = 0;
+        Acc16<L> rho_acc, pair_acc;
         bool mob = r0 < v.nm;
         int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
-        for (int k = b + lane; k < c; k += L) {
+        for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
           double3 sh = A.S[en >> NB_SHIFT_BITS];
@@ -282,12 +310,12 @@ namespace flyg {
           if (r2 < Tb.rc2) {
             double r = sqrt(r2);
             int tj = A.multi ? v.type[j] : 0;
-            rho += spl64<0>(Tb, Tb.phi[tj + nt * ti], r, e);
-            if (mob) pair += (j < v.nm ? 0.5 : 1.0) * spl64<0>(Tb, Tb.V[vix(nt, ti, tj)], r, e);
+            rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], r, e));
+            if (mob) pair_acc.add(m, (j < v.nm ? 0.5 : 1.0) * spl64<0>(Tb, Tb.V[vix(nt, ti, tj)], r, e));
           }
         }
-        rho = group_sum<L>(rho);
-        pair = group_sum<L>(pair);
+        double rho = rho_acc.total();
+        double pair = pair_acc.total();
         if (lane == 0) {
           double rt;
           if (base) {
@@ -296,13 +324,16 @@ namespace flyg {
           } else {
             rt = mob ? rho : rho + v.rhof[r0];
           }
-          acc += spl64<0>(Tb, Tb.F[ti], rt, e) + pair;
+          v.fp[r0] = spl64<0>(Tb, Tb.F[ti], rt, e) + pair;
         }
       }
-      double s[1] = {acc};
-      tm.template sum<1>(s);
+      __syncthreads();
+      double vv[Team<T>::NQ][4] = {};
+      for (int i = tm.tid, q = 0; i < v.nr; i += T, q = (q + 1) % Team<T>::NQ) vv[q][0] = __dadd_rn(vv[q][0], v.fp[i]);
+      double o[1];
+      tm.template sum<1>(o, vv);
       err |= tm.any_or(e);
-      return s[0];
+      return o[0];
     }
 
     // cur[i] = x + c * d for the mobile atoms (d may be null).
@@ -605,7 +636,7 @@ namespace flyg {
 
     template <int T, int L, bool MIX>
     __global__ __launch_bounds__(T) void k_search(BatchArgs A) {
-      __shared__ double red[4 * (T / 64)];
+      __shared__ double red[4 * T];
       __shared__ double lds_core[128];
       __shared__ int s_id;
       Team<T> tm{red, static_cast<int>(threadIdx.x)};

@@ -182,6 +182,13 @@ int flyg_eam_set_precision(flyg_eam eam, int precision) {
   });
 }
 
+int flyg_eam_set_lanes(flyg_eam eam, int lanes) {
+  return guard(eam->ctx, [&] {
+    require(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16, "lanes must be 0, 4, 8 or 16");
+    eam->lanes = lanes;
+  });
+}
+
 int flyg_eam_table_eval(flyg_eam eam, int kind, int a, int b, const double* x, int64_t m, int deriv, double* out) {
   return guard(eam->ctx, [&] {
     require(kind >= 0 && kind <= 2, "kind must be 0 (F), 1 (phi) or 2 (V)");

@@ -198,6 +198,45 @@ namespace flyg {
     return v;
   }
 
+  // Row sums that do not depend on how many lanes share a row: entry k of a row always goes to slot
+  // k mod 16, each slot adds its entries in increasing k, and the 16 slots meet in the fixed xor tree
+  // (8, 4, 2, 1).  A lane of an L-lane group owns slots lane + q L, q < 16 / L; tree levels >= L are
+  // done in registers, the rest with cross-lane shuffles.  Every lane gets the row total.
+  template <int L>
+  struct Acc16 {
+    static_assert(L >= 1 && L <= 16 && (16 % L) == 0, "L must divide 16");
+    static constexpr int NS = 16 / L;
+    double v[NS];
+    __device__ Acc16() {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) v[q] = 0.0;
+    }
+    // m = index of the entry within this lane's stride (k = lane + m L).
+    // (__dadd_rn keeps the accumulation unfused, so every L rounds identically.)
+    __device__ void add(int m, double x) {
+      if (NS == 1) {
+        v[0] = __dadd_rn(v[0], x);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+          if (q == m % NS) v[q] = __dadd_rn(v[q], x);
+      }
+    }
+    __device__ double total() {
+#pragma unroll
+      for (int bit = 8; bit >= 1; bit >>= 1) {
+        if (bit >= L) {
+#pragma unroll
+          for (int q = 0; q < NS; ++q)
+            if (((q * L) & bit) == 0 && q + bit / L < NS) v[q] = __dadd_rn(v[q], v[q + bit / L]);
+        } else {
+          v[0] = __dadd_rn(v[0], __shfl_xor(v[0], bit, 64));
+        }
+      }
+      return v[0];
+    }
+  };
+
   // Deterministic block reduction of one double (fixed tree order); result valid in thread 0.
   template <int NT>
   __device__ inline double block_sum(double v, double* sh) {

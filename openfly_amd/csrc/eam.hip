@@ -17,6 +17,7 @@
 //              dm = (V'' + F'_g phi''_{a->g} + F'_a phi''_{g->a}) s + dF'_g phi'_{a->g} + dF'_a phi'_{g->a}
 //      which is H v for the analytic Hessian of eam.cpp:122-233 (pinned against it in the tests).
 #include <cstring>
+#include <map>
 
 #include "eam_host.hpp"
 
@@ -26,12 +27,42 @@ namespace flyg {
 
     constexpr int NT = 256;
 
+    // Image shift of a packed entry: sum_a s_a * basis column a, s_a in {-1, 0, 1}.  Exact for
+    // orthorhombic boxes (each component is 0 or +-L), computed in registers (no table lookup).
     struct Shifts {
-      double3 v[27];
+      double b[3][3];   // b[row][col]
+      __device__ double3 of(unsigned code) const {
+        int s0 = static_cast<int>(code % 3u) - 1;
+        int s1 = static_cast<int>((code / 3u) % 3u) - 1;
+        int s2 = static_cast<int>(code / 9u) - 1;
+        return make_double3(s0 * b[0][0] + s1 * b[0][1] + s2 * b[0][2], s0 * b[1][0] + s1 * b[1][1] + s2 * b[1][2],
+                            s0 * b[2][0] + s1 * b[2][1] + s2 * b[2][2]);
+      }
     };
 
     __device__ inline int vidx(EAMTables const& T, int a, int b) { return a > b ? a + T.ntypes * b : b + T.ntypes * a; }
 
+    // One neighbour entry: image position difference and squared distance.
+    struct PairGeo {
+      double dx, dy, dz, r2, w;
+      unsigned j;
+    };
+
+    __device__ inline PairGeo geo(Shifts const& S, double4 const* __restrict__ pos, unsigned en, double4 const& pi) {
+      PairGeo p;
+      p.j = en & NB_IDX_MASK;
+      double3 sh = S.of(en >> NB_SHIFT_BITS);
+      double4 pj = pos[p.j];
+      p.dx = (pj.x + sh.x) - pi.x;
+      p.dy = (pj.y + sh.y) - pi.y;
+      p.dz = (pj.z + sh.z) - pi.z;
+      p.r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+      p.w = pj.w;
+      return p;
+    }
+
+    // Density pass.  Each lane walks its share of the row two entries at a time so that two position
+    // gathers (and then two table gathers) are in flight per lane.
     template <int L, bool MULTI, bool ENERGY>
     __global__ __launch_bounds__(NT) void k_density(EAMTables T, Shifts S, PairArgs A, double4* __restrict__ posf,
                                                     double* __restrict__ eatom, unsigned* __restrict__ err) {
@@ -39,7 +70,7 @@ namespace flyg {
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool act = i < A.n;
-      double rho = 0, vs = 0;
+      Acc16<L> rho, vs;
       unsigned e = 0;
       double4 pi = make_double4(0, 0, 0, 0);
       int ti = 0;
@@ -48,28 +79,30 @@ namespace flyg {
         ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
-        for (int k = lane; k < c; k += L) {
-          unsigned en = row[k];
-          unsigned j = en & NB_IDX_MASK;
-          double3 sh = S.v[en >> NB_SHIFT_BITS];
-          double4 pj = A.pos[j];
-          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
-          double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < T.rc2) {
-            double r = sqrt(r2);
-            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
-            rho += spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e);
-            if (ENERGY) vs += spl64<0>(T, T.V[vidx(T, ti, tj)], r, e);
+        auto one = [&](int m, PairGeo const& p) {
+          if (p.r2 < T.rc2) {
+            double r = sqrt(p.r2);
+            int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
+            rho.add(m, spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e));
+            if (ENERGY) vs.add(m, spl64<0>(T, T.V[vidx(T, ti, tj)], r, e));
           }
+        };
+        int m = 0;
+        for (; lane + (m + 1) * L < c; m += 2) {
+          unsigned e0 = row[lane + m * L], e1 = row[lane + (m + 1) * L];
+          PairGeo p0 = geo(S, A.pos, e0, pi), p1 = geo(S, A.pos, e1, pi);
+          one(m, p0);
+          one(m + 1, p1);
         }
+        if (lane + m * L < c) one(m, geo(S, A.pos, row[lane + m * L], pi));
       }
-      rho = group_sum<L>(rho);
-      if (ENERGY) vs = group_sum<L>(vs);
+      double rt = rho.total();
+      double vt = ENERGY ? vs.total() : 0.0;
       if (act && lane == 0) {
         double f, fp;
-        spl64_ffp(T, T.F[ti], rho, f, fp, e);
+        spl64_ffp(T, T.F[ti], rt, f, fp, e);
         posf[i] = make_double4(pi.x, pi.y, pi.z, fp);
-        if (ENERGY) eatom[i] = 0.5 * vs + f;
+        if (ENERGY) eatom[i] = 0.5 * vt + f;
       }
       if (e) atomicOr(err, e);
     }
@@ -82,57 +115,54 @@ namespace flyg {
       int lane = static_cast<int>(gid % L);
       bool inr = i < A.n;
       bool act = inr && !A.frozen[i];
-      double gx = 0, gy = 0, gz = 0;
+      Acc16<L> gx, gy, gz;
       unsigned e = 0;
       if (act) {
         double4 pi = posf[i];
         int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
-        for (int k = lane; k < c; k += L) {
-          unsigned en = row[k];
-          unsigned j = en & NB_IDX_MASK;
-          double3 sh = S.v[en >> NB_SHIFT_BITS];
-          double4 pj = posf[j];
-          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
-          double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+        auto one = [&](int m, PairGeo const& p) {
+          if (p.r2 < T.rc2) {
+            int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
+            double cf;
             if (MIX) {
-              float rf = sqrtf(static_cast<float>(r2));
+              float rf = sqrtf(static_cast<float>(p.r2));
               float vp = spl32f<1>(T, T.V[vidx(T, ti, tj)], rf, e);
               float pa = spl32f<1>(T, T.phi[tj + T.ntypes * ti], rf, e);
               float pb = MULTI ? spl32f<1>(T, T.phi[ti + T.ntypes * tj], rf, e) : pa;
-              float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(pj.w) * pb;
-              double cf = static_cast<double>(mag / rf);
-              gx -= cf * dx;
-              gy -= cf * dy;
-              gz -= cf * dz;
+              float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(p.w) * pb;
+              cf = static_cast<double>(mag / rf);
             } else {
-              double r = sqrt(r2);
+              double r = sqrt(p.r2);
               double vp = spl64<1>(T, T.V[vidx(T, ti, tj)], r, e);
               double pa = spl64<1>(T, T.phi[tj + T.ntypes * ti], r, e);
               double pb = MULTI ? spl64<1>(T, T.phi[ti + T.ntypes * tj], r, e) : pa;
-              double mag = vp + pi.w * pa + pj.w * pb;
-              double cf = mag / r;
-              gx -= cf * dx;
-              gy -= cf * dy;
-              gz -= cf * dz;
+              cf = (vp + pi.w * pa + p.w * pb) / r;
             }
+            gx.add(m, -cf * p.dx);
+            gy.add(m, -cf * p.dy);
+            gz.add(m, -cf * p.dz);
           }
+        };
+        int m = 0;
+        for (; lane + (m + 1) * L < c; m += 2) {
+          unsigned e0 = row[lane + m * L], e1 = row[lane + (m + 1) * L];
+          PairGeo p0 = geo(S, posf, e0, pi), p1 = geo(S, posf, e1, pi);
+          one(m, p0);
+          one(m + 1, p1);
         }
+        if (lane + m * L < c) one(m, geo(S, posf, row[lane + m * L], pi));
       }
-      gx = group_sum<L>(gx);
-      gy = group_sum<L>(gy);
-      gz = group_sum<L>(gz);
+      double tx = gx.total(), ty = gy.total(), tz = gz.total();
       if (L >= 3 && inr && lane < 3) {
-        double v = lane == 0 ? gx : (lane == 1 ? gy : gz);
+        double v = lane == 0 ? tx : (lane == 1 ? ty : tz);
         grad[3 * i + lane] = act ? v : 0.0;
       }
-      if (L < 3 && inr && lane == 0) {   // fewer than 3 lanes per atom: lane 0 writes all components
-        grad[3 * i] = act ? gx : 0.0;
-        grad[3 * i + 1] = act ? gy : 0.0;
-        grad[3 * i + 2] = act ? gz : 0.0;
+      if (L < 3 && inr && lane == 0) {
+        grad[3 * i] = act ? tx : 0.0;
+        grad[3 * i + 1] = act ? ty : 0.0;
+        grad[3 * i + 2] = act ? tz : 0.0;
       }
       if (e) atomicOr(err, e);
     }
@@ -157,7 +187,7 @@ namespace flyg {
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
           unsigned j = en & NB_IDX_MASK;
-          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double3 sh = S.of(en >> NB_SHIFT_BITS);
           double4 pj = A.pos[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
@@ -205,7 +235,7 @@ namespace flyg {
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
           unsigned j = en & NB_IDX_MASK;
-          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double3 sh = S.of(en >> NB_SHIFT_BITS);
           double4 pj = A.pos[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
@@ -265,7 +295,7 @@ namespace flyg {
         for (int k = lane; k < c; k += L) {
           unsigned en = row[k];
           unsigned j = en & NB_IDX_MASK;
-          double3 sh = S.v[en >> NB_SHIFT_BITS];
+          double3 sh = S.of(en >> NB_SHIFT_BITS);
           double4 pj = A.pos[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
@@ -292,7 +322,8 @@ namespace flyg {
 
     Shifts shifts_of(BoxDesc const& b) {
       Shifts s;
-      shift_table(b, s.v);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) s.b[r][c] = b.B[r][c];
       return s;
     }
 
@@ -401,6 +432,55 @@ namespace flyg {
     return E.ctx->hscal[0];
   }
 
+  namespace {
+    void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L) {
+      switch (L) {
+        case 4:
+          launch_density<4>(E, S, A, false);
+          launch_force<4>(E, S, A, d_grad, precision);
+          break;
+        case 16:
+          launch_density<16>(E, S, A, false);
+          launch_force<16>(E, S, A, d_grad, precision);
+          break;
+        default:
+          launch_density<8>(E, S, A, false);
+          launch_force<8>(E, S, A, d_grad, precision);
+      }
+    }
+
+    // Lanes per atom row: the row sums do not depend on it (Acc16), so it is picked by timing.
+    int tuned_lanes(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision) {
+      if (E.lanes > 0) return E.lanes;
+      auto key = std::make_pair(A.n, precision);
+      auto it = E.tuned.find(key);
+      if (it != E.tuned.end()) return it->second;
+      hipStream_t st = E.ctx->stream;
+      hipEvent_t a, b;
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      int best = 8;
+      float best_ms = 1e30f;
+      for (int L : {4, 8, 16}) {
+        gradient_with(E, S, A, d_grad, precision, L);   // warm
+        HIPCHK(hipEventRecord(a, st));
+        for (int r = 0; r < 3; ++r) gradient_with(E, S, A, d_grad, precision, L);
+        HIPCHK(hipEventRecord(b, st));
+        HIPCHK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = L;
+        }
+      }
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      E.tuned[key] = best;
+      return best;
+    }
+  }  // namespace
+
   void eam_gradient(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen, double* d_grad,
                     int precision) {
     require(nl.built, "List used before rebuild");
@@ -408,8 +488,8 @@ namespace flyg {
     E.posf.reserve(nl.n);
     Shifts S = shifts_of(nl.box);
     PairArgs A = pair_args(nl, types, frozen);
-    launch_density<LANES>(E, S, A, false);
-    launch_force<LANES>(E, S, A, d_grad, precision);
+    int L = tuned_lanes(E, S, A, d_grad, precision);
+    gradient_with(E, S, A, d_grad, precision, L);
     HIPCHK(hipGetLastError());
   }
 

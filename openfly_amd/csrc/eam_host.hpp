@@ -2,6 +2,7 @@
 #pragma once
 
 #include <istream>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -64,6 +65,8 @@ struct flyg_eam_s {
   flyg::DBuf<double> vec_in, vec_out;
   std::vector<uint32_t> last_types;
   std::vector<uint8_t> last_frozen;
+  int lanes = 0;                                   // lanes per atom row, 0 = timed choice
+  std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes
 };
 
 struct flyg_list_s {
@@ -86,6 +89,12 @@ struct flyg_list_s {
   flyg::DBuf<int> ncnt;          // entries per atom
   flyg::DBuf<int> scan_tmp;
   flyg::DBuf<double> stage;      // host staging of xyz vectors
+  flyg::DBuf<unsigned> lstate;   // device build counters
+  unsigned* hstate = nullptr;    // pinned mirror
+  long gcap = 0;                 // ghost slots
+  ~flyg_list_s() {
+    if (hstate) (void)hipHostFree(hstate);
+  }
 };
 
 namespace flyg {

@@ -290,6 +290,234 @@ namespace flyg {
       if (!WRITE) counts[i] = c;
     }
 
+    // ---- single-sync rebuild path ----------------------------------------------------------------
+    // lstate[0] ghosts generated, [1] longest row, [2] largest staged cell block, [3] overflow bits.
+    enum : unsigned { OV_GHOST = 1u, OV_STAGE = 2u };
+
+    __global__ void k_clear(int* __restrict__ a, long na, int* __restrict__ b, long nb, unsigned* __restrict__ st) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i < na) a[i] = 0;
+      if (i < nb) b[i] = 0;
+      if (i < 4) st[i] = 0u;
+    }
+
+    __global__ void k_canon_reals(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos,
+                                  double4* __restrict__ ent, unsigned* __restrict__ eid) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+      canon_image(b, x);
+      double4 p = make_double4(x[0], x[1], x[2], 0.0);
+      pos[i] = p;
+      ent[i] = p;
+      eid[i] = (13u << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+    }
+
+    // make_ghosts with a fixed capacity; the counter keeps counting past it so the host can resize.
+    __global__ void k_ghosts_cap(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
+                                 unsigned* __restrict__ eid, unsigned* __restrict__ st, long gcap) {
+      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i >= n) return;
+      double cx[27][3];
+      int cs[27];
+      int m = 1;
+      double4 p = pos[i];
+      cx[0][0] = p.x;
+      cx[0][1] = p.y;
+      cx[0][2] = p.z;
+      cs[0] = 13;
+      int const pow3[3] = {1, 3, 9};
+      for (int ax = 0; ax < 3; ++ax) {
+        if (!b.per[ax]) continue;
+        int const end = m;
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+          for (int c = 0; c < end; ++c) {
+            double im[3] = {cx[c][0], cx[c][1], cx[c][2]};
+            if (gen_image(b, rc, sgn, im, ax)) {
+              cx[m][0] = im[0];
+              cx[m][1] = im[1];
+              cx[m][2] = im[2];
+              cs[m] = cs[c] + sgn * pow3[ax];
+              ++m;
+            }
+          }
+        }
+      }
+      unsigned g = static_cast<unsigned>(m - 1);
+      if (g == 0) return;
+      unsigned base = atomicAdd(&st[0], g);
+      for (int k = 1; k < m; ++k) {
+        long slot = static_cast<long>(base) + (k - 1);
+        if (slot >= gcap) {
+          atomicOr(&st[3], static_cast<unsigned>(OV_GHOST));
+          return;
+        }
+        ent[n + slot] = make_double4(cx[k][0], cx[k][1], cx[k][2], 0.0);
+        eid[n + slot] = (static_cast<unsigned>(cs[k]) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+      }
+    }
+
+    __device__ inline long live_entries(unsigned const* st, long n, long gcap) {
+      long g = static_cast<long>(st[0]);
+      return n + (g < gcap ? g : gcap);
+    }
+
+    __global__ void k_cell_count_cap(GridDesc g, double4 const* __restrict__ ent, long n, long gcap,
+                                     unsigned const* __restrict__ st, int* __restrict__ ecell, int* __restrict__ cnt) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= live_entries(st, n, gcap)) return;
+      double4 p = ent[e];
+      double x[3] = {p.x, p.y, p.z};
+      int c3[3];
+      int c = cell_idx(g, x, c3);
+      ecell[e] = c;
+      atomicAdd(&cnt[c], 1);
+    }
+
+    __global__ void k_scatter_cap(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
+                                  int const* __restrict__ ecell, long n, long gcap, unsigned const* __restrict__ st,
+                                  int const* __restrict__ cstart, int* __restrict__ cursor, double4* __restrict__ spos,
+                                  unsigned* __restrict__ sid) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= live_entries(st, n, gcap)) return;
+      int c = ecell[e];
+      int slot = cstart[c] + atomicAdd(&cursor[c], 1);
+      spos[slot] = ent[e];
+      sid[slot] = eid[e];
+    }
+
+    // List::build_neigh_list for all real atoms of one cell: the 3x3x3 block of cells around it is
+    // staged in LDS in z-major cell order, each cell sorted by packed id (ranks computed from the ids,
+    // so the rows do not depend on the atomic scatter order), then one thread per real atom of the
+    // cell tests every staged candidate.  Rows come out in the same order as cell-by-cell scanning.
+    constexpr int STAGE_CAP = 1024;
+    constexpr int BUILD_T = 128;
+
+    __global__ __launch_bounds__(BUILD_T) void k_build_lds(GridDesc g, double4 const* __restrict__ spos,
+                                                           unsigned const* __restrict__ sid,
+                                                           int const* __restrict__ cstart, unsigned* __restrict__ nbr,
+                                                           int* __restrict__ ncnt, int K, unsigned* __restrict__ st) {
+      __shared__ double4 lpos[STAGE_CAP];
+      __shared__ unsigned lid[STAGE_CAP];
+      __shared__ int cb[27], ce[27], lb[28];
+      __shared__ int ncell, own_slot;
+      int const c = blockIdx.x;
+      int const own_b = cstart[c], own_e = cstart[c + 1];
+      if (own_b == own_e) return;   // uniform: empty cell
+      int const tid = threadIdx.x;
+      if (tid == 0) {
+        int cz = c / g.prod[2], rem = c - cz * g.prod[2];
+        int cy = rem / g.prod[1], cx = rem - cy * g.prod[1];
+        int k = 0, tot = 0;
+        for (int z = max(cz - 1, 0); z < min(cz + 2, g.shape[2]); ++z)
+          for (int y = max(cy - 1, 0); y < min(cy + 2, g.shape[1]); ++y)
+            for (int x = max(cx - 1, 0); x < min(cx + 2, g.shape[0]); ++x) {
+              int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
+              if (cc == c) own_slot = k;
+              cb[k] = cstart[cc];
+              ce[k] = cstart[cc + 1];
+              lb[k] = tot;
+              tot += ce[k] - cb[k];
+              ++k;
+            }
+        lb[k] = tot;
+        ncell = k;
+        if (tot > STAGE_CAP) {
+          atomicOr(&st[3], static_cast<unsigned>(OV_STAGE));
+          atomicMax(&st[2], static_cast<unsigned>(tot));
+        }
+      }
+      __syncthreads();
+      int const total = lb[ncell];
+      if (total > STAGE_CAP) return;   // uniform: the host falls back to the unstaged build
+      for (int k = 0; k < ncell; ++k) {
+        int const b = cb[k], cnt = ce[k] - b;
+        for (int t = tid; t < cnt; t += BUILD_T) {
+          unsigned id = sid[b + t];
+          int rank = 0;
+          for (int u = 0; u < cnt; ++u) rank += sid[b + u] < id ? 1 : 0;
+          lpos[lb[k] + rank] = spos[b + t];
+          lid[lb[k] + rank] = id;
+        }
+      }
+      __syncthreads();
+      double const rc2 = g.rc * g.rc;
+      int const ob = lb[own_slot], oe = lb[own_slot + 1];
+      for (int t = ob + tid; t < oe; t += BUILD_T) {
+        unsigned id = lid[t];
+        if ((id >> NB_SHIFT_BITS) != 13u) continue;   // ghosts have no rows
+        long i = id & NB_IDX_MASK;
+        double4 p = lpos[t];
+        double xi[3] = {p.x, p.y, p.z};
+        unsigned* row = nbr + i * static_cast<long>(K);
+        int cnt = 0;
+        for (int s = 0; s < total; ++s) {
+          if (s == t) continue;
+          double4 q = lpos[s];
+          double xn[3] = {q.x, q.y, q.z};
+          if (dist2(xi, xn) < rc2) {
+            if (cnt < K) row[cnt] = lid[s];
+            ++cnt;
+          }
+        }
+        ncnt[i] = cnt;
+        atomicMax(&st[1], static_cast<unsigned>(cnt));
+      }
+    }
+
+    // Unstaged fallback (a 3x3x3 block with more than STAGE_CAP entries): cells sorted in place.
+    __global__ void k_cell_sort2(int ncells, int const* __restrict__ cstart, double4* __restrict__ spos,
+                                 unsigned* __restrict__ sid) {
+      int c = blockIdx.x * blockDim.x + threadIdx.x;
+      if (c >= ncells) return;
+      int b = cstart[c], e = cstart[c + 1];
+      for (int i = b + 1; i < e; ++i) {
+        unsigned key = sid[i];
+        double4 v = spos[i];
+        int j = i - 1;
+        while (j >= b && sid[j] > key) {
+          sid[j + 1] = sid[j];
+          spos[j + 1] = spos[j];
+          --j;
+        }
+        sid[j + 1] = key;
+        spos[j + 1] = v;
+      }
+    }
+
+    __global__ void k_build2(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid, long n,
+                             long gcap, unsigned const* __restrict__ st, int const* __restrict__ cstart,
+                             unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ mx) {
+      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (e >= live_entries(st, n, gcap)) return;
+      unsigned id = sid[e];
+      if ((id >> NB_SHIFT_BITS) != 13u) return;
+      long i = id & NB_IDX_MASK;
+      double4 p = spos[e];
+      double xi[3] = {p.x, p.y, p.z};
+      int c3[3];
+      cell_idx(g, xi, c3);
+      double const rc2 = g.rc * g.rc;
+      int cnt = 0;
+      unsigned* row = nbr + i * static_cast<long>(K);
+      for (int z = max(c3[2] - 1, 0); z < min(c3[2] + 2, g.shape[2]); ++z)
+        for (int y = max(c3[1] - 1, 0); y < min(c3[1] + 2, g.shape[1]); ++y)
+          for (int x = max(c3[0] - 1, 0); x < min(c3[0] + 2, g.shape[0]); ++x) {
+            int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
+            for (int s = cstart[cc]; s < cstart[cc + 1]; ++s) {
+              if (s == e) continue;
+              double4 q = spos[s];
+              double xn[3] = {q.x, q.y, q.z};
+              if (dist2(xi, xn) < rc2) {
+                if (cnt < K) row[cnt] = sid[s];
+                ++cnt;
+              }
+            }
+          }
+      ncnt[i] = cnt;
+      atomicMax(mx, static_cast<unsigned>(cnt));
+    }
+
     // ---- exclusive scan (int), two levels per recursion --------------------------------------
     constexpr int SCAN_B = 1024;
 
@@ -376,68 +604,81 @@ namespace flyg {
     }
   }
 
+  // One rebuild costs one host synchronisation: capacities (ghost slots, row length) come from the
+  // previous build; the device records what it needed and the build is redone only on overflow.
   void list_rebuild(flyg_list_s& nl, double const* d_xyz, long n) {
     require(n > 0, "List::rebuild with no atoms");
     require(n < MAX_ATOMS, "too many atoms for the packed neighbour format");
     flyg_ctx_s& ctx = *nl.ctx;
     hipStream_t st = ctx.stream;
-    nl.n = n;
+    if (!nl.hstate) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&nl.hstate), 4 * sizeof(unsigned), hipHostMallocDefault));
+    nl.lstate.reserve(4);
     nl.built = false;
-    nl.pos.reserve(n);
-    k_canon<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get());
-
-    // Ghost count, then ghost images.
-    unsigned* ctr = ctx.dflags.get() + 1;
-    HIPCHK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
-    k_ghosts<false><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nullptr, nullptr, ctr, 0);
-    HIPCHK(hipMemcpyAsync(ctx.hflags, ctr, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    long G = ctx.hflags[0];
-    long m = n + G;
-    nl.nghost = G;
-    // (DBuf::reserve does not preserve contents: size the entry arrays before anything is written.)
-    nl.ent.reserve(m);
-    nl.eid.reserve(m);
-    k_reals<<<nblk(n, TPB), TPB, 0, st>>>(nl.pos.get(), n, nl.ent.get(), nl.eid.get());
-    HIPCHK(hipMemsetAsync(ctr, 0, sizeof(unsigned), st));
-    k_ghosts<true><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), ctr, m);
-
-    // Counting sort into cells.
-    int nc = nl.grid.ncells;
-    nl.ccount.reserve(nc);
-    nl.cstart.reserve(nc);
-    nl.ccursor.reserve(nc);
-    nl.ecell.reserve(m);
-    nl.spos.reserve(m);
-    nl.sid.reserve(m);
-    HIPCHK(hipMemsetAsync(nl.ccount.get(), 0, nc * sizeof(int), st));
-    HIPCHK(hipMemsetAsync(nl.ccursor.get(), 0, nc * sizeof(int), st));
-    k_cell_count<<<nblk(m, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), m, nl.ecell.get(), nl.ccount.get());
-    HIPCHK(hipMemcpyAsync(nl.cstart.get(), nl.ccount.get(), nc * sizeof(int), hipMemcpyDeviceToDevice, st));
-    exclusive_scan(ctx, nl.cstart.get(), nc, nl.scan_tmp);
-    k_scatter<<<nblk(m, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), m, nl.cstart.get(),
-                                           nl.ccursor.get(), nl.spos.get(), nl.sid.get());
-    k_cell_sort<<<nblk(nc, TPB), TPB, 0, st>>>(nc, nl.cstart.get(), nl.ccount.get(), nl.spos.get(), nl.sid.get());
-
-    // Neighbour rows; grow K and redo the last step if any row overflowed.
-    if (nl.K == 0) nl.K = 64;
-    nl.ncnt.reserve(n);
-    HIPCHK(hipMemsetAsync(nl.ncnt.get(), 0, n * sizeof(int), st));
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
-      unsigned* mx = ctx.dflags.get() + 2;
-      HIPCHK(hipMemsetAsync(mx, 0, sizeof(unsigned), st));
-      k_build<<<nblk(m, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), m, nl.cstart.get(), nl.ccount.get(),
-                                           nl.nbr.get(), nl.ncnt.get(), nl.K, mx);
-      HIPCHK(hipMemcpyAsync(ctx.hflags + 1, mx, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      int maxc = static_cast<int>(ctx.hflags[1]);
-      if (maxc <= nl.K) break;
-      require(attempt == 0, "neighbour rows overflowed twice");
-      nl.K = (maxc + 15) / 16 * 16 + 16;
+    if (n != nl.n) {
+      nl.gcap = 0;
+      nl.K = 0;
     }
-    HIPCHK(hipGetLastError());
-    nl.built = true;
+    nl.n = n;
+    if (nl.gcap == 0) nl.gcap = std::max(4096l, n);
+    if (nl.K == 0) nl.K = 64;
+    int const nc = nl.grid.ncells;
+    nl.pos.reserve(n);
+    nl.cstart.reserve(nc + 1);
+    nl.ccursor.reserve(nc);
+    nl.ncnt.reserve(n);
+    unsigned* dst = nl.lstate.get();
+
+    for (int attempt = 0; attempt < 3; ++attempt) {
+      long const cap = n + nl.gcap;
+      nl.ent.reserve(cap);
+      nl.eid.reserve(cap);
+      nl.ecell.reserve(cap);
+      nl.spos.reserve(cap);
+      nl.sid.reserve(cap);
+      nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
+      long nz = std::max<long>(nc + 1, 4);
+      k_clear<<<nblk(nz, TPB), TPB, 0, st>>>(nl.cstart.get(), nc + 1, nl.ccursor.get(), nc, dst);
+      k_canon_reals<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get());
+      k_ghosts_cap<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), dst, nl.gcap);
+      k_cell_count_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), n, nl.gcap, dst, nl.ecell.get(),
+                                                       nl.cstart.get());
+      exclusive_scan(ctx, nl.cstart.get(), nc + 1, nl.scan_tmp);
+      k_scatter_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst,
+                                                    nl.cstart.get(), nl.ccursor.get(), nl.spos.get(), nl.sid.get());
+      k_build_lds<<<nc, BUILD_T, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), nl.cstart.get(), nl.nbr.get(),
+                                          nl.ncnt.get(), nl.K, dst);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(nl.hstate, dst, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      unsigned const G = nl.hstate[0], maxrow = nl.hstate[1], flags = nl.hstate[3];
+      if (flags & OV_GHOST) {   // more ghost images than slots: grow and start again
+        nl.gcap = static_cast<long>(G) + static_cast<long>(G) / 4 + 4096;
+        continue;
+      }
+      nl.nghost = G;
+      if (flags & OV_STAGE) {   // a 3x3x3 cell block too large for LDS: unstaged build
+        k_cell_sort2<<<nblk(nc, TPB), TPB, 0, st>>>(nc, nl.cstart.get(), nl.spos.get(), nl.sid.get());
+        for (int a2 = 0; a2 < 2; ++a2) {
+          nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
+          HIPCHK(hipMemsetAsync(dst + 1, 0, sizeof(unsigned), st));
+          k_build2<<<nblk(n + nl.gcap, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), n, nl.gcap, dst,
+                                                          nl.cstart.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, dst + 1);
+          HIPCHK(hipMemcpyAsync(nl.hstate, dst, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
+          HIPCHK(hipStreamSynchronize(st));
+          if (static_cast<int>(nl.hstate[1]) <= nl.K) break;
+          nl.K = (static_cast<int>(nl.hstate[1]) + 15) / 16 * 16 + 16;
+        }
+        nl.built = true;
+        return;
+      }
+      if (static_cast<int>(maxrow) > nl.K) {   // a row overflowed: widen the rows and build again
+        nl.K = (static_cast<int>(maxrow) + 15) / 16 * 16 + 16;
+        continue;
+      }
+      nl.built = true;
+      return;
+    }
+    throw Error(FLYG_DEVICE_ERROR, "neighbour list build did not converge");
   }
 
   void list_update(flyg_list_s& nl, double const* d_delta, long n) {

@@ -93,6 +93,7 @@ _SIGS = {
     "flyg_eam_destroy": [_vp],
     "flyg_eam_info": [_vp, C.POINTER(C.c_int), _d, _d, C.c_char_p],
     "flyg_eam_set_precision": [_vp, C.c_int],
+    "flyg_eam_set_lanes": [_vp, C.c_int],
     "flyg_eam_table_eval": [_vp, C.c_int, C.c_int, C.c_int, _d, C.c_int64, C.c_int, _d],
     "flyg_list_create": [_vp, _d, C.POINTER(C.c_int), C.c_double, C.POINTER(_vp)],
     "flyg_list_destroy": [_vp],
@@ -256,6 +257,9 @@ class EAM:
 
     def set_precision(self, precision):
         self.ctx.check(lib().flyg_eam_set_precision(self.h, precision))
+
+    def set_lanes(self, lanes):
+        self.ctx.check(lib().flyg_eam_set_lanes(self.h, lanes))
 
     def table(self, kind, a, b, x, deriv=0):
         x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)

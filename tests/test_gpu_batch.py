@@ -126,3 +126,22 @@ def test_alone_equals_in_mix(ctx, setup):
     assert a["exit"] == m["exit"] and a["steps"] == m["steps"] and a["grads"] == m["grads"]
     assert a["barrier"] == m["barrier"]
     assert np.array_equal(a["pos"], m["pos"])
+
+
+def test_results_independent_of_kernel_shape(ctx, setup):
+    # Row sums use a 16-slot tree and workgroup dot products a 512-slot tree, so the autotuner's choice of
+    # (block threads, lanes per atom) cannot change any result bit.
+    S = setup
+    st = starts(S, 4, seed0=51)
+    outs = []
+    for T, L in ((256, 8), (512, 16), (128, 8), (256, 4)):
+        b = fa.Batch(ctx, S["pot"], fa.Box(S["basis"]), S["mn"], S["t"], dimer=dict(f2norm=1e-3), precision=fa.MIXED,
+                     block_threads=T, lanes_per_atom=L)
+        ids = b.submit([c for c, _, _, _ in st], [RADIUS] * len(st), np.stack([x for _, _, x, _ in st]),
+                       np.stack([a for _, _, _, a in st]))
+        b.run()
+        outs.append([b.result(i, positions=True) for i in ids])
+    for other in outs[1:]:
+        for a, m in zip(outs[0], other):
+            assert a["exit"] == m["exit"] and a["steps"] == m["steps"] and a["barrier"] == m["barrier"]
+            assert np.array_equal(a["pos"], m["pos"])

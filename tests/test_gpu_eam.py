@@ -182,3 +182,18 @@ def test_list_cutoff_smaller_than_potential_raises(ctx, wen_gpu):
     nl.rebuild(pos)
     with pytest.raises(fa.FlyError):
         wen_gpu.energy(np.zeros(len(pos), np.uint32), np.zeros(len(pos), np.uint8), nl)
+
+
+@pytest.mark.parametrize("precision", [fa.FP64, fa.MIXED])
+def test_gradient_independent_of_lanes(ctx, precision):
+    # Row sums use a fixed 16-slot tree, so 4, 8 and 16 lanes per atom give bitwise identical gradients.
+    pot = fa.EAM(ctx, systems.potential_path(("Fe", "X")))
+    pot.set_precision(precision)
+    s = systems.config2(seed=5, cells=10)
+    nl = fa.List(ctx, fa.Box(s.basis), 5.3)
+    nl.rebuild(s.pos)
+    out = []
+    for L in (4, 8, 16):
+        pot.set_lanes(L)
+        out.append(pot.gradient(s.types, s.frozen, nl))
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[2])
