@@ -141,6 +141,28 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
     return out, (s, path, centres, radii, starts, axes, dimer)
 
 
+def serve_bench(ctx, fa, systems, n_req=100, rate=200.0):
+    """Config 4: Poisson arrivals at 200/s of requests with their own bcc Fe systems (N log-uniform in
+    [2000, 250000], 0.5-2 % vacancies + interstitials), 8-64 searches each, radii U(4, 9) A, served with
+    continuous batching; latency = arrival to last search done."""
+    from openfly_amd import serve
+    pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
+    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
+    serve.serve(srv, serve.make_requests(4, rate=rate, seed=40, n_max=20000))   # warm-up + shape autotune
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(n_req, rate=rate, seed=41)
+    gen = time.perf_counter() - t0
+    out = serve.serve(srv, reqs)
+    st = srv.status()
+    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, bcc Fe N log-uniform [2000, 250000] with 0.5-2% "
+                        "vacancies + interstitials, 8-64 searches per request, radius U(4,9) A, kick 0.6 A, "
+                        "|g| < 1e-3 eV/A, FP32 pair / FP64 accumulation",
+               request_generation_s=gen, launches=st["launches"], kernel_ms=st["kernel_ms"],
+               block_threads=st["block_threads"], lanes_per_atom=st["lanes_per_atom"],
+               max_atoms=int(max(r.system.n for r in reqs)))
+    return out
+
+
 def cpu_search_baseline(data_args, cpu_sample=16):
     """FP64 oracle, one search per thread over all host cores (find.cpp:697-732 with threads = 1)."""
     from oracle import pyoracle as po
@@ -178,6 +200,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-search", action="store_true", help="skip the batched saddle-search leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served saddle-search leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box of this build has a "
@@ -301,6 +324,8 @@ def main():
         if not args.no_cpu:
             sr["cpu_baseline"] = cpu_search_baseline(sargs)
         out["searches"] = sr
+    if not args.no_serve:
+        out["serving"] = serve_bench(ctx, fa, systems)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, path)
     print(json.dumps(out), flush=True)

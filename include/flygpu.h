@@ -30,6 +30,7 @@ extern "C" {
 #define FLYG_INVALID_ARG 1  /* reference: fly::RuntimeError thrown by verify()/error() (core.hpp:89-138) */
 #define FLYG_SPLINE_OOB 2   /* reference: fly::Spline::OutOfBounds (spline.hpp:46-51, thrown at :141-143) */
 #define FLYG_DEVICE_ERROR 3 /* HIP runtime failure */
+#define FLYG_BUSY 4         /* serving: the device memory budget cannot take this now; retry after searches finish */
 
 /* Pointer location for `where` arguments. */
 #define FLYG_HOST 0
@@ -53,6 +54,7 @@ typedef struct flyg_steplbfgs_s* flyg_steplbfgs;
 typedef struct flyg_lbfgs_s* flyg_lbfgs;
 typedef struct flyg_dimer_s* flyg_dimer;
 typedef struct flyg_batch_s* flyg_batch;
+typedef struct flyg_server_s* flyg_server;
 
 /* ---------------------------------------------------------------------------------------------------
  * Context: a device, one HIP stream (default priority) and the last error message.
@@ -241,6 +243,61 @@ int flyg_batch_run(flyg_batch b);
 int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double* pos, double* axis);
 /* Mobile / region atom counts of search `id`. */
 int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Serving: continuous batching of saddle-search requests from many systems (a kMC loop's catalogue
+ * misses, Master::find_mechs -> find_batch, find.cpp:211-312,684-755, one system per request stream).
+ * Each admitted system stays resident on the device (positions, base densities, types) until removed
+ * and idle; each search keeps its region lists and L-BFGS histories on the device until it ends.
+ * Queued searches of any systems are packed into launches of the persistent search kernel on up to
+ * `streams` streams; a stream that finishes takes whatever is queued then, so requests that arrive
+ * while others run join the next launch and their work overlaps on the device.  Admission is gated
+ * by a device memory budget.  Results are bitwise independent of what else runs beside a search.
+ * No reference counterpart as an API: the reference runs the same searches as OpenMP tasks.
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct {
+  int precision;        /* FLYG_MIXED (default) or FLYG_FP64 */
+  int block_threads;    /* workgroup size, 0 = autotuned on the first launch */
+  int lanes_per_atom;   /* lanes per atom row, 0 = autotuned */
+  double list_skin;     /* initial Verlet skin of the per-search lists (A), 0 = default */
+  int max_concurrent;   /* resident workgroups per launch, 0 = 2 per compute unit */
+  int streams;          /* concurrent launches, 0 = 4 */
+  int max_batch;        /* searches per launch, 0 = 4 per compute unit */
+  int64_t mem_budget;   /* device bytes for systems + launches, 0 = 80% of the free memory at creation */
+} flyg_server_options;
+
+typedef struct {
+  int64_t queued, in_flight, systems, device_bytes, budget_bytes, launches, searches_done;
+  double kernel_ms;     /* summed kernel time of finished launches */
+  int busy_streams, block_threads, lanes_per_atom;
+} flyg_server_state;
+
+void flyg_server_default_options(flyg_server_options* opt);
+int flyg_server_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt, const flyg_rotor_options* ropt,
+                       const flyg_server_options* sopt, flyg_server* out);
+int flyg_server_destroy(flyg_server s);
+/* Admit a system (a relaxed minimum); FLYG_BUSY when the memory budget cannot hold it now. */
+int flyg_server_add_system(flyg_server s, const double basis[9], const int periodic[3], const double* min_pos,
+                           const uint32_t* types, int64_t n, int64_t* sys_id);
+/* The system is freed once none of its searches is queued or running. */
+int flyg_server_remove_system(flyg_server s, int64_t sys_id);
+/* Queue `count` searches on a system.  Search k freezes every atom farther than radius[k] from
+ * centre[k]; its start state is the base positions except the atoms kick_atoms[kick_offsets[k] ..
+ * kick_offsets[k+1]) at kick_pos (3 per atom), and its dimer axis is kick_axis on those atoms, zero
+ * elsewhere (the sparse form of what Master::perturb writes, find.cpp:1147-1214). */
+int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* centre, const double* radius,
+                       const int64_t* kick_offsets, const int64_t* kick_atoms, const double* kick_pos,
+                       const double* kick_axis, int64_t* ids);
+/* Harvest finished launches and start queued work on free streams; waits up to wait_ms for at least one
+ * search to finish.  Writes up to `cap` finished search ids. */
+int flyg_server_progress(flyg_server s, double wait_ms, int64_t* done_ids, int cap, int* n_done);
+/* Result of a finished search: stats (energy = FP64 barrier), mobile atom count, and (any may be NULL)
+ * the mobile atoms' global indices, saddle positions and axis (3 per mobile atom). */
+int flyg_server_result(flyg_server s, int64_t id, flyg_search_stats* stats, int* n_mobile, int64_t* atoms,
+                       double* pos, double* axis);
+/* Forget a finished search's result. */
+int flyg_server_release(flyg_server s, int64_t id);
+int flyg_server_status(flyg_server s, flyg_server_state* state);
 
 #ifdef __cplusplus
 }

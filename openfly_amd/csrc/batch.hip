@@ -18,6 +18,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
 #include <numeric>
 #include <thread>
 
@@ -40,6 +43,11 @@ namespace flyg {
       long v3_off;         // first element of this search's 3*nm start/out vectors
       long ws_off;         // first element of this search's workspace (nvec * 3*nm)
       double skin;         // Verlet skin of the rows (r_L = r_c + skin)
+      // The search's system, resident on the device for as long as any of its searches runs.
+      double4 const* sbase;   // canonical base positions, by global atom index
+      double const* srho;     // FP64 base densities
+      uint8_t const* stype;   // type ids
+      double3 const* sS;      // the 27 image shifts of the box, by shift code
     };
 
     struct SearchOut {
@@ -55,14 +63,12 @@ namespace flyg {
 
     struct BatchArgs {
       EAMTables T;
-      double3 S[27];
       SearchHdr const* hdr;
       int nsearch;
       double4* cur;              // region positions (base for S1, current evaluation point for M)
-      double4 const* base;
-      uint8_t const* rtype;
-      int const* rgidx;
-      double const* rho_base;    // global FP64 base densities
+      double4* base;             // region base positions (gathered from the system at search start)
+      uint8_t* rtype;            // region type ids (gathered)
+      int const* rgidx;          // region atom -> global atom of the system
       double* rhof;              // frozen part of S1 densities
       double* fp;                // F'(rho) per region atom
       int const* rowptr;
@@ -190,6 +196,8 @@ namespace flyg {
 
     struct SView {
       int nm, nr, n3;
+      double3 const* S;     // image shifts (LDS copy)
+      double const* srho;   // system base densities
       double4* cur;
       double4 const* base;
       uint8_t const* type;
@@ -218,14 +226,15 @@ namespace flyg {
         for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
-          double3 sh = A.S[en >> NB_SHIFT_BITS];
+          double3 sh = v.S[en >> NB_SHIFT_BITS];
           double4 pj = v.cur[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < Tb.rc2) {
             int tj = A.multi ? v.type[j] : 0;
             if (MIX) {
-              rho_acc.add(m, static_cast<double>(spl32f<0>(Tb, Tb.phi[tj + nt * ti], sqrtf(static_cast<float>(r2)), e)));
+              float r2f = static_cast<float>(r2);
+              rho_acc.add(m, static_cast<double>(spl32f<0>(Tb, Tb.phi[tj + nt * ti], r2f * __frsqrt_rn(r2f), e)));
             } else {
               rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], sqrt(r2), e));
             }
@@ -247,7 +256,7 @@ namespace flyg {
         for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
-          double3 sh = A.S[en >> NB_SHIFT_BITS];
+          double3 sh = v.S[en >> NB_SHIFT_BITS];
           double4 pj = v.cur[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
@@ -256,12 +265,14 @@ namespace flyg {
             double fpj = v.fp[j];
             double cf;
             if (MIX) {
-              float rf = sqrtf(static_cast<float>(r2));
+              float r2f = static_cast<float>(r2);
+              float rinv = __frsqrt_rn(r2f);
+              float rf = r2f * rinv;
               float vp = spl32f<1>(Tb, Tb.V[vix(nt, ti, tj)], rf, e);
               float pa = spl32f<1>(Tb, Tb.phi[tj + nt * ti], rf, e);
               float pb = A.multi ? spl32f<1>(Tb, Tb.phi[ti + nt * tj], rf, e) : pa;
               float mag = vp + static_cast<float>(fpi) * pa + static_cast<float>(fpj) * pb;
-              cf = static_cast<double>(mag / rf);
+              cf = static_cast<double>(mag * rinv);
             } else {
               double r = sqrt(r2);
               double vp = spl64<1>(Tb, Tb.V[vix(nt, ti, tj)], r, e);
@@ -303,7 +314,7 @@ namespace flyg {
         for (int k = b + lane, m = 0; k < c; k += L, ++m) {
           unsigned en = v.ent[k];
           int j = static_cast<int>(en & NB_IDX_MASK);
-          double3 sh = A.S[en >> NB_SHIFT_BITS];
+          double3 sh = v.S[en >> NB_SHIFT_BITS];
           double4 pj = v.cur[j];
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
@@ -319,7 +330,7 @@ namespace flyg {
         if (lane == 0) {
           double rt;
           if (base) {
-            rt = A.rho_base[v.gidx[r0]];
+            rt = v.srho[v.gidx[r0]];
             if (!mob) v.rhof[r0] = rt - rho;
           } else {
             rt = mob ? rho : rho + v.rhof[r0];
@@ -404,13 +415,15 @@ namespace flyg {
     };
 
     template <int T, int L, bool MIX>
-    __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core) {
+    __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S) {
       SearchHdr const h = A.hdr[sid];
       DimOpt const& o = A.o;
       SView v;
       v.nm = h.nm;
       v.nr = h.nr;
       v.n3 = 3 * h.nm;
+      v.S = lds_S;
+      v.srho = h.srho;
       v.cur = A.cur + h.reg_off;
       v.base = A.base + h.reg_off;
       v.type = A.rtype + h.reg_off;
@@ -442,8 +455,16 @@ namespace flyg {
       long grads = 0;
       int rot_iters = 0;
 
-      // Base state: region at base positions, frozen density parts, base local energy.
-      for (int i = tm.tid; i < v.nr; i += T) v.cur[i] = v.base[i];
+      // Gather the region from the resident system, then the base state: region at base positions,
+      // frozen density parts, base local energy.
+      if (tm.tid < 27) lds_S[tm.tid] = h.sS[tm.tid];
+      for (int i = tm.tid; i < v.nr; i += T) {
+        int gi = v.gidx[i];
+        double4 b = h.sbase[gi];
+        A.base[h.reg_off + i] = b;
+        A.rtype[h.reg_off + i] = h.stype[gi];
+        v.cur[i] = b;
+      }
       __syncthreads();
       double e_base = region_energy<T, L>(A, v, tm, true, err);
       for (int i = tm.tid; i < n3; i += T) {
@@ -639,6 +660,7 @@ namespace flyg {
       __shared__ double red[4 * T];
       __shared__ double lds_core[128];
       __shared__ int s_id;
+      __shared__ double3 lds_S[27];
       Team<T> tm{red, static_cast<int>(threadIdx.x)};
       while (true) {
         if (threadIdx.x == 0) s_id = static_cast<int>(atomicAdd(A.queue, 1u));
@@ -646,7 +668,7 @@ namespace flyg {
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX>(A, sid, tm, lds_core);
+        run_search<T, L, MIX>(A, sid, tm, lds_core, lds_S);
       }
     }
 
@@ -654,7 +676,8 @@ namespace flyg {
 }  // namespace flyg
 
 // =====================================================================================================
-// Host side: region extraction, packing, launch, re-queue, autotuning and the C-ABI
+// Host side: resident systems, region extraction, packed launches ("flights"), re-queue, autotuning,
+// the batch C-ABI (Master::find_batch) and the serving C-ABI (continuous batching over many systems).
 // =====================================================================================================
 
 namespace flyg {
@@ -727,85 +750,174 @@ namespace flyg {
       }
     };
 
+    // One base system (a relaxed minimum), resident on the device while any of its searches is queued
+    // or running: canonical positions, FP64 base densities, types and the box's image shifts.
+    struct SysState {
+      int64_t id = 0;
+      long n = 0;
+      double basis[9];
+      int periodic[3];
+      BoxDesc box{};
+      std::vector<double> base;       // canonical, strictly inside [0, L)
+      std::vector<uint32_t> types;
+      HostGrid grid;
+      DBuf<double4> d_base;
+      DBuf<double> d_rho;
+      DBuf<uint8_t> d_type;
+      DBuf<double3> d_S;
+      long active = 0;                // searches queued or in flight
+      bool retired = false;           // removal requested; freed once idle
+      std::size_t dev_bytes() const { return d_base.cap * sizeof(double4) + d_rho.cap * 8 + d_type.cap + d_S.cap * sizeof(double3); }
+      double skin_cap(double rc) const {
+        return 0.5 * std::fmin(box.ext[0], std::fmin(box.ext[1], box.ext[2])) - rc;
+      }
+    };
+
+    // Device bytes a system will hold, and the transient peak of building its base densities.
+    std::size_t sys_bytes_estimate(long n) { return static_cast<std::size_t>(n) * (32 + 8 + 1) + 27 * 24 + 4096; }
+    std::size_t sys_transient_estimate(long n) { return static_cast<std::size_t>(n) * 700; }
+
+    std::unique_ptr<SysState> make_system(flyg_ctx_s* ctx, flyg_eam_s* eam, double const basis[9], int const periodic[3],
+                                          double const* min_pos, uint32_t const* types, long n, double list_skin) {
+      auto S = std::make_unique<SysState>();
+      S->n = n;
+      std::memcpy(S->basis, basis, sizeof(S->basis));
+      std::memcpy(S->periodic, periodic, sizeof(S->periodic));
+      S->box = make_box(basis, periodic);
+      require(S->box.ortho && periodic[0] && periodic[1] && periodic[2],
+              "batched searches need a fully periodic orthorhombic box");
+      require(n < MAX_ATOMS, "too many atoms");
+      S->base.resize(3 * n);
+      S->types.assign(types, types + n);
+      std::vector<double4> b4(n);
+      std::vector<uint8_t> t8(n);
+      for (long i = 0; i < n; ++i) {
+        require(types[i] < static_cast<uint32_t>(eam->tab.ntypes), "TypeID out of range");
+        for (int k = 0; k < 3; ++k) {
+          // Strictly inside [0, L): the canonical image of a coordinate just below 0 rounds to L, and the
+          // cell grid and the image shifts must agree on which side of the box an atom is.
+          double x = min_pos[3 * i + k];
+          double w = x - S->box.ext[k] * std::floor(x * S->box.inv_ext[k]);
+          if (w >= S->box.ext[k]) w -= S->box.ext[k];
+          if (w < 0) w = 0;
+          S->base[3 * i + k] = w;
+        }
+        b4[i] = make_double4(S->base[3 * i], S->base[3 * i + 1], S->base[3 * i + 2], 0.0);
+        t8[i] = static_cast<uint8_t>(types[i]);
+      }
+      double rl = eam->tab.rc + list_skin;
+      for (int k = 0; k < 3; ++k) require(S->box.ext[k] > 2.0 * rl, "box too small for the search region cut-off");
+      S->grid.build(S->base.data(), n, S->box.ext, rl);
+      hipStream_t st = ctx->stream;
+      S->d_base.reserve(n);
+      S->d_rho.reserve(n);
+      S->d_type.reserve(n);
+      S->d_S.reserve(27);
+      double3 sh[27];
+      shift_table(S->box, sh);
+      HIPCHK(hipMemcpyAsync(S->d_base.get(), b4.data(), n * sizeof(double4), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(S->d_type.get(), t8.data(), n, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(S->d_S.get(), sh, sizeof(sh), hipMemcpyHostToDevice, st));
+      {   // global FP64 base densities from a full list at r_c
+        auto nl = make_list(ctx, basis, periodic, eam->tab.rc);
+        DBuf<double> dpos;
+        dpos.reserve(3 * n);
+        DBuf<uint32_t> dt;
+        dt.reserve(n);
+        HIPCHK(hipMemcpyAsync(dpos.get(), S->base.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dt.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        list_rebuild(*nl, dpos.get(), n);
+        eam_density(*eam, *nl, dt.get(), S->d_rho.get());
+        unsigned fl = ctx->take_flags();   // synchronises the stream before the scratch goes away
+        if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+      }
+      return S;
+    }
+
     struct Region {
       int nm = 0, nr = 0;
       std::vector<int> gidx;
       std::vector<int> rowlen;      // nr
       std::vector<unsigned> ent;    // packed (shift, local)
-      std::string err;
     };
 
     struct Pending {
       int64_t id;
       int launches = 0;
+      SysState* sys = nullptr;
       int64_t centre;
       double radius;
       double skin;
       std::vector<int> mobile;      // global indices (sorted)
       std::vector<double> x, ax;    // 3*nm start state (continuous with the base positions)
+      std::shared_ptr<Region> region;   // built for `skin`, reset when the skin grows
     };
 
-  }  // namespace
-}  // namespace flyg
-
-struct flyg_batch_s {
-  flyg_ctx_s* ctx = nullptr;
-  flyg_eam_s* eam = nullptr;
-  flyg_dimer_options opt;
-  flyg_rotor_options ropt;
-  flyg_batch_options bopt;
-  double basis[9];
-  int periodic[3];
-  flyg::BoxDesc box;
-  long n = 0;
-  std::vector<double> base;
-  std::vector<uint32_t> types;
-  flyg::HostGrid grid;
-  flyg::DBuf<double> rho_base;
-  std::vector<flyg::Pending> pending;
-  struct Result {
-    int done = 0;
-    flyg_search_stats st{};
-    std::vector<int> mobile;
-    std::vector<double> x, ax;
-    int nm = 0, nr = 0;
-    double max_disp = 0;
-  };
-  std::vector<Result> results;
-  int tuned_T = 0, tuned_L = 0;
-  // device staging (grown on demand)
-  flyg::DBuf<flyg::SearchHdr> d_hdr;
-  flyg::DBuf<double4> d_cur, d_base;
-  flyg::DBuf<uint8_t> d_type;
-  flyg::DBuf<int> d_gidx, d_rowptr;
-  flyg::DBuf<double> d_rhof, d_fp, d_ws, d_sx, d_sax, d_ox, d_oax;
-  flyg::DBuf<unsigned> d_ent, d_queue;
-  flyg::DBuf<flyg::SearchOut> d_out;
-  double last_kernel_ms = 0;
-};
-
-namespace flyg {
-  namespace {
+    struct Result {
+      int done = 0;
+      flyg_search_stats st{};
+      std::vector<int> mobile;
+      std::vector<double> x, ax;
+      int nr = 0;
+    };
 
     double min_image(double d, double L) { return d - L * std::round(d / L); }
 
-    void mobile_set(flyg_batch_s const& B, int64_t centre, double R, std::vector<int>& out) {
+    void mobile_set(SysState const& S, int64_t centre, double R, std::vector<int>& out) {
       out.clear();
-      double const* pc = &B.base[3 * centre];
-      B.grid.query(pc, R, [&](int j, int, double) { out.push_back(j); });
+      S.grid.query(&S.base[3 * centre], R, [&](int j, int, double) { out.push_back(j); });
       std::sort(out.begin(), out.end());
       out.erase(std::unique(out.begin(), out.end()), out.end());
     }
 
-    Region build_region(flyg_batch_s const& B, Pending const& P, double rL, std::vector<int>& local_of) {
-      Region g;
+    // Queues one search of system S: mobile set around `centre`, start state continuous with the base.
+    // Start positions/axis come either dense (3n arrays) or sparse (listed atoms only; the rest at base).
+    Pending make_pending(SysState& S, int64_t id, int64_t centre, double radius, double skin, double const* sp,
+                         double const* sa, int64_t const* kidx, long nk) {
+      require(centre >= 0 && centre < S.n, "centre out of range");
+      require(radius > 0 && radius < 0.5 * std::fmin(S.box.ext[0], std::fmin(S.box.ext[1], S.box.ext[2])) - 1.0,
+              "active radius must be positive and below half the box");
+      Pending P;
+      P.id = id;
+      P.sys = &S;
+      P.centre = centre;
+      P.radius = radius;
+      P.skin = skin;
+      mobile_set(S, centre, radius, P.mobile);
+      int nm = static_cast<int>(P.mobile.size());
+      P.x.resize(3 * nm);
+      P.ax.assign(3 * nm, 0.0);
+      for (int i = 0; i < nm; ++i)
+        for (int c = 0; c < 3; ++c) P.x[3 * i + c] = S.base[3 * P.mobile[i] + c];
+      auto put = [&](long g, double const* p, double const* a) {
+        auto it = std::lower_bound(P.mobile.begin(), P.mobile.end(), static_cast<int>(g));
+        if (it == P.mobile.end() || *it != g) return;   // frozen atoms keep their base position
+        long i = it - P.mobile.begin();
+        for (int c = 0; c < 3; ++c) {
+          double bse = S.base[3 * g + c];
+          P.x[3 * i + c] = bse + min_image(p[c] - bse, S.box.ext[c]);
+          P.ax[3 * i + c] = a[c];
+        }
+      };
+      if (kidx) {
+        for (long q = 0; q < nk; ++q) {
+          require(kidx[q] >= 0 && kidx[q] < S.n, "perturbed atom index out of range");
+          put(kidx[q], sp + 3 * q, sa + 3 * q);
+        }
+      } else {
+        for (int i = 0; i < nm; ++i) put(P.mobile[i], sp + 3L * P.mobile[i], sa + 3L * P.mobile[i]);
+      }
+      return P;
+    }
+
+    void build_region(SysState const& S, Pending const& P, double rL, std::vector<int>& local_of, Region& g) {
       int const nm = static_cast<int>(P.mobile.size());
       g.nm = nm;
       g.gidx = P.mobile;
       for (int i = 0; i < nm; ++i) local_of[P.mobile[i]] = i;
       std::vector<int> s1;
       for (int i = 0; i < nm; ++i)
-        B.grid.query(&B.base[3 * P.mobile[i]], rL, [&](int j, int, double r2) {
+        S.grid.query(&S.base[3 * P.mobile[i]], rL, [&](int j, int, double r2) {
           if (r2 < rL * rL && local_of[j] == -1) {
             local_of[j] = -2;   // collected into S1
             s1.push_back(j);
@@ -816,12 +928,13 @@ namespace flyg {
       g.gidx.insert(g.gidx.end(), s1.begin(), s1.end());
       g.nr = static_cast<int>(g.gidx.size());
       g.rowlen.assign(g.nr, 0);
+      g.ent.clear();
       std::vector<unsigned> row;
       for (int r = 0; r < g.nr; ++r) {
         int gi = g.gidx[r];
         bool mob = r < nm;
         row.clear();
-        B.grid.query(&B.base[3 * gi], rL, [&](int j, int sc, double r2) {
+        S.grid.query(&S.base[3 * gi], rL, [&](int j, int sc, double r2) {
           if (!(r2 < rL * rL)) return;
           if (j == gi && sc == 13) return;
           int lj = local_of[j];
@@ -837,8 +950,6 @@ namespace flyg {
         g.ent.insert(g.ent.end(), row.begin(), row.end());
       }
       for (int j : g.gidx) local_of[j] = -1;
-      for (int j : s1) local_of[j] = -1;
-      return g;
     }
 
     template <typename F>
@@ -855,6 +966,26 @@ namespace flyg {
           for (long i = t; i < n; i += nt) f(i, t);
         });
       for (auto& x : th) x.join();
+    }
+
+    // Builds the missing regions of `work` (in parallel; one -1-filled scratch map per thread).
+    void ensure_regions(std::vector<Pending>& work, double rc) {
+      std::vector<long> need;
+      long nmax = 0;
+      for (long k = 0; k < static_cast<long>(work.size()); ++k)
+        if (!work[k].region) {
+          need.push_back(k);
+          nmax = std::max(nmax, work[k].sys->n);
+        }
+      if (need.empty()) return;
+      std::vector<std::vector<int>> scratch(16);
+      parallel_for(static_cast<long>(need.size()), [&](long q, unsigned t) {
+        if (static_cast<long>(scratch[t].size()) < nmax) scratch[t].assign(nmax, -1);
+        Pending& P = work[need[q]];
+        auto g = std::make_shared<Region>();
+        build_region(*P.sys, P, rc + P.skin, scratch[t], *g);
+        P.region = std::move(g);
+      });
     }
 
     using KernelFn = void (*)(BatchArgs);
@@ -894,158 +1025,231 @@ namespace flyg {
       return d;
     }
 
-    // Runs one launch over `work`; fills outs (one per work item).
-    void launch(flyg_batch_s& B, std::vector<Pending*> const& work, int T, int L, DimOpt const& dopt,
-                std::vector<SearchOut>& outs, std::vector<Region>& regions) {
-      flyg_ctx_s& ctx = *B.ctx;
-      long const ns = static_cast<long>(work.size());
-      regions.assign(ns, Region{});
-      std::vector<std::vector<int>> scratch(16, std::vector<int>());
-      parallel_for(ns, [&](long k, unsigned t) {
-        if (scratch[t].empty()) scratch[t].assign(B.n, -1);
-        Pending const& P = *work[k];
-        regions[k] = build_region(B, P, B.eam->tab.rc + P.skin, scratch[t]);
-      });
-      std::vector<SearchHdr> hdr(ns);
+    // Pinned host buffer (hipHostMalloc), grown on demand.
+    struct Pinned {
+      char* p = nullptr;
+      std::size_t cap = 0;
+      Pinned() = default;
+      Pinned(Pinned const&) = delete;
+      Pinned& operator=(Pinned const&) = delete;
+      ~Pinned() {
+        if (p) (void)hipHostFree(p);
+      }
+      void reserve(std::size_t n) {
+        if (n <= cap) return;
+        if (p) HIPCHK(hipHostFree(p));
+        p = nullptr;
+        cap = 0;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p), n, hipHostMallocDefault));
+        cap = n;
+      }
+    };
+
+    // Byte layout of one launch: sub-arrays at 256-byte aligned offsets of one allocation.
+    struct Layout {
+      std::size_t size = 0;
+      template <typename T>
+      std::size_t add(long count) {
+        std::size_t o = (size + 255) & ~static_cast<std::size_t>(255);
+        size = o + static_cast<std::size_t>(std::max<long>(count, 1)) * sizeof(T);
+        return o;
+      }
+    };
+
+    // One packed launch of the persistent search kernel on its own stream: uploads in one copy, the
+    // kernel, and the results back into pinned memory in one copy, all asynchronous.
+    struct Flight {
+      hipStream_t st = nullptr;
+      bool own_stream = false;
+      hipEvent_t e_a = nullptr, e_b = nullptr, e_done = nullptr;
+      bool busy = false;
+      std::vector<Pending> work;
+      long n3 = 0;
+      DBuf<char> d_in, d_scr;
+      Pinned h_in, h_out;
+      std::size_t o_out = 0, o_ox = 0, o_oax = 0, out_bytes = 0;
+      Flight() = default;
+      Flight(Flight const&) = delete;
+      Flight& operator=(Flight const&) = delete;
+      void init(hipStream_t s, bool own) {
+        st = s;
+        own_stream = own;
+        HIPCHK(hipEventCreate(&e_a));
+        HIPCHK(hipEventCreate(&e_b));
+        HIPCHK(hipEventCreate(&e_done));
+      }
+      ~Flight() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (e_a) (void)hipEventDestroy(e_a);
+        if (e_b) (void)hipEventDestroy(e_b);
+        if (e_done) (void)hipEventDestroy(e_done);
+        if (own_stream && st) (void)hipStreamDestroy(st);
+      }
+      std::size_t dev_bytes() const { return d_in.cap + d_scr.cap; }
+      void release_memory() {   // idle flights only
+        DBuf<char> a(std::move(d_in)), b(std::move(d_scr));
+      }
+      bool finished() const {
+        hipError_t e = hipEventQuery(e_done);
+        if (e == hipErrorNotReady) return false;
+        HIPCHK(e);
+        return true;
+      }
+      float kernel_ms() const {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e_a, e_b));
+        return ms;
+      }
+    };
+
+    // Device bytes a launch of `work` needs (regions must be built).
+    std::size_t flight_bytes(std::vector<Pending> const& work, DimOpt const& dopt) {
+      std::size_t b = 4096;
+      for (auto const& P : work) {
+        Region const& g = *P.region;
+        b += sizeof(SearchHdr) + static_cast<std::size_t>(g.nr) * (4 + 4 + 32 + 32 + 1 + 8 + 8) + g.ent.size() * 4 +
+             static_cast<std::size_t>(3 * g.nm) * 8 * (4 + nvec(dopt.n, dopt.rn)) + sizeof(SearchOut) + 3 * 256;
+      }
+      return b;
+    }
+
+    // Packs `work` (regions built) into F and launches it asynchronously on F.st.
+    void flight_launch(flyg_eam_s& E, Flight& F, std::vector<Pending>&& work, int T, int L, DimOpt const& dopt, bool mix,
+                       int slots) {
+      require(!F.busy, "internal: flight already in use");
+      F.work = std::move(work);
+      long const ns = static_cast<long>(F.work.size());
       long nreg = 0, nrow = 0, nent = 0, n3 = 0, nws = 0;
+      std::vector<SearchHdr> hdr(ns);
       for (long k = 0; k < ns; ++k) {
-        Region const& g = regions[k];
+        Pending const& P = F.work[k];
+        Region const& g = *P.region;
         require(g.nm > 0, "a search has no mobile atoms");
         require(g.nr < (1 << NB_SHIFT_BITS), "search region too large");
-        hdr[k].nm = g.nm;
-        hdr[k].nr = g.nr;
-        hdr[k].reg_off = nreg;
-        hdr[k].row_off = nrow;
-        hdr[k].v3_off = n3;
-        hdr[k].ws_off = nws;
-        hdr[k].skin = work[k]->skin;
+        SearchHdr& h = hdr[k];
+        h.nm = g.nm;
+        h.nr = g.nr;
+        h.reg_off = nreg;
+        h.row_off = nrow;
+        h.v3_off = n3;
+        h.ws_off = nws;
+        h.skin = P.skin;
+        h.sbase = P.sys->d_base.get();
+        h.srho = P.sys->d_rho.get();
+        h.stype = P.sys->d_type.get();
+        h.sS = P.sys->d_S.get();
         nreg += g.nr;
         nrow += g.nr + 1;
         nent += static_cast<long>(g.ent.size());
         n3 += 3L * g.nm;
         nws += static_cast<long>(nvec(dopt.n, dopt.rn)) * 3L * g.nm;
       }
-      std::vector<double4> hbase(nreg);
-      std::vector<uint8_t> htype(nreg);
-      std::vector<int> hgidx(nreg), hrow(nrow);
-      std::vector<unsigned> hent(nent);
-      std::vector<double> hsx(n3), hsax(n3);
+      require(nent < (1l << 31), "too many neighbour entries in one launch");
+      F.n3 = n3;
+      Layout in;
+      std::size_t o_hdr = in.add<SearchHdr>(ns), o_gidx = in.add<int>(nreg), o_row = in.add<int>(nrow),
+                  o_ent = in.add<unsigned>(nent), o_sx = in.add<double>(n3), o_sax = in.add<double>(n3);
+      Layout sc;
+      std::size_t o_cur = sc.add<double4>(nreg), o_base = sc.add<double4>(nreg), o_type = sc.add<uint8_t>(nreg),
+                  o_rhof = sc.add<double>(nreg), o_fp = sc.add<double>(nreg), o_ws = sc.add<double>(nws),
+                  o_q = sc.add<unsigned>(1);
+      std::size_t zero_end = sc.size;
+      F.o_out = sc.add<SearchOut>(ns);
+      F.o_ox = sc.add<double>(n3);
+      F.o_oax = sc.add<double>(n3);
+      F.out_bytes = sc.size - F.o_out;
+      // Grow with slack: freeing device or pinned memory synchronises the whole device, which would
+      // stall the other streams' launches.
+      auto grow = [](std::size_t need, std::size_t cap) { return need > cap ? need + need / 2 : need; };
+      F.h_in.reserve(grow(in.size, F.h_in.cap));
+      F.h_out.reserve(grow(F.out_bytes, F.h_out.cap));
+      char* hb = F.h_in.p;
+      std::memcpy(hb + o_hdr, hdr.data(), ns * sizeof(SearchHdr));
+      int* hg = reinterpret_cast<int*>(hb + o_gidx);
+      int* hr = reinterpret_cast<int*>(hb + o_row);
+      unsigned* he = reinterpret_cast<unsigned*>(hb + o_ent);
+      double* hx = reinterpret_cast<double*>(hb + o_sx);
+      double* ha = reinterpret_cast<double*>(hb + o_sax);
       long e0 = 0;
       for (long k = 0; k < ns; ++k) {
-        Region const& g = regions[k];
-        Pending const& P = *work[k];
-        long ro = hdr[k].reg_off;
-        for (int r = 0; r < g.nr; ++r) {
-          int gi = g.gidx[r];
-          hbase[ro + r] = make_double4(B.base[3 * gi], B.base[3 * gi + 1], B.base[3 * gi + 2], 0.0);
-          htype[ro + r] = static_cast<uint8_t>(B.types[gi]);
-          hgidx[ro + r] = gi;
-        }
+        Pending const& P = F.work[k];
+        Region const& g = *P.region;
+        std::copy(g.gidx.begin(), g.gidx.end(), hg + hdr[k].reg_off);
         long rp = hdr[k].row_off;
+        long e1 = e0;
         for (int r = 0; r < g.nr; ++r) {
-          hrow[rp + r] = static_cast<int>(e0);
-          e0 += g.rowlen[r];
+          hr[rp + r] = static_cast<int>(e1);
+          e1 += g.rowlen[r];
         }
-        hrow[rp + g.nr] = static_cast<int>(e0);
-        std::copy(g.ent.begin(), g.ent.end(), hent.begin() + (hrow[rp]));
-        std::copy(P.x.begin(), P.x.end(), hsx.begin() + hdr[k].v3_off);
-        std::copy(P.ax.begin(), P.ax.end(), hsax.begin() + hdr[k].v3_off);
+        hr[rp + g.nr] = static_cast<int>(e1);
+        std::copy(g.ent.begin(), g.ent.end(), he + e0);
+        e0 = e1;
+        std::copy(P.x.begin(), P.x.end(), hx + hdr[k].v3_off);
+        std::copy(P.ax.begin(), P.ax.end(), ha + hdr[k].v3_off);
       }
-      require(nent < (1l << 31), "too many neighbour entries in one launch");
-      hipStream_t st = ctx.stream;
-      auto up = [&](auto& buf, auto const& v) {
-        buf.reserve(v.size() + 1);
-        if (!v.empty()) HIPCHK(hipMemcpyAsync(buf.get(), v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, st));
-      };
-      up(B.d_hdr, hdr);
-      up(B.d_base, hbase);
-      up(B.d_type, htype);
-      up(B.d_gidx, hgidx);
-      up(B.d_rowptr, hrow);
-      up(B.d_ent, hent);
-      up(B.d_sx, hsx);
-      up(B.d_sax, hsax);
-      B.d_cur.reserve(nreg + 1);
-      B.d_rhof.reserve(nreg + 1);
-      B.d_fp.reserve(nreg + 1);
-      B.d_ws.reserve(nws + 1);
-      B.d_ox.reserve(n3 + 1);
-      B.d_oax.reserve(n3 + 1);
-      B.d_out.reserve(ns + 1);
-      B.d_queue.reserve(1);
-      HIPCHK(hipMemsetAsync(B.d_queue.get(), 0, sizeof(unsigned), st));
-      HIPCHK(hipMemsetAsync(B.d_ws.get(), 0, (nws + 1) * sizeof(double), st));
+      F.d_in.reserve(grow(in.size, F.d_in.cap));
+      F.d_scr.reserve(grow(sc.size, F.d_scr.cap));
+      char* di = F.d_in.get();
+      char* ds = F.d_scr.get();
+      HIPCHK(hipMemcpyAsync(di, hb, in.size, hipMemcpyHostToDevice, F.st));
+      HIPCHK(hipMemsetAsync(ds + o_ws, 0, zero_end - o_ws, F.st));   // workspace and queue counter
 
       BatchArgs A{};
-      A.T = B.eam->tab;
-      shift_table(B.box, A.S);
-      A.hdr = B.d_hdr.get();
+      A.T = E.tab;
+      A.hdr = reinterpret_cast<SearchHdr const*>(di + o_hdr);
       A.nsearch = static_cast<int>(ns);
-      A.cur = B.d_cur.get();
-      A.base = B.d_base.get();
-      A.rtype = B.d_type.get();
-      A.rgidx = B.d_gidx.get();
-      A.rho_base = B.rho_base.get();
-      A.rhof = B.d_rhof.get();
-      A.fp = B.d_fp.get();
-      A.rowptr = B.d_rowptr.get();
-      A.ent = B.d_ent.get();
-      A.ws = B.d_ws.get();
-      A.start_x = B.d_sx.get();
-      A.start_ax = B.d_sax.get();
-      A.out_x = B.d_ox.get();
-      A.out_ax = B.d_oax.get();
-      A.out = B.d_out.get();
-      A.queue = B.d_queue.get();
+      A.cur = reinterpret_cast<double4*>(ds + o_cur);
+      A.base = reinterpret_cast<double4*>(ds + o_base);
+      A.rtype = reinterpret_cast<uint8_t*>(ds + o_type);
+      A.rgidx = reinterpret_cast<int const*>(di + o_gidx);
+      A.rhof = reinterpret_cast<double*>(ds + o_rhof);
+      A.fp = reinterpret_cast<double*>(ds + o_fp);
+      A.rowptr = reinterpret_cast<int const*>(di + o_row);
+      A.ent = reinterpret_cast<unsigned const*>(di + o_ent);
+      A.ws = reinterpret_cast<double*>(ds + o_ws);
+      A.start_x = reinterpret_cast<double const*>(di + o_sx);
+      A.start_ax = reinterpret_cast<double const*>(di + o_sax);
+      A.out_x = reinterpret_cast<double*>(ds + F.o_ox);
+      A.out_ax = reinterpret_cast<double*>(ds + F.o_oax);
+      A.out = reinterpret_cast<SearchOut*>(ds + F.o_out);
+      A.queue = reinterpret_cast<unsigned*>(ds + o_q);
       A.o = dopt;
-      A.multi = B.eam->tab.ntypes > 1;
+      A.multi = E.tab.ntypes > 1;
 
-      int slots = B.bopt.max_concurrent > 0 ? B.bopt.max_concurrent : 2 * ctx.cus;
       unsigned grid = static_cast<unsigned>(std::max<long>(1, std::min<long>(slots, ns)));
-      KernelFn fn = kernel_for(T, L, B.bopt.precision == FLYG_MIXED);
-      hipEvent_t e_a, e_b;
-      HIPCHK(hipEventCreate(&e_a));
-      HIPCHK(hipEventCreate(&e_b));
-      HIPCHK(hipEventRecord(e_a, st));
-      hipLaunchKernelGGL(fn, dim3(grid), dim3(T), 0, st, A);
+      KernelFn fn = kernel_for(T, L, mix);
+      HIPCHK(hipEventRecord(F.e_a, F.st));
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(T), 0, F.st, A);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(e_b, st));
-      outs.resize(ns);
-      HIPCHK(hipMemcpyAsync(outs.data(), B.d_out.get(), ns * sizeof(SearchOut), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, e_a, e_b));
-      B.last_kernel_ms = ms;
-      (void)hipEventDestroy(e_a);
-      (void)hipEventDestroy(e_b);
+      HIPCHK(hipEventRecord(F.e_b, F.st));
+      HIPCHK(hipMemcpyAsync(F.h_out.p, ds + F.o_out, F.out_bytes, hipMemcpyDeviceToHost, F.st));
+      HIPCHK(hipEventRecord(F.e_done, F.st));
+      F.busy = true;
     }
 
-    void collect(flyg_batch_s& B, std::vector<Pending*> const& work, std::vector<SearchOut> const& outs,
-                 std::vector<Region> const& regions, std::vector<Pending>& requeue, double skin_cap) {
-      long ns = static_cast<long>(work.size());
-      long n3 = 0;
-      for (long k = 0; k < ns; ++k) n3 += 3L * regions[k].nm;
-      std::vector<double> ox(n3), oax(n3);
-      hipStream_t st = B.ctx->stream;
-      if (n3) {
-        HIPCHK(hipMemcpyAsync(ox.data(), B.d_ox.get(), n3 * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(oax.data(), B.d_oax.get(), n3 * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-      }
+    // After F finished: finished searches go to `done`, searches that left their region (and may
+    // still grow it) come back in `requeue` with a larger skin.
+    template <typename Done>
+    void flight_harvest(Flight& F, double rc, Done&& done, std::vector<Pending>& requeue) {
+      SearchOut const* outs = reinterpret_cast<SearchOut const*>(F.h_out.p);
+      double const* ox = reinterpret_cast<double const*>(F.h_out.p + (F.o_ox - F.o_out));
+      double const* oax = reinterpret_cast<double const*>(F.h_out.p + (F.o_oax - F.o_out));
       long off = 0;
-      for (long k = 0; k < ns; ++k) {
-        Pending& P = *work[k];
+      for (long k = 0; k < static_cast<long>(F.work.size()); ++k) {
+        Pending& P = F.work[k];
         SearchOut const& o = outs[k];
-        int nm = regions[k].nm;
+        int nm = P.region->nm;
         ++P.launches;
-        if (o.exit_code == EXIT_REGION && P.skin < skin_cap) {
-          Pending q = P;
-          q.skin = std::min(skin_cap, std::max(P.skin * 1.5, 2.0 * o.max_disp + 0.5));
+        double cap = P.sys->skin_cap(rc);
+        if (o.exit_code == EXIT_REGION && P.skin < cap) {
+          Pending q = std::move(P);
+          q.skin = std::min(cap, std::max(q.skin * 1.5, 2.0 * o.max_disp + 0.5));
+          q.region.reset();
           requeue.push_back(std::move(q));
           off += 3L * nm;
           continue;
         }
-        auto& R = B.results[P.id];
+        Result R;
         R.done = 1;
         R.st.exit_code = o.exit_code;
         R.st.steps = o.steps;
@@ -1056,43 +1260,177 @@ namespace flyg {
         R.st.max_disp = o.max_disp;
         R.st.launches = P.launches;
         R.mobile = P.mobile;
-        R.x.assign(ox.begin() + off, ox.begin() + off + 3L * nm);
-        R.ax.assign(oax.begin() + off, oax.begin() + off + 3L * nm);
-        R.nm = nm;
-        R.nr = regions[k].nr;
-        R.max_disp = o.max_disp;
+        R.x.assign(ox + off, ox + off + 3L * nm);
+        R.ax.assign(oax + off, oax + off + 3L * nm);
+        R.nr = P.region->nr;
         off += 3L * nm;
+        done(P, std::move(R));
       }
+      F.work.clear();
+      F.busy = false;
     }
 
-    void autotune(flyg_batch_s& B) {
-      if (B.bopt.block_threads > 0 && B.bopt.lanes_per_atom > 0) {
-        B.tuned_T = B.bopt.block_threads;
-        B.tuned_L = B.bopt.lanes_per_atom;
-        return;
-      }
-      if (B.tuned_T) return;
-      // Time a short prefix (20 dimer steps) of up to 64 queued searches with each candidate shape.
-      std::vector<Pending*> work;
-      for (auto& P : B.pending) {
+    // Times a 20-step prefix of up to 64 searches with each candidate shape on F (synchronously).
+    void autotune_shape(flyg_eam_s& E, Flight& F, std::vector<Pending> const& queue, DimOpt d, bool mix, int slots,
+                        int& T_out, int& L_out) {
+      std::vector<Pending> work;
+      for (auto const& P : queue) {
         if (work.size() >= 64) break;
-        work.push_back(&P);
+        work.push_back(P);
       }
       if (work.empty()) return;
-      DimOpt d = dim_opt(B.opt, B.ropt);
+      ensure_regions(work, E.tab.rc);
       d.max_steps = std::min(d.max_steps, 20);
       int const cand[][2] = {{256, 4}, {256, 8}, {256, 16}, {512, 8}, {512, 16}, {128, 8}};
       double best = 1e300;
-      std::vector<SearchOut> outs;
-      std::vector<Region> regions;
       for (auto const& c : cand) {
-        launch(B, work, c[0], c[1], d, outs, regions);
-        if (B.last_kernel_ms < best) {
-          best = B.last_kernel_ms;
-          B.tuned_T = c[0];
-          B.tuned_L = c[1];
+        std::vector<Pending> w = work;
+        flight_launch(E, F, std::move(w), c[0], c[1], d, mix, slots);
+        HIPCHK(hipEventSynchronize(F.e_done));
+        double ms = F.kernel_ms();
+        F.work.clear();
+        F.busy = false;
+        if (ms < best) {
+          best = ms;
+          T_out = c[0];
+          L_out = c[1];
         }
       }
+    }
+
+    void fill_positions(SysState const& S, Result const& R, double* pos, double* axis) {
+      if (pos) {
+        std::memcpy(pos, S.base.data(), 3 * S.n * sizeof(double));
+        for (std::size_t i = 0; i < R.mobile.size() && 3 * i + 2 < R.x.size(); ++i)
+          for (int c = 0; c < 3; ++c) pos[3 * R.mobile[i] + c] = R.x[3 * i + c];
+      }
+      if (axis) {
+        std::memset(axis, 0, 3 * S.n * sizeof(double));
+        for (std::size_t i = 0; i < R.mobile.size() && 3 * i + 2 < R.ax.size(); ++i)
+          for (int c = 0; c < 3; ++c) axis[3 * R.mobile[i] + c] = R.ax[3 * i + c];
+      }
+    }
+
+  }  // namespace
+}  // namespace flyg
+
+struct flyg_batch_s {
+  flyg_ctx_s* ctx = nullptr;
+  flyg_eam_s* eam = nullptr;
+  flyg_dimer_options opt;
+  flyg_rotor_options ropt;
+  flyg_batch_options bopt;
+  std::unique_ptr<flyg::SysState> sys;
+  std::vector<flyg::Pending> pending;
+  std::vector<flyg::Result> results;
+  int tuned_T = 0, tuned_L = 0;
+  flyg::Flight flight;
+};
+
+struct flyg_server_s {
+  flyg_ctx_s* ctx = nullptr;
+  flyg_eam_s* eam = nullptr;
+  flyg_dimer_options opt;
+  flyg_rotor_options ropt;
+  flyg_server_options sopt;
+  std::map<int64_t, std::unique_ptr<flyg::SysState>> systems;
+  int64_t next_sys = 0;
+  std::deque<flyg::Pending> queue;
+  std::vector<std::unique_ptr<flyg::Flight>> flights;
+  std::map<int64_t, flyg::Result> results;
+  std::map<int64_t, int64_t> owner;            // search id -> system id
+  int64_t next_id = 0;
+  std::deque<int64_t> completed;
+  std::size_t budget = 0;
+  int tuned_T = 0, tuned_L = 0;
+  int64_t launches = 0, searches_done = 0;
+  double kernel_ms = 0;
+
+  std::size_t used() const {
+    std::size_t b = 0;
+    for (auto const& kv : systems) b += kv.second->dev_bytes();
+    for (auto const& f : flights) b += f->dev_bytes();
+    return b;
+  }
+};
+
+namespace flyg {
+  namespace {
+
+    int slots_of(flyg_ctx_s const& ctx, int max_concurrent) { return max_concurrent > 0 ? max_concurrent : 2 * ctx.cus; }
+
+    // Harvests finished flights, frees idle retired systems, and launches queued searches on free
+    // flights (as many as the memory budget allows, FIFO).  Returns how many searches finished.
+    long server_step(flyg_server_s& S) {
+      long fin = 0;
+      double const rc = S.eam->tab.rc;
+      for (auto& fp : S.flights) {
+        Flight& F = *fp;
+        if (!F.busy || !F.finished()) continue;
+        S.kernel_ms += F.kernel_ms();
+        std::vector<Pending> requeue;
+        flight_harvest(F, rc, [&](Pending& P, Result&& R) {
+          --P.sys->active;
+          S.results[P.id] = std::move(R);
+          S.completed.push_back(P.id);
+          ++S.searches_done;
+          ++fin;
+        }, requeue);
+        for (auto it = requeue.rbegin(); it != requeue.rend(); ++it) S.queue.push_front(std::move(*it));
+      }
+      for (auto it = S.systems.begin(); it != S.systems.end();) {
+        if (it->second->retired && it->second->active == 0) {
+          it = S.systems.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      if (S.queue.empty()) return fin;
+      DimOpt d = dim_opt(S.opt, S.ropt);
+      bool mix = S.sopt.precision == FLYG_MIXED;
+      int slots = slots_of(*S.ctx, S.sopt.max_concurrent);
+      if (!S.tuned_T) {
+        if (S.sopt.block_threads > 0 && S.sopt.lanes_per_atom > 0) {
+          S.tuned_T = S.sopt.block_threads;
+          S.tuned_L = S.sopt.lanes_per_atom;
+        } else {
+          for (auto& fp : S.flights) {
+            if (fp->busy) continue;
+            std::vector<Pending> q(S.queue.begin(), S.queue.end());
+            autotune_shape(*S.eam, *fp, q, d, mix, slots, S.tuned_T, S.tuned_L);
+            break;
+          }
+          if (!S.tuned_T) return fin;
+        }
+      }
+      int const cap = S.sopt.max_batch > 0 ? S.sopt.max_batch : 4 * S.ctx->cus;
+      for (auto& fp : S.flights) {
+        if (S.queue.empty()) break;
+        Flight& F = *fp;
+        if (F.busy) continue;
+        std::vector<Pending> work;
+        while (!S.queue.empty() && static_cast<int>(work.size()) < cap) {
+          work.push_back(std::move(S.queue.front()));
+          S.queue.pop_front();
+        }
+        ensure_regions(work, rc);
+        // Memory gate: keep the prefix that fits beside everything resident (at least one search when
+        // nothing else is in flight, so the queue always drains).
+        std::size_t other = S.used() - F.dev_bytes();
+        bool alone = true;
+        for (auto const& g : S.flights)
+          if (g->busy) alone = false;
+        while (work.size() > 1 || (!alone && !work.empty())) {
+          std::size_t need = std::max(flight_bytes(work, d), F.dev_bytes());
+          if (other + need <= S.budget) break;
+          S.queue.push_front(std::move(work.back()));
+          work.pop_back();
+        }
+        if (work.empty()) break;
+        flight_launch(*S.eam, F, std::move(work), S.tuned_T, S.tuned_L, d, mix, slots);
+        ++S.launches;
+      }
+      return fin;
     }
 
   }  // namespace
@@ -1116,6 +1454,21 @@ namespace {
       return FLYG_INVALID_ARG;
     }
   }
+
+  void take_options(flyg_dimer_options const* opt, flyg_rotor_options const* ropt, flyg_dimer_options& o,
+                    flyg_rotor_options& r) {
+    if (opt) {
+      o = *opt;
+    } else {
+      flyg_dimer_default_options(&o);
+    }
+    if (ropt) {
+      r = *ropt;
+    } else {
+      flyg_rotor_default_options(&r);
+    }
+    require(o.n > 0 && o.n <= 32 && r.n > 0 && r.n <= 32, "batched searches support L-BFGS histories up to 32");
+  }
 }  // namespace
 
 extern "C" {
@@ -1131,61 +1484,16 @@ int flyg_batch_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt,
     auto b = std::make_unique<flyg_batch_s>();
     b->ctx = ctx;
     b->eam = eam;
-    if (opt) {
-      b->opt = *opt;
-    } else {
-      flyg_dimer_default_options(&b->opt);
-    }
-    if (ropt) {
-      b->ropt = *ropt;
-    } else {
-      flyg_rotor_default_options(&b->ropt);
-    }
+    take_options(opt, ropt, b->opt, b->ropt);
     if (bopt) {
       b->bopt = *bopt;
     } else {
       flyg_batch_default_options(&b->bopt);
     }
     if (b->bopt.list_skin <= 0) b->bopt.list_skin = 2.0;
-    require(b->opt.n > 0 && b->opt.n <= 32 && b->ropt.n > 0 && b->ropt.n <= 32,
-            "batched searches support L-BFGS histories up to 32");
     require(b->bopt.precision == FLYG_MIXED || b->bopt.precision == FLYG_FP64, "bad precision");
-    std::memcpy(b->basis, basis, sizeof(b->basis));
-    std::memcpy(b->periodic, periodic, sizeof(b->periodic));
-    b->box = make_box(basis, periodic);
-    require(b->box.ortho && periodic[0] && periodic[1] && periodic[2],
-            "batched searches need a fully periodic orthorhombic box");
-    b->n = n;
-    b->base.resize(3 * n);
-    b->types.assign(types, types + n);
-    for (long i = 0; i < n; ++i) {
-      require(types[i] < static_cast<uint32_t>(eam->tab.ntypes), "TypeID out of range");
-      for (int k = 0; k < 3; ++k) {
-        // Strictly inside [0, L): the canonical image of a coordinate just below 0 rounds to L, and the
-        // cell grid and the image shifts must agree on which side of the box an atom is.
-        double x = min_pos[3 * i + k];
-        double w = x - b->box.ext[k] * std::floor(x * b->box.inv_ext[k]);
-        if (w >= b->box.ext[k]) w -= b->box.ext[k];
-        if (w < 0) w = 0;
-        b->base[3 * i + k] = w;
-      }
-    }
-    double rl = eam->tab.rc + b->bopt.list_skin;
-    for (int k = 0; k < 3; ++k) require(b->box.ext[k] > 2.0 * rl, "box too small for the search region cut-off");
-    b->grid.build(b->base.data(), n, b->box.ext, rl);
-    // Global FP64 base densities.
-    auto nl = make_list(ctx, basis, periodic, eam->tab.rc);
-    DBuf<double> dpos;
-    dpos.reserve(3 * n);
-    DBuf<uint32_t> dt;
-    dt.reserve(n);
-    HIPCHK(hipMemcpyAsync(dpos.get(), b->base.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(dt.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-    list_rebuild(*nl, dpos.get(), n);
-    b->rho_base.reserve(n);
-    eam_density(*eam, *nl, dt.get(), b->rho_base.get());
-    unsigned fl = ctx->take_flags();
-    if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+    b->sys = make_system(ctx, eam, basis, periodic, min_pos, types, n, b->bopt.list_skin);
+    b->flight.init(ctx->stream, false);
     ctx->retain();
     *out = b.release();
   });
@@ -1205,33 +1513,14 @@ int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const doub
                       const double* start_axis, int64_t* ids) {
   return bguard(b->ctx, [&] {
     require(count >= 0 && centre && radius && start_pos && start_axis, "null argument");
-    long const n = b->n;
+    long const n = b->sys->n;
     for (int k = 0; k < count; ++k) {
-      require(centre[k] >= 0 && centre[k] < n, "centre out of range");
-      require(radius[k] > 0 && radius[k] < 0.5 * std::fmin(b->box.ext[0], std::fmin(b->box.ext[1], b->box.ext[2])) - 1.0,
-              "active radius must be positive and below half the box");
-      Pending P;
-      P.id = static_cast<int64_t>(b->results.size());
-      P.centre = centre[k];
-      P.radius = radius[k];
-      P.skin = b->bopt.list_skin;
-      mobile_set(*b, centre[k], radius[k], P.mobile);
-      int nm = static_cast<int>(P.mobile.size());
-      P.x.resize(3 * nm);
-      P.ax.resize(3 * nm);
-      double const* sp = start_pos + 3 * n * static_cast<long>(k);
-      double const* sa = start_axis + 3 * n * static_cast<long>(k);
-      for (int i = 0; i < nm; ++i) {
-        int g = P.mobile[i];
-        for (int c = 0; c < 3; ++c) {
-          double bse = b->base[3 * g + c];
-          P.x[3 * i + c] = bse + min_image(sp[3 * g + c] - bse, b->box.ext[c]);
-          P.ax[3 * i + c] = sa[3 * g + c];
-        }
-      }
+      int64_t id = static_cast<int64_t>(b->results.size());
+      b->pending.push_back(make_pending(*b->sys, id, centre[k], radius[k], b->bopt.list_skin,
+                                        start_pos + 3 * n * static_cast<long>(k),
+                                        start_axis + 3 * n * static_cast<long>(k), nullptr, 0));
       b->results.emplace_back();
-      if (ids) ids[k] = P.id;
-      b->pending.push_back(std::move(P));
+      if (ids) ids[k] = id;
     }
   });
 }
@@ -1239,20 +1528,23 @@ int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const doub
 int flyg_batch_run(flyg_batch b) {
   return bguard(b->ctx, [&] {
     if (b->pending.empty()) return;
-    autotune(*b);
     DimOpt d = dim_opt(b->opt, b->ropt);
-    double cap = 0.5 * std::fmin(b->box.ext[0], std::fmin(b->box.ext[1], b->box.ext[2])) - b->eam->tab.rc;
+    bool mix = b->bopt.precision == FLYG_MIXED;
+    int slots = slots_of(*b->ctx, b->bopt.max_concurrent);
+    if (b->bopt.block_threads > 0 && b->bopt.lanes_per_atom > 0) {
+      b->tuned_T = b->bopt.block_threads;
+      b->tuned_L = b->bopt.lanes_per_atom;
+    } else if (!b->tuned_T) {
+      autotune_shape(*b->eam, b->flight, b->pending, d, mix, slots, b->tuned_T, b->tuned_L);
+    }
     std::vector<Pending> todo = std::move(b->pending);
     b->pending.clear();
     for (int round = 0; round < 6 && !todo.empty(); ++round) {
-      std::vector<Pending*> work;
-      for (auto& P : todo) work.push_back(&P);
-      std::vector<SearchOut> outs;
-      std::vector<Region> regions;
-      launch(*b, work, b->tuned_T, b->tuned_L, d, outs, regions);
-      std::vector<Pending> requeue;
-      collect(*b, work, outs, regions, requeue, cap);
-      todo = std::move(requeue);
+      ensure_regions(todo, b->eam->tab.rc);
+      flight_launch(*b->eam, b->flight, std::move(todo), b->tuned_T, b->tuned_L, d, mix, slots);
+      HIPCHK(hipEventSynchronize(b->flight.e_done));
+      todo.clear();
+      flight_harvest(b->flight, b->eam->tab.rc, [&](Pending& P, Result&& R) { b->results[P.id] = std::move(R); }, todo);
     }
     for (auto& P : todo) {   // still outside the largest region the box allows
       auto& R = b->results[P.id];
@@ -1270,16 +1562,7 @@ int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double
     auto const& R = b->results[id];
     require(R.done, "search has not run yet");
     if (stats) *stats = R.st;
-    if (pos) {
-      std::memcpy(pos, b->base.data(), 3 * b->n * sizeof(double));
-      for (std::size_t i = 0; i < R.mobile.size() && 3 * i + 2 < R.x.size(); ++i)
-        for (int c = 0; c < 3; ++c) pos[3 * R.mobile[i] + c] = R.x[3 * i + c];
-    }
-    if (axis) {
-      std::memset(axis, 0, 3 * b->n * sizeof(double));
-      for (std::size_t i = 0; i < R.mobile.size() && 3 * i + 2 < R.ax.size(); ++i)
-        for (int c = 0; c < 3; ++c) axis[3 * R.mobile[i] + c] = R.ax[3 * i + c];
-    }
+    fill_positions(*b->sys, R, pos, axis);
   });
 }
 
@@ -1289,6 +1572,186 @@ int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region) {
     auto const& R = b->results[id];
     if (n_mobile) *n_mobile = static_cast<int>(R.mobile.size());
     if (n_region) *n_region = R.nr;
+  });
+}
+
+// ------------------------------------------------------------------------------------------------
+// Serving
+// ------------------------------------------------------------------------------------------------
+
+void flyg_server_default_options(flyg_server_options* o) {
+  *o = flyg_server_options{FLYG_MIXED, 0, 0, 2.0, 0, 4, 0, 0};
+}
+
+int flyg_server_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt, const flyg_rotor_options* ropt,
+                       const flyg_server_options* sopt, flyg_server* out) {
+  return bguard(ctx, [&] {
+    require(ctx && eam && out, "null argument");
+    require(eam->ctx == ctx, "potential belongs to another context");
+    auto s = std::make_unique<flyg_server_s>();
+    s->ctx = ctx;
+    s->eam = eam;
+    take_options(opt, ropt, s->opt, s->ropt);
+    if (sopt) {
+      s->sopt = *sopt;
+    } else {
+      flyg_server_default_options(&s->sopt);
+    }
+    if (s->sopt.list_skin <= 0) s->sopt.list_skin = 2.0;
+    if (s->sopt.streams <= 0) s->sopt.streams = 4;
+    require(s->sopt.streams <= 32, "at most 32 streams");
+    require(s->sopt.precision == FLYG_MIXED || s->sopt.precision == FLYG_FP64, "bad precision");
+    if (s->sopt.mem_budget > 0) {
+      s->budget = static_cast<std::size_t>(s->sopt.mem_budget);
+    } else {
+      std::size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      s->budget = fr / 10 * 8;
+    }
+    for (int k = 0; k < s->sopt.streams; ++k) {
+      hipStream_t st;
+      HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      auto f = std::make_unique<Flight>();
+      f->init(st, true);
+      s->flights.push_back(std::move(f));
+    }
+    ctx->retain();
+    *out = s.release();
+  });
+}
+
+int flyg_server_destroy(flyg_server s) {
+  if (!s) return FLYG_OK;
+  flyg_ctx_s* ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  delete s;   // flights synchronise their streams first
+  ctx->release();
+  return FLYG_OK;
+}
+
+int flyg_server_add_system(flyg_server s, const double basis[9], const int periodic[3], const double* min_pos,
+                           const uint32_t* types, int64_t n, int64_t* sys_id) {
+  return bguard(s->ctx, [&] {
+    require(basis && periodic && min_pos && types && sys_id && n > 0, "null or empty argument");
+    std::size_t need = sys_bytes_estimate(n) + sys_transient_estimate(n);
+    if (s->used() + need > s->budget) {   // idle launch buffers are the first to go
+      for (auto& f : s->flights)
+        if (!f->busy) f->release_memory();
+    }
+    if (s->used() + need > s->budget) {
+      throw Error(FLYG_BUSY, "device memory budget full: admit the system after running searches finish");
+    }
+    auto sys = make_system(s->ctx, s->eam, basis, periodic, min_pos, types, n, s->sopt.list_skin);
+    sys->id = s->next_sys++;
+    *sys_id = sys->id;
+    s->systems[sys->id] = std::move(sys);
+  });
+}
+
+int flyg_server_remove_system(flyg_server s, int64_t sys_id) {
+  return bguard(s->ctx, [&] {
+    auto it = s->systems.find(sys_id);
+    require(it != s->systems.end() && !it->second->retired, "unknown system id");
+    it->second->retired = true;
+    if (it->second->active == 0) s->systems.erase(it);
+  });
+}
+
+int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* centre, const double* radius,
+                       const int64_t* kick_offsets, const int64_t* kick_atoms, const double* kick_pos,
+                       const double* kick_axis, int64_t* ids) {
+  return bguard(s->ctx, [&] {
+    auto it = s->systems.find(sys_id);
+    require(it != s->systems.end() && !it->second->retired, "unknown system id");
+    require(count >= 0 && centre && radius && kick_offsets && (kick_offsets[count] == 0 || (kick_atoms && kick_pos && kick_axis)),
+            "null argument");
+    SysState& S = *it->second;
+    std::vector<Pending> add;
+    for (int k = 0; k < count; ++k) {
+      long a = kick_offsets[k], b = kick_offsets[k + 1];
+      require(a >= 0 && b >= a, "kick offsets must be non-decreasing");
+      add.push_back(make_pending(S, s->next_id + k, centre[k], radius[k], s->sopt.list_skin, kick_pos + 3 * a,
+                                 kick_axis + 3 * a, kick_atoms + a, b - a));
+    }
+    for (int k = 0; k < count; ++k) {
+      int64_t id = s->next_id++;
+      s->owner[id] = sys_id;
+      if (ids) ids[k] = id;
+      s->queue.push_back(std::move(add[k]));
+      ++S.active;
+    }
+  });
+}
+
+int flyg_server_progress(flyg_server s, double wait_ms, int64_t* done_ids, int cap, int* n_done) {
+  return bguard(s->ctx, [&] {
+    require(n_done && (cap == 0 || done_ids), "null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    while (true) {
+      server_step(*s);
+      if (!s->completed.empty() || wait_ms <= 0) break;
+      bool busy = false;
+      for (auto const& f : s->flights) busy = busy || f->busy;
+      if (!busy && s->queue.empty()) break;
+      double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (el >= wait_ms) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    int k = 0;
+    while (k < cap && !s->completed.empty()) {
+      done_ids[k++] = s->completed.front();
+      s->completed.pop_front();
+    }
+    *n_done = k;
+  });
+}
+
+int flyg_server_result(flyg_server s, int64_t id, flyg_search_stats* stats, int* n_mobile, int64_t* atoms, double* pos,
+                       double* axis) {
+  return bguard(s->ctx, [&] {
+    auto it = s->results.find(id);
+    require(it != s->results.end() && it->second.done, "search has not finished (or unknown id)");
+    Result const& R = it->second;
+    if (stats) *stats = R.st;
+    if (n_mobile) *n_mobile = static_cast<int>(R.mobile.size());
+    for (std::size_t i = 0; i < R.mobile.size(); ++i) {
+      if (atoms) atoms[i] = R.mobile[i];
+      for (int c = 0; c < 3; ++c) {
+        if (pos && 3 * i + c < R.x.size()) pos[3 * i + c] = R.x[3 * i + c];
+        if (axis && 3 * i + c < R.ax.size()) axis[3 * i + c] = R.ax[3 * i + c];
+      }
+    }
+  });
+}
+
+int flyg_server_release(flyg_server s, int64_t id) {
+  return bguard(s->ctx, [&] {
+    s->results.erase(id);
+    s->owner.erase(id);
+  });
+}
+
+int flyg_server_status(flyg_server s, flyg_server_state* st) {
+  return bguard(s->ctx, [&] {
+    require(st, "null argument");
+    st->queued = static_cast<int64_t>(s->queue.size());
+    int64_t infl = 0;
+    int busy = 0;
+    for (auto const& f : s->flights)
+      if (f->busy) {
+        infl += static_cast<int64_t>(f->work.size());
+        ++busy;
+      }
+    st->in_flight = infl;
+    st->busy_streams = busy;
+    st->systems = static_cast<int64_t>(s->systems.size());
+    st->device_bytes = static_cast<int64_t>(s->used());
+    st->budget_bytes = static_cast<int64_t>(s->budget);
+    st->launches = s->launches;
+    st->searches_done = s->searches_done;
+    st->kernel_ms = s->kernel_ms;
+    st->block_threads = s->tuned_T;
+    st->lanes_per_atom = s->tuned_L;
   });
 }
 

@@ -222,6 +222,12 @@ namespace flyg {
           if (q == m % NS) v[q] = __dadd_rn(v[q], x);
       }
     }
+    // Add to slot q (q < NS; a compile-time constant at the call sites, so the selects fold away).
+    __device__ void add_slot(int q, double x) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (s == q) v[s] = __dadd_rn(v[s], x);
+    }
     __device__ double total() {
 #pragma unroll
       for (int bit = 8; bit >= 1; bit >>= 1) {

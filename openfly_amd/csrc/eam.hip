@@ -51,19 +51,45 @@ namespace flyg {
     __device__ inline PairGeo geo(Shifts const& S, double4 const* __restrict__ pos, unsigned en, double4 const& pi) {
       PairGeo p;
       p.j = en & NB_IDX_MASK;
-      double3 sh = S.of(en >> NB_SHIFT_BITS);
+      unsigned const code = en >> NB_SHIFT_BITS;
       double4 pj = pos[p.j];
-      p.dx = (pj.x + sh.x) - pi.x;
-      p.dy = (pj.y + sh.y) - pi.y;
-      p.dz = (pj.z + sh.z) - pi.z;
+      if (code != 13u) {   // image of a periodic neighbour (rows of interior atoms never take this branch)
+        double3 sh = S.of(code);
+        pj.x += sh.x;
+        pj.y += sh.y;
+        pj.z += sh.z;
+      }
+      p.dx = pj.x - pi.x;
+      p.dy = pj.y - pi.y;
+      p.dz = pj.z - pi.z;
       p.r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
       p.w = pj.w;
       return p;
     }
 
-    // Density pass.  Each lane walks its share of the row two entries at a time so that two position
-    // gathers (and then two table gathers) are in flight per lane.
-    template <int L, bool MULTI, bool ENERGY>
+    // Walks lane `lane`'s share of a row (entries k = lane + m L): NS = 16 / L entries per step, entry m
+    // of the step going to row-sum slot m (compile time), so every lane-count gives the same sums; for
+    // L = 16 two entries per step share slot 0 in order.  All loads of a step are issued first.
+    template <int L, typename F>
+    __device__ inline void walk_row(unsigned const* __restrict__ row, int c, int lane, Shifts const& S,
+                                    double4 const* __restrict__ pos, double4 const& pi, F&& one) {
+      constexpr int NS = 16 / L;
+      constexpr int U = NS == 1 ? 2 : NS;
+      int m = 0;
+      for (; lane + (m + U - 1) * L < c; m += U) {
+        PairGeo p[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) p[q] = geo(S, pos, row[lane + (m + q) * L], pi);
+#pragma unroll
+        for (int q = 0; q < U; ++q) one(q % NS, p[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < U - 1; ++q)
+        if (lane + (m + q) * L < c) one(q % NS, geo(S, pos, row[lane + (m + q) * L], pi));
+    }
+
+    // Density pass.  MIX (and not ENERGY): r and the density spline in FP32, sums in FP64.
+    template <int L, bool MULTI, bool ENERGY, bool MIX>
     __global__ __launch_bounds__(NT) void k_density(EAMTables T, Shifts S, PairArgs A, double4* __restrict__ posf,
                                                     double* __restrict__ eatom, unsigned* __restrict__ err) {
       long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
@@ -79,22 +105,20 @@ namespace flyg {
         ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
-        auto one = [&](int m, PairGeo const& p) {
+        walk_row<L>(row, c, lane, S, A.pos, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            double r = sqrt(p.r2);
             int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
-            rho.add(m, spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e));
-            if (ENERGY) vs.add(m, spl64<0>(T, T.V[vidx(T, ti, tj)], r, e));
+            if (MIX && !ENERGY) {
+              float r2f = static_cast<float>(p.r2);
+              float rf = r2f * __frsqrt_rn(r2f);
+              rho.add_slot(q, static_cast<double>(spl32f<0>(T, T.phi[tj + T.ntypes * ti], rf, e)));
+            } else {
+              double r = sqrt(p.r2);
+              rho.add_slot(q, spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e));
+              if (ENERGY) vs.add_slot(q, spl64<0>(T, T.V[vidx(T, ti, tj)], r, e));
+            }
           }
-        };
-        int m = 0;
-        for (; lane + (m + 1) * L < c; m += 2) {
-          unsigned e0 = row[lane + m * L], e1 = row[lane + (m + 1) * L];
-          PairGeo p0 = geo(S, A.pos, e0, pi), p1 = geo(S, A.pos, e1, pi);
-          one(m, p0);
-          one(m + 1, p1);
-        }
-        if (lane + m * L < c) one(m, geo(S, A.pos, row[lane + m * L], pi));
+        });
       }
       double rt = rho.total();
       double vt = ENERGY ? vs.total() : 0.0;
@@ -122,17 +146,19 @@ namespace flyg {
         int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
-        auto one = [&](int m, PairGeo const& p) {
+        walk_row<L>(row, c, lane, S, posf, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
             int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
             double cf;
             if (MIX) {
-              float rf = sqrtf(static_cast<float>(p.r2));
+              float r2f = static_cast<float>(p.r2);
+              float rinv = __frsqrt_rn(r2f);
+              float rf = r2f * rinv;
               float vp = spl32f<1>(T, T.V[vidx(T, ti, tj)], rf, e);
               float pa = spl32f<1>(T, T.phi[tj + T.ntypes * ti], rf, e);
               float pb = MULTI ? spl32f<1>(T, T.phi[ti + T.ntypes * tj], rf, e) : pa;
               float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(p.w) * pb;
-              cf = static_cast<double>(mag / rf);
+              cf = static_cast<double>(mag * rinv);
             } else {
               double r = sqrt(p.r2);
               double vp = spl64<1>(T, T.V[vidx(T, ti, tj)], r, e);
@@ -140,19 +166,11 @@ namespace flyg {
               double pb = MULTI ? spl64<1>(T, T.phi[ti + T.ntypes * tj], r, e) : pa;
               cf = (vp + pi.w * pa + p.w * pb) / r;
             }
-            gx.add(m, -cf * p.dx);
-            gy.add(m, -cf * p.dy);
-            gz.add(m, -cf * p.dz);
+            gx.add_slot(q, -cf * p.dx);
+            gy.add_slot(q, -cf * p.dy);
+            gz.add_slot(q, -cf * p.dz);
           }
-        };
-        int m = 0;
-        for (; lane + (m + 1) * L < c; m += 2) {
-          unsigned e0 = row[lane + m * L], e1 = row[lane + (m + 1) * L];
-          PairGeo p0 = geo(S, posf, e0, pi), p1 = geo(S, posf, e1, pi);
-          one(m, p0);
-          one(m + 1, p1);
-        }
-        if (lane + m * L < c) one(m, geo(S, posf, row[lane + m * L], pi));
+        });
       }
       double tx = gx.total(), ty = gy.total(), tz = gz.total();
       if (L >= 3 && inr && lane < 3) {
@@ -378,22 +396,30 @@ namespace flyg {
   }
 
   namespace {
+    // energy = true: FP64 densities and pair energies (energies are always FP64); otherwise the density
+    // pass of a gradient in the given precision.
     template <int L>
-    void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy) {
+    void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy, int precision = FLYG_FP64) {
       hipStream_t st = E.ctx->stream;
       unsigned* err = E.ctx->dflags.get();
       bool multi = E.tab.ntypes > 1;
+      bool mix = precision == FLYG_MIXED;
       unsigned g = grid_for(A.n, L);
+      double4* pf = E.posf.get();
       if (multi) {
         if (energy)
-          k_density<L, true, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), E.eatom.get(), err);
+          k_density<L, true, true, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, E.eatom.get(), err);
+        else if (mix)
+          k_density<L, true, false, true><<<g, NT, 0, st>>>(E.tab, S, A, pf, nullptr, err);
         else
-          k_density<L, true, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), nullptr, err);
+          k_density<L, true, false, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, nullptr, err);
       } else {
         if (energy)
-          k_density<L, false, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), E.eatom.get(), err);
+          k_density<L, false, true, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, E.eatom.get(), err);
+        else if (mix)
+          k_density<L, false, false, true><<<g, NT, 0, st>>>(E.tab, S, A, pf, nullptr, err);
         else
-          k_density<L, false, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), nullptr, err);
+          k_density<L, false, false, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, nullptr, err);
       }
     }
 
@@ -436,15 +462,15 @@ namespace flyg {
     void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L) {
       switch (L) {
         case 4:
-          launch_density<4>(E, S, A, false);
+          launch_density<4>(E, S, A, false, precision);
           launch_force<4>(E, S, A, d_grad, precision);
           break;
         case 16:
-          launch_density<16>(E, S, A, false);
+          launch_density<16>(E, S, A, false, precision);
           launch_force<16>(E, S, A, d_grad, precision);
           break;
         default:
-          launch_density<8>(E, S, A, false);
+          launch_density<8>(E, S, A, false, precision);
           launch_force<8>(E, S, A, d_grad, precision);
       }
     }

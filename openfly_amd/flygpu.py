@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libflygpu.so")
 
-FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR = 0, 1, 2, 3
+FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR, FLYG_BUSY = 0, 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 FP64, MIXED = 0, 1
 EXIT_NAMES = {0: "success", 1: "convex", 2: "iter_max", 3: "collision", 4: "uninit"}
@@ -39,6 +39,10 @@ class SplineOutOfBounds(FlyError):
 
 class DeviceError(FlyError):
     pass
+
+
+class Busy(FlyError):
+    """Serving: the device memory budget cannot take this now (retry after searches finish)."""
 
 
 _d = C.POINTER(C.c_double)
@@ -79,6 +83,22 @@ class SearchStats(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("block_threads", C.c_int), ("lanes_per_atom", C.c_int),
                 ("list_skin", C.c_double), ("max_concurrent", C.c_int)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("block_threads", C.c_int), ("lanes_per_atom", C.c_int),
+                ("list_skin", C.c_double), ("max_concurrent", C.c_int), ("streams", C.c_int), ("max_batch", C.c_int),
+                ("mem_budget", C.c_int64)]
+
+
+class ServerState(C.Structure):
+    _fields_ = [("queued", C.c_int64), ("in_flight", C.c_int64), ("systems", C.c_int64), ("device_bytes", C.c_int64),
+                ("budget_bytes", C.c_int64), ("launches", C.c_int64), ("searches_done", C.c_int64),
+                ("kernel_ms", C.c_double), ("busy_streams", C.c_int), ("block_threads", C.c_int),
+                ("lanes_per_atom", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _SIGS = {
@@ -131,9 +151,20 @@ _SIGS = {
     "flyg_batch_run": [_vp],
     "flyg_batch_result": [_vp, C.c_int64, C.POINTER(SearchStats), _d, _d],
     "flyg_batch_region": [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "flyg_server_default_options": [C.POINTER(ServerOptions)],
+    "flyg_server_create": [_vp, _vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), C.POINTER(ServerOptions),
+                           C.POINTER(_vp)],
+    "flyg_server_destroy": [_vp],
+    "flyg_server_add_system": [_vp, _d, C.POINTER(C.c_int), _d, _u32, C.c_int64, _i64],
+    "flyg_server_remove_system": [_vp, C.c_int64],
+    "flyg_server_submit": [_vp, C.c_int64, C.c_int, _i64, _d, _i64, _i64, _d, _d, _i64],
+    "flyg_server_progress": [_vp, C.c_double, _i64, C.c_int, C.POINTER(C.c_int)],
+    "flyg_server_result": [_vp, C.c_int64, C.POINTER(SearchStats), C.POINTER(C.c_int), _i64, _d, _d],
+    "flyg_server_release": [_vp, C.c_int64],
+    "flyg_server_status": [_vp, C.POINTER(ServerState)],
 }
 _VOID = {"flyg_lbfgs_default_options", "flyg_dimer_default_options", "flyg_rotor_default_options",
-         "flyg_batch_default_options"}
+         "flyg_batch_default_options", "flyg_server_default_options"}
 
 
 def lib():
@@ -167,6 +198,8 @@ def _raise(ctx_h, rc):
         raise SplineOutOfBounds(msg)
     if rc == FLYG_DEVICE_ERROR:
         raise DeviceError(msg)
+    if rc == FLYG_BUSY:
+        raise Busy(msg)
     raise FlyError(msg)
 
 
@@ -543,3 +576,92 @@ class Batch:
         out["n_mobile"] = nm.value
         out["n_region"] = nr.value
         return out
+
+
+def server_options(**kw):
+    return _fill(ServerOptions, lib().flyg_server_default_options, kw)
+
+
+class Server:
+    """Continuous-batching saddle-search server over many resident systems (serving form of
+    Master::find_batch).  Systems are admitted under a device memory budget; searches are queued with
+    sparse start states and packed into launches on several streams as they arrive."""
+
+    def __init__(self, ctx: Context, pot: EAM, dimer=None, rotor=None, **opts):
+        self.ctx = ctx
+        self.pot = pot
+        self.opt = dimer_options(**(dimer or {}))
+        self.ropt = rotor_options(**(rotor or {}))
+        self.sopt = server_options(**opts)
+        h = _vp()
+        ctx.check(lib().flyg_server_create(ctx.h, pot.h, C.byref(self.opt), C.byref(self.ropt), C.byref(self.sopt),
+                                           C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_server_destroy(self.h)
+            self.h = None
+
+    def add_system(self, box: Box, min_pos, types):
+        """System id, or raises Busy when the memory budget cannot hold it now."""
+        pos = np.ascontiguousarray(min_pos, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        sid = C.c_int64()
+        b, p = box.args()
+        self.ctx.check(lib().flyg_server_add_system(self.h, b, p, pos.ctypes.data_as(_d), t.ctypes.data_as(_u32),
+                                                    len(pos), C.byref(sid)))
+        return sid.value
+
+    def remove_system(self, sys_id):
+        self.ctx.check(lib().flyg_server_remove_system(self.h, int(sys_id)))
+
+    def submit(self, sys_id, centres, radii, kicks):
+        """kicks[k] = (atom indices, positions (m, 3), axis (m, 3)) of search k's perturbed atoms."""
+        c = np.ascontiguousarray(np.atleast_1d(centres), dtype=np.int64)
+        r = np.ascontiguousarray(np.atleast_1d(radii), dtype=np.float64)
+        k = len(c)
+        offs = np.zeros(k + 1, np.int64)
+        offs[1:] = np.cumsum([len(kk[0]) for kk in kicks])
+        atoms = np.ascontiguousarray(np.concatenate([np.asarray(kk[0], np.int64) for kk in kicks]) if k else
+                                     np.zeros(0, np.int64), dtype=np.int64)
+        kp = np.ascontiguousarray(np.concatenate([np.asarray(kk[1], np.float64).reshape(-1, 3) for kk in kicks])
+                                  if k else np.zeros((0, 3)), dtype=np.float64)
+        ka = np.ascontiguousarray(np.concatenate([np.asarray(kk[2], np.float64).reshape(-1, 3) for kk in kicks])
+                                  if k else np.zeros((0, 3)), dtype=np.float64)
+        ids = np.zeros(k, np.int64)
+        self.ctx.check(lib().flyg_server_submit(self.h, int(sys_id), k, c.ctypes.data_as(_i64), r.ctypes.data_as(_d),
+                                                offs.ctypes.data_as(_i64), atoms.ctypes.data_as(_i64),
+                                                kp.ctypes.data_as(_d), ka.ctypes.data_as(_d), ids.ctypes.data_as(_i64)))
+        return ids
+
+    def progress(self, wait_ms=0.0, cap=4096):
+        """Ids of searches that finished since the last call (waits up to wait_ms for one)."""
+        buf = np.zeros(cap, np.int64)
+        n = C.c_int()
+        self.ctx.check(lib().flyg_server_progress(self.h, float(wait_ms), buf.ctypes.data_as(_i64), cap, C.byref(n)))
+        return buf[: n.value].copy()
+
+    def result(self, sid, positions=False, release=False):
+        st = SearchStats()
+        nm = C.c_int()
+        self.ctx.check(lib().flyg_server_result(self.h, int(sid), C.byref(st), C.byref(nm), None, None, None))
+        out = st.as_dict()
+        out["barrier"] = st.energy
+        out["exit"] = st.exit_code
+        out["n_mobile"] = nm.value
+        if positions:
+            atoms = np.zeros(nm.value, np.int64)
+            pos = np.zeros((nm.value, 3))
+            ax = np.zeros((nm.value, 3))
+            self.ctx.check(lib().flyg_server_result(self.h, int(sid), None, None, atoms.ctypes.data_as(_i64),
+                                                    pos.ctypes.data_as(_d), ax.ctypes.data_as(_d)))
+            out.update(atoms=atoms, pos=pos, axis=ax)
+        if release:
+            self.ctx.check(lib().flyg_server_release(self.h, int(sid)))
+        return out
+
+    def status(self):
+        st = ServerState()
+        self.ctx.check(lib().flyg_server_status(self.h, C.byref(st)))
+        return st.as_dict()

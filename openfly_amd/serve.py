@@ -1,0 +1,160 @@
+"""Serving loop: saddle-search requests from a simulated kMC loop, served with continuous batching.
+
+A kMC loop (SKMC::skmc, skmc.hpp:240-262) asks Master::find_mechs (find.cpp:211-312) for new mechanisms
+whenever its catalogue misses an environment; each miss becomes a request of 8-64 dimer searches on the
+current system.  Here the requests arrive as a Poisson process; each carries its own system (bcc Fe,
+N log-uniform in [2000, 250000] atoms, 0.5-2 % vacancies + interstitials), which the server admits
+under its device memory budget, keeps resident while the request's searches run, and frees afterwards.
+Searches of requests that arrive while others run join the next launch (flyg_server_*).
+
+Latency of a request = time from its arrival to the completion of its last search.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from . import flygpu as fg
+from . import systems
+
+
+@dataclasses.dataclass
+class Request:
+    t: float                 # arrival time (s after the start)
+    system: systems.System
+    centres: np.ndarray
+    radii: np.ndarray
+    kicks: list              # per search: (atoms, positions (m, 3), axis (m, 3))
+    sys_id: int = -1
+    remaining: int = 0
+    admitted: float = -1.0
+    done: float = -1.0
+    results: dict = dataclasses.field(default_factory=dict)
+
+
+def sample_centres(s: systems.System, k, rng, r=3.0):
+    """k random defect-adjacent atoms: pick a random defect (vacancy site or interstitial), then a random
+    atom within r of it."""
+    L = np.diag(s.basis)
+    pts = []
+    if getattr(s, "vacancies", None) is not None and len(s.vacancies):
+        pts.append(np.asarray(s.vacancies))
+    if getattr(s, "interstitial_ids", None) is not None and len(s.interstitial_ids):
+        pts.append(s.pos[s.interstitial_ids])
+    pts = np.concatenate(pts) if pts else s.pos
+    out = np.empty(k, np.int64)
+    for q in range(k):
+        p = pts[rng.integers(len(pts))]
+        d = s.pos - p
+        d -= L * np.round(d / L)
+        near = np.nonzero(np.einsum("ij,ij->i", d, d) < r * r)[0]
+        out[q] = rng.choice(near) if len(near) else rng.integers(s.n)
+    return out
+
+
+def sparse_kick(s: systems.System, centre, radius, rng, stddev=0.6, r_pert=4.0):
+    """Master::perturb-style start (find.cpp:1147-1214) in sparse form: every non-frozen atom within r_pert
+    of the centre gets a Gaussian kick of standard deviation sd ~ N(stddev, stddev/3) (redrawn while
+    negative) scaled by (1 - r / r_pert) (the centre: 1), and a standard-normal axis component; the axis
+    is normalised.  Returns (atoms, positions, axis)."""
+    L = np.diag(s.basis)
+    d = s.pos - s.pos[centre]
+    d -= L * np.round(d / L)
+    r = np.sqrt(np.einsum("ij,ij->i", d, d))
+    sd = -1.0
+    while sd < 0:
+        sd = rng.normal(stddev, stddev / 3.0)
+    near = np.nonzero((r < r_pert) & (r <= radius))[0]
+    pos = s.pos[near].copy()
+    ax = np.zeros((len(near), 3))
+    for q, j in enumerate(near):
+        scale = 1.0 if j == centre else (1.0 - r[j] / r_pert)
+        pos[q] += rng.normal(0.0, sd, 3) * scale
+        ax[q] = rng.normal(0.0, 1.0, 3)
+    ax /= np.linalg.norm(ax)
+    return near.astype(np.int64), pos, ax
+
+
+def make_requests(count, rate=200.0, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), searches=(8, 64),
+                  radii=(4.0, 9.0)):
+    """Poisson arrivals at `rate`/s; per request a fresh bcc Fe system with N log-uniform in [n_min, n_max]
+    atoms and a defect fraction uniform in `defects` (half vacancies, half tetrahedral interstitials),
+    and a uniform number of searches in `searches` with radii uniform in `radii`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for q in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        cells = max(10, int(round((n_target / 2.0) ** (1.0 / 3.0))))
+        frac = rng.uniform(*defects)
+        n_def = max(2, int(round(frac * 2 * cells ** 3)))
+        s = systems.defect_system(cells, n_def // 2, n_def - n_def // 2, int(rng.integers(1 << 30)))
+        k = int(rng.integers(searches[0], searches[1] + 1))
+        centres = sample_centres(s, k, rng)
+        rad = rng.uniform(radii[0], radii[1], k)
+        kicks = [sparse_kick(s, int(c), float(R), rng) for c, R in zip(centres, rad)]
+        reqs.append(Request(float(t[q]), s, centres, rad, kicks))
+    return reqs
+
+
+def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_seconds=600.0):
+    """Replays `reqs` against `server` in real time (arrival times relative to the call).  Admission is
+    FIFO; a request whose system the memory budget cannot hold waits until running requests finish.
+    Returns a summary dict; per-search results land in each request's `results`."""
+    pending = collections.deque()
+    owner = {}
+    i = 0
+    t0 = time.perf_counter()
+    live = 0
+    while i < len(reqs) or pending or live:
+        now = time.perf_counter() - t0
+        if now > max_seconds:
+            raise fg.FlyError(f"serving did not finish within {max_seconds} s")
+        while i < len(reqs) and reqs[i].t <= now:
+            pending.append(reqs[i])
+            i += 1
+        while pending:
+            r = pending[0]
+            try:
+                r.sys_id = server.add_system(fg.Box(r.system.basis), r.system.pos, r.system.types)
+            except fg.Busy:
+                break
+            ids = server.submit(r.sys_id, r.centres, r.radii, r.kicks)
+            r.remaining = len(ids)
+            r.admitted = time.perf_counter() - t0
+            for sid in ids:
+                owner[int(sid)] = r
+            live += 1
+            pending.popleft()
+        wait = poll_ms
+        if i < len(reqs):
+            wait = min(wait, max(0.0, (reqs[i].t - (time.perf_counter() - t0)) * 1e3))
+        for sid in server.progress(wait_ms=wait):
+            r = owner.pop(int(sid))
+            r.results[int(sid)] = server.result(sid, positions=keep_positions, release=True)
+            r.remaining -= 1
+            if r.remaining == 0:
+                r.done = time.perf_counter() - t0
+                server.remove_system(r.sys_id)
+                live -= 1
+    wall = time.perf_counter() - t0
+    lat = np.array([r.done - r.t for r in reqs])
+    res = [x for r in reqs for x in r.results.values()]
+    return {
+        "requests": len(reqs),
+        "searches": len(res),
+        "converged": int(sum(1 for x in res if x["exit"] == 0)),
+        "wall_s": wall,
+        "searches_per_s": len(res) / wall,
+        "converged_per_s": sum(1 for x in res if x["exit"] == 0) / wall,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_max_s": float(lat.max()),
+        "mean_queue_wait_s": float(np.mean([r.admitted - r.t for r in reqs])),
+        "offered_requests_per_s": len(reqs) / max(reqs[-1].t, 1e-9),
+        "mean_atoms": float(np.mean([r.system.n for r in reqs])),
+        "mean_searches_per_request": float(np.mean([len(r.centres) for r in reqs])),
+    }
