@@ -29,6 +29,14 @@ R_CUT = 5.3
 SKIN = (1.1 ** (1.0 / 3.0) - 1.0) * R_CUT   # LBFGS::minimise skin policy (lbfgs.cpp:39)
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(counts, pos_bytes=8):
     """B_atom = 8 n_i + 6 s_p + 56 per atom-gradient (two passes): list entries (4 B) read once per
     pass, own position read once per pass, type x2, row offset x2, F' write + read, gradient write."""
@@ -80,13 +88,27 @@ def perturbed_starts(sys_, centres, radii, seed, stddev=0.6, r_pert=4.0):
     return starts, axes
 
 
-def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
-    """Config 3: 256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials,
-    active radius U(4, 9) A around defect-adjacent atoms, kick 0.6 A, |g| < 1e-3 eV/A."""
-    import torch  # noqa: F401  (device already initialised by the caller)
+WEN = os.path.join(ROOT, "tests", "golden", "wen.eam.fs")   # the reference's Fe potential (test fixture)
+
+
+def relax(ctx, fa, pot, s):
+    """min0: the system relaxed by the device L-BFGS (plain FP64 form) to |g| < 1e-4 eV/A."""
+    mn, g, failed, it = fa.LBFGS(ctx, fa.Box(s.basis), f2norm=1e-4, iter_max=5000).minimise(
+        s.pos, s.types, np.zeros(s.n, np.uint8), pot)
+    if failed:
+        raise RuntimeError("relaxation of the base system failed")
+    return mn
+
+
+def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
+    """Config 3: 256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials
+    (relaxed: min0), active radius U(4, 9) A around defect-adjacent atoms, kick 0.6 A, |g| < 1e-3 eV/A,
+    FP32 pair arithmetic / FP64 accumulation, curvature, dots and barriers; potential wen.eam.fs."""
     s = systems.config3(3)
-    path = systems.potential_path(("Fe",))
-    pot = fa.EAM(ctx, path)
+    pot = fa.EAM(ctx, WEN)
+    t0 = time.perf_counter()
+    s.pos = relax(ctx, fa, pot, s)
+    t_relax = time.perf_counter() - t0
     rng = np.random.default_rng(33)
     cand = systems.defect_adjacent_atoms(s)
     centres = rng.choice(cand, n_search, replace=True)
@@ -94,10 +116,12 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
     starts, axes = perturbed_starts(s, centres, radii, seed=34)
     box = fa.Box(s.basis)
     dimer = dict(f2norm=1e-3)
-    out = {"workload": "bcc Fe 20^3 + 20 vacancies + 40 interstitials (config 3), 256 searches, radius U(4,9) A, "
-                       "kick 0.6 A, |g| < 1e-3 eV/A, start from the unrelaxed defect lattice"}
+    out = {"workload": f"bcc Fe 20^3 + 20 vacancies + 40 interstitials (config 3, {s.n} atoms) relaxed to min0, "
+                       f"{n_search} searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, potential wen.eam.fs",
+           "relax_seconds": t_relax}
     res = {}
     for name, prec in (("batched_mixed", fa.MIXED), ("batched_fp64", fa.FP64)):
+        log(f"search leg: {name} (autotune run + timed run of {n_search} searches)")
         b = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
         ids = b.submit(centres, radii, starts, axes)
         b.run()   # first run includes the shape autotuner
@@ -109,10 +133,15 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
         r = [b2.result(i) for i in ids2]
         res[name] = r
         ok = sum(1 for x in r if x["exit"] == 0)
+        ms = np.array([x["device_ms"] for x in r])
         out[name] = {"seconds": el, "converged": ok, "searches_per_s": n_search / el,
-                     "converged_per_s": ok / el, "mean_grads": float(np.mean([x["grads"] for x in r])),
+                     "converged_per_s": ok / el, "exits": np.bincount([x["exit"] for x in r], minlength=7).tolist(),
+                     "mean_grads": float(np.mean([x["grads"] for x in r])),
                      "mean_mobile": float(np.mean([x["n_mobile"] for x in r])),
-                     "relaunched": int(sum(1 for x in r if x["launches"] > 1))}
+                     "mean_region": float(np.mean([x["n_region"] for x in r])),
+                     "relaunched": int(sum(1 for x in r if x["launches"] > 1)),
+                     "search_device_ms_p50": float(np.percentile(ms, 50)),
+                     "search_device_ms_p99": float(np.percentile(ms, 99))}
     # Agreement of the mixed form with the FP64 form from the same starts (the FP64 batched path matches
     # the FP64 oracle's exits, steps and barriers in tests/test_gpu_batch.py).
     m, f = res["batched_mixed"], res["batched_fp64"]
@@ -123,7 +152,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
     out["batched_mixed"]["searches_per_s_dE_1e-3"] = good / el
     out["batched_mixed"]["max_abs_dE_vs_fp64"] = float(max(agree)) if agree else None
     # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
-    frozen_all = []
+    log(f"search leg: plain FP64 sample of {plain_sample}")
     dim = fa.Dimer(ctx, box, dimer=dimer)
     L = np.diag(s.basis)
     t0 = time.perf_counter()
@@ -132,49 +161,53 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4, cpu_sample=16):
         d = s.pos - s.pos[centres[k]]
         d -= L * np.round(d / L)
         fz = (np.einsum("ij,ij->i", d, d) > radii[k] ** 2).astype(np.uint8)
-        frozen_all.append(fz)
         plain.append(dim.find_sp(starts[k], axes[k], s.types, fz, s.pos, pot))
     el = time.perf_counter() - t0
     out["plain_fp64_gpu"] = {"sample": plain_sample, "seconds": el, "searches_per_s": plain_sample / el,
                              "exits_match_batched_fp64": int(sum(1 for k in range(plain_sample)
                                                                  if plain[k]["exit"] == f[k]["exit"]))}
-    return out, (s, path, centres, radii, starts, axes, dimer)
+    return out, (s, centres, radii, starts, axes, dimer, m)
 
 
-def serve_bench(ctx, fa, systems, n_req=100, rate=200.0):
-    """Config 4: Poisson arrivals at 200/s of requests with their own bcc Fe systems (N log-uniform in
-    [2000, 250000], 0.5-2 % vacancies + interstitials), 8-64 searches each, radii U(4, 9) A, served with
-    continuous batching; latency = arrival to last search done."""
+def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
+    """Config 4: Poisson arrivals at 200/s of requests from 12 kMC systems (bcc Fe, N log-uniform in
+    [2000, 250000], 0.5-2 % vacancies + interstitials, each relaxed), 8-64 searches each, radii U(4, 9) A,
+    served with continuous batching; latency = arrival to last search done."""
     from openfly_amd import serve
-    pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
-    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
-    serve.serve(srv, serve.make_requests(4, rate=rate, seed=40, n_max=20000))   # warm-up + shape autotune
+    pot = fa.EAM(ctx, WEN)
+    log(f"serving leg: relaxing {pool_size} kMC systems")
     t0 = time.perf_counter()
-    reqs = serve.make_requests(n_req, rate=rate, seed=41)
-    gen = time.perf_counter() - t0
-    out = serve.serve(srv, reqs)
+    pool = serve.make_pool(pool_size, seed=41, relax=lambda sy: relax(ctx, fa, pot, sy))
+    t_pool = time.perf_counter() - t0
+    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
+    log("serving leg: warm-up")
+    serve.serve(srv, serve.make_requests(pool[:2], 4, rate=rate, seed=40, searches=(8, 16)))   # + shape autotune
+    log(f"serving leg: generating {n_req} requests")
+    reqs = serve.make_requests(pool, n_req, rate=rate, seed=42)
+    log("serving leg: replay")
+    out = serve.serve(srv, reqs, report=log)
     st = srv.status()
-    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, bcc Fe N log-uniform [2000, 250000] with 0.5-2% "
-                        "vacancies + interstitials, 8-64 searches per request, radius U(4,9) A, kick 0.6 A, "
-                        "|g| < 1e-3 eV/A, FP32 pair / FP64 accumulation",
-               request_generation_s=gen, launches=st["launches"], kernel_ms=st["kernel_ms"],
+    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, from {pool_size} relaxed kMC systems (bcc Fe, N "
+                        "log-uniform [2000, 250000], 0.5-2% vacancies + interstitials), 8-64 searches per request, "
+                        "radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, FP32 pair / FP64 accumulation, wen.eam.fs",
+               pool_relax_seconds=t_pool, launches=st["launches"], kernel_ms=st["kernel_ms"],
                block_threads=st["block_threads"], lanes_per_atom=st["lanes_per_atom"],
-               max_atoms=int(max(r.system.n for r in reqs)))
+               pool_atoms=[int(p.n) for p in pool])
     return out
 
 
 def cpu_search_baseline(data_args, cpu_sample=16):
-    """FP64 oracle, one search per thread over all host cores (find.cpp:697-732 with threads = 1)."""
+    """FP64 oracle, one search per thread over the host cores (find.cpp:697-732 with threads = 1), on the
+    first cpu_sample starts of the batched leg; also the agreement of the device's mixed-precision
+    barriers with the oracle's for the same starts."""
     from oracle import pyoracle as po
-    s, path, centres, radii, starts, axes, dimer = data_args
+    import concurrent.futures as cf
+    s, centres, radii, starts, axes, dimer, mixed = data_args
     threads = os.cpu_count() or 1
     k = min(cpu_sample, threads)
-    data = po.DataEAM(path)
+    data = po.DataEAM(WEN)
+    box = po.Box(s.basis)
     L = np.diag(s.basis)
-    # frozen masks differ per search; run them one batch per distinct mask through find_sp_many
-    t0 = time.perf_counter()
-    exits = []
-    import concurrent.futures as cf
 
     def one(i):
         d = s.pos - s.pos[centres[i]]
@@ -182,13 +215,28 @@ def cpu_search_baseline(data_args, cpu_sample=16):
         fz = (np.einsum("ij,ij->i", d, d) > radii[i] ** 2).astype(np.uint8)
         return po.find_sp(po.Box(s.basis), data, starts[i], axes[i], s.types, fz, s.pos, threads=1, dimer=dimer)
 
+    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=k) as ex:
-        for r in ex.map(one, range(k)):
-            exits.append(r["exit"])
+        res = list(ex.map(one, range(k)))
     el = time.perf_counter() - t0
+
+    def energy(p):
+        nl = po.List(box, data.r_cut)
+        nl.rebuild(p, threads)
+        return po.energy(data, nl, s.types, np.zeros(s.n, np.uint8), threads)
+
+    e0 = energy(s.pos)
+    both, within, worst = 0, 0, 0.0
+    for i, r in enumerate(res):
+        if r["exit"] == 0 and mixed[i]["exit"] == 0:
+            d = abs((energy(r["pos"]) - e0) - mixed[i]["barrier"])
+            both += 1
+            within += int(d <= 1e-3)
+            worst = max(worst, d)
     return {"value": k / el, "unit": "searches/s", "cores": k, "kind": "port",
             "sample": f"FP64 oracle Dimer::find_sp on the first {k} of the same starts, one search per thread, "
-                      f"{el:.1f} s, {sum(1 for e in exits if e == 0)} converged"}
+                      f"{el:.1f} s, {sum(1 for r in res if r['exit'] == 0)} converged",
+            "oracle_vs_mixed": {"both_converged": both, "within_1e-3eV": within, "max_abs_dE": worst}}
 
 
 def main():
@@ -270,6 +318,7 @@ def main():
             b.record(stream)
             ev.append((a, b))
 
+    log(f"gradient leg: {s.n} atoms, {args.warmup} warm-up + {args.steps} timed steps")
     for _ in range(args.warmup):
         step(False)
     ctx.synchronize()
@@ -321,12 +370,16 @@ def main():
     }
     if not args.no_search:
         sr, sargs = search_bench(ctx, fa, systems)
+        log("search leg done")
         if not args.no_cpu:
+            log("search leg: CPU oracle sample")
             sr["cpu_baseline"] = cpu_search_baseline(sargs)
         out["searches"] = sr
     if not args.no_serve:
         out["serving"] = serve_bench(ctx, fa, systems)
+        log("serving leg done")
     if not args.no_cpu:
+        log("gradient leg: CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(s, path)
     print(json.dumps(out), flush=True)
 

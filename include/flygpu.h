@@ -197,6 +197,9 @@ typedef struct {
   double energy;   /* batched searches: FP64 barrier E(saddle) - E(base); plain dimer: 0 */
   double max_disp; /* batched searches: largest displacement of a mobile atom from its base position */
   int launches;    /* batched searches: kernel launches used (a search that outgrew its region is re-run) */
+  int rot_iters;   /* batched searches: rotor iterations */
+  double device_ms; /* batched searches: device time its workgroup spent on the search (last launch) */
+  double device_grad_ms; /* ... of which in gradient evaluations */
 } flyg_search_stats;
 
 void flyg_dimer_default_options(flyg_dimer_options* opt);

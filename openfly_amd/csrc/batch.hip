@@ -6,11 +6,13 @@
 // whole dimer search (Rotor::eff_gradient rotor.cpp:49-164, Dimer::find_sp dimer.cpp:28-171, both L-BFGS
 // cores lbfgs_core.cpp:29-87) with workgroup barriers only.  A search works on its active region:
 //   M  = mobile atoms (minimum-image distance to the centre <= radius, the rest of the system is Frozen),
-//   S1 = frozen atoms within r_L = r_cut + skin of a mobile atom (their density, hence F', can change).
-// Neighbour rows are built once from the base positions with cut-off r_L (a Verlet skin): mobile rows
+//   S1 = frozen atoms within r_cut + skin_f of a mobile atom (their density, hence F', can change).
+// Neighbour rows are built once from the base positions with Verlet skins: mobile rows
 // hold every region neighbour, S1 rows only the mobile ones, and the frozen part of an S1 density is the
-// base density minus the mobile part.  The rows stay complete while d1 + d2 < skin for the two largest
-// mobile displacements from the base; a search that breaks this is re-run with a larger skin.  Every 3N vector of
+// base density minus the mobile part.  Mobile-frozen pairs are listed within r_cut + skin_f and
+// mobile-mobile pairs within r_cut + skin_m, so the rows stay complete while d1 < skin_f and d1 + d2 <
+// skin_m for the two largest mobile displacements from the base; a search that breaks this is re-run
+// with larger skins.  Every 3N vector of
 // the reference is zero on frozen atoms, so the device keeps only the 3*|M| mobile components.
 // Barrier = E_loc(saddle) - E_loc(base) in FP64, where E_loc collects every term of the EAM energy that
 // depends on mobile positions (sum over M u S1 of F(rho) plus the pairs with a mobile member).
@@ -23,6 +25,7 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 
 #include "plain.hpp"
 
@@ -33,8 +36,15 @@ namespace flyg {
     constexpr int EXIT_REGION = 5;
     constexpr int EXIT_OOB = 6;
     // Workspace vectors per search (each 3*nm doubles): 10 named vectors, then the translation core
-    // (m s-vectors, m y-vectors, x_prev, g_prev, q) and the rotation core (the same with its own m).
-    inline int nvec(int m, int rm) { return 16 + 2 * (m + rm); }
+    // (m s-vectors, m y-vectors, x_prev, g_prev, q), the rotation core (the same with its own m), and
+    // the positions at the last active-list refresh.
+    __host__ __device__ inline int nvec(int m, int rm) { return 17 + 2 * (m + rm); }
+
+    // Active lists: the part of each candidate row within r_c + ACT_SKIN_F (pairs with a frozen atom)
+    // or r_c + ACT_SKIN_M (mobile pairs) at the last refresh, rebuilt on the device when the mobile
+    // atoms have moved too far since (a Verlet list inside the candidate rows).
+    constexpr double ACT_SKIN_F = 0.5;
+    constexpr double ACT_SKIN_M = 1.0;
 
     struct SearchHdr {
       int nm, nr;          // mobile atoms, region atoms (M u S1)
@@ -42,7 +52,8 @@ namespace flyg {
       long row_off;        // first row pointer (nr + 1 entries)
       long v3_off;         // first element of this search's 3*nm start/out vectors
       long ws_off;         // first element of this search's workspace (nvec * 3*nm)
-      double skin;         // Verlet skin of the rows (r_L = r_c + skin)
+      double skin_f;       // Verlet skin of mobile-frozen pairs (and of S1)
+      double skin_m;       // Verlet skin of mobile-mobile pairs (>= skin_f)
       // The search's system, resident on the device for as long as any of its searches runs.
       double4 const* sbase;   // canonical base positions, by global atom index
       double const* srho;     // FP64 base densities
@@ -51,9 +62,12 @@ namespace flyg {
     };
 
     struct SearchOut {
-      int exit_code, steps, rot_iters, pad;
+      int exit_code, steps, rot_iters, refreshes;   // refreshes: active-list rebuilds
       long grads;
       double curv, barrier, e_base, max_disp;
+      double max_pair;   // largest d1 + d2 seen
+      long ticks;        // device wall-clock ticks the workgroup spent on this search
+      long grad_ticks;   // ... of which inside region gradients
     };
 
     struct DimOpt {
@@ -73,6 +87,8 @@ namespace flyg {
       double* fp;                // F'(rho) per region atom
       int const* rowptr;
       unsigned const* ent;
+      unsigned* act;             // active entries (same layout as ent)
+      int* alen;                 // active entries per row
       double* ws;
       double const* start_x;
       double const* start_ax;
@@ -81,7 +97,7 @@ namespace flyg {
       SearchOut* out;
       unsigned* queue;
       DimOpt o;
-      int multi;
+      int t0;   // the one species present when the kernel is instantiated with MULTI = false
     };
 
     // ---------------------------------------------------------------------------------------------
@@ -100,34 +116,39 @@ namespace flyg {
         return v;
       }
 
-      // Sums up to four values over the workgroup with a tree that does not depend on T: element i of a
-      // vector belongs to virtual slot i mod 512, each slot adds in increasing i, and the 512 slots meet
-      // in the xor tree 256, 128, ..., 1 (levels >= T in registers, 64..T/2 through LDS, < 64 by
-      // shuffles).  v[q][k] is the slot tid + q T of value k; every thread gets the totals.
-      static constexpr int VS = 512;
+      // Sums up to four values over the workgroup in an order that does not depend on T: element i of a
+      // vector belongs to virtual slot i mod 1024, each slot adds its elements in increasing i; the
+      // slots form 16 virtual wavefronts of 64 (slot s in wavefront s / 64, lane s mod 64), each reduced
+      // by the xor shuffle tree 32, 16, ..., 1; the 16 wavefront sums are then added in order 0..15.
+      // v[q][k] is the slot tid + q T of value k; every thread gets the totals.  Two barriers.
+      static constexpr int VS = 1024;
       static constexpr int NQ = VS / T;
+      static constexpr int VW = VS / 64;
 
       template <int K>
       __device__ void tree(double (&v)[NQ][4]) const {
 #pragma unroll
-        for (int bit = VS / 2; bit >= T; bit >>= 1)
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int bit = 32; bit >= 1; bit >>= 1) v[q][k] = __dadd_rn(v[q][k], __shfl_xor(v[q][k], bit, 64));
+        if ((tid & 63) == 0) {
 #pragma unroll
           for (int q = 0; q < NQ; ++q)
-            if (((q * T) & bit) == 0 && q + bit / T < NQ)
 #pragma unroll
-              for (int k = 0; k < K; ++k) v[q][k] = __dadd_rn(v[q][k], v[q + bit / T][k]);
-        for (int bit = T / 2; bit >= 64; bit >>= 1) {
-#pragma unroll
-          for (int k = 0; k < K; ++k) red[4 * tid + k] = v[0][k];
-          __syncthreads();
-#pragma unroll
-          for (int k = 0; k < K; ++k) v[0][k] = __dadd_rn(v[0][k], red[4 * (tid ^ bit) + k]);
-          __syncthreads();
+            for (int k = 0; k < K; ++k) red[4 * ((tid >> 6) + q * (T / 64)) + k] = v[q][k];
         }
+        __syncthreads();
+        double t[K];
 #pragma unroll
-        for (int bit = 32; bit >= 1; bit >>= 1)
+        for (int k = 0; k < K; ++k) t[k] = red[k];
+        for (int w = 1; w < VW; ++w)
 #pragma unroll
-          for (int k = 0; k < K; ++k) v[0][k] = __dadd_rn(v[0][k], __shfl_xor(v[0][k], bit, 64));
+          for (int k = 0; k < K; ++k) t[k] = __dadd_rn(t[k], red[4 * w + k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[0][k] = t[k];
       }
 
       template <int K>
@@ -139,7 +160,12 @@ namespace flyg {
 
       __device__ double dot(double const* a, double const* b, int n) const {
         double v[NQ][4] = {};
-        for (int i = tid, q = 0; i < n; i += T, q = (q + 1) % NQ) v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
+        for (int i0 = 0; i0 < n; i0 += VS)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            int i = i0 + tid + q * T;
+            if (i < n) v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
+          }
         double o[1];
         sum<1>(o, v);
         return o[0];
@@ -148,10 +174,15 @@ namespace flyg {
       __device__ void dot2(double const* a, double const* b, double const* c, double const* d, int n, double& x,
                            double& y) const {
         double v[NQ][4] = {};
-        for (int i = tid, q = 0; i < n; i += T, q = (q + 1) % NQ) {
-          v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
-          v[q][1] = __dadd_rn(v[q][1], c[i] * d[i]);
-        }
+        for (int i0 = 0; i0 < n; i0 += VS)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            int i = i0 + tid + q * T;
+            if (i < n) {
+              v[q][0] = __dadd_rn(v[q][0], a[i] * b[i]);
+              v[q][1] = __dadd_rn(v[q][1], c[i] * d[i]);
+            }
+          }
         double o[2];
         sum<2>(o, v);
         x = o[0];
@@ -206,83 +237,263 @@ namespace flyg {
       double* fp;
       int const* rowptr;
       unsigned const* ent;
+      unsigned* act;
+      int* alen;
+      double act_f, act_m;   // active-list skins (mobile-frozen, mobile-mobile)
     };
 
     __device__ inline int vix(int nt, int a, int b) { return a > b ? a + nt * b : b + nt * a; }
 
+    // ---------------------------------------------------------------------------------------------
+    // Pipelined neighbour walks.  A thread handles U = 4 entries per step: all entry loads, then all
+    // position loads, then all table loads are issued before any arithmetic, and an entry that is
+    // past the row end or beyond r_c takes a dummy in-range table slot and is not accumulated, so the
+    // sums equal those of the plain one-entry-at-a-time loop bit for bit.
+    // ---------------------------------------------------------------------------------------------
+
+    constexpr int U = 4;
+
+    // Spline segment of x (reference Spline::fetch); `use` is false for entries that do not count.
+    template <bool MIX>
+    struct Seg {
+      using R = typename std::conditional<MIX, float, double>::type;
+      int i;
+      R t;
+      bool use;
+    };
+
+    template <bool MIX>
+    __device__ inline Seg<MIX> seg_of(SplineDesc const& s, typename Seg<MIX>::R x, bool in, unsigned& e) {
+      using R = typename Seg<MIX>::R;
+      R q = x * (MIX ? static_cast<R>(s.inv_dxf) : static_cast<R>(s.inv_dx));
+      bool ok = (q < static_cast<R>(s.count)) && !(q <= static_cast<R>(-1));
+      if (in && !ok) e |= ERR_SPLINE_OOB;
+      Seg<MIX> g;
+      g.use = in && ok;
+      g.i = g.use ? static_cast<int>(q) : 0;
+      g.t = g.use ? x - static_cast<R>(g.i) * (MIX ? static_cast<R>(s.dxf) : static_cast<R>(s.dx)) : static_cast<R>(0);
+      return g;
+    }
+
+    template <bool MIX>
+    __device__ inline auto coef_of(EAMTables const& T, SplineDesc const& s, Seg<MIX> const& g) {
+      if constexpr (MIX) {
+        return T.c32[s.off + g.i];
+      } else {
+        return T.c64[s.off + g.i];
+      }
+    }
+
+    template <typename C, typename R>
+    __device__ inline R poly0(C const& c, R t) { return c.x + t * (c.y + t * (c.z + t * c.w)); }
+    template <typename C, typename R>
+    __device__ inline R poly1(C const& c, R t) { return c.y + t * (static_cast<R>(2) * c.z + t * static_cast<R>(3) * c.w); }
+
+    // Geometry of U entries of one row: entry k_q = k0 + q * stride (valid while < c).
+    template <bool MIX>
+    struct Quad {
+      using R = typename Seg<MIX>::R;
+      double dx[U], dy[U], dz[U], w[U];
+      int j[U];
+      bool in[U];
+      R r[U], rinv[U];
+    };
+
+    template <bool MIX>
+    __device__ inline void load_quad(SView const& v, unsigned const* ent, double rc2, double4 const& pi, int k0,
+                                     int stride, int b, int c, Quad<MIX>& Q) {
+      unsigned en[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        int k = k0 + q * stride;
+        en[q] = ent[k < c ? k : b];
+      }
+      double4 pj[U];
+      double3 sh[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        Q.j[q] = static_cast<int>(en[q] & NB_IDX_MASK);
+        pj[q] = v.cur[Q.j[q]];
+        sh[q] = v.S[en[q] >> NB_SHIFT_BITS];
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        Q.dx[q] = (pj[q].x + sh[q].x) - pi.x;
+        Q.dy[q] = (pj[q].y + sh[q].y) - pi.y;
+        Q.dz[q] = (pj[q].z + sh[q].z) - pi.z;
+        Q.w[q] = pj[q].w;
+        double r2 = Q.dx[q] * Q.dx[q] + Q.dy[q] * Q.dy[q] + Q.dz[q] * Q.dz[q];
+        Q.in[q] = (k0 + q * stride < c) && (r2 < rc2);
+        if constexpr (MIX) {
+          float r2f = Q.in[q] ? static_cast<float>(r2) : 1.0f;
+          Q.rinv[q] = __frsqrt_rn(r2f);
+          Q.r[q] = r2f * Q.rinv[q];
+        } else {
+          Q.r[q] = Q.in[q] ? sqrt(r2) : 1.0;
+          Q.rinv[q] = 1.0;
+        }
+      }
+    }
+
+    // Region rows staged in LDS per workgroup size (96 KB at 1024 threads: one workgroup per CU anyway).
+    template <int T>
+    __host__ __device__ constexpr int lds_rows() { return T >= 1024 ? 3072 : (T >= 512 ? 1536 : 1024); }
+
+    // Rebuilds the active lists from the candidate rows at the current region positions (stable:
+    // an active row is the ordered subsequence of its candidate row, whatever the workgroup shape).
+    template <int T, int L>
+    __device__ void refresh_active(BatchArgs const& A, SView const& v, Team<T> const& tm) {
+      double const cf = A.T.rc + v.act_f, cm = A.T.rc + v.act_m;
+      double const cf2 = cf * cf, cm2 = cm * cm;
+      int const lane = tm.tid % L;
+      int const gbase = (tm.tid & 63) - lane;   // first lane of this row group in the wavefront
+      for (int r0 = tm.tid / L; r0 < v.nm; r0 += T / L) {
+        double4 pi = v.cur[r0];
+        int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
+        int cnt = 0;
+        for (int k0 = b; k0 < c; k0 += L) {
+          int k = k0 + lane;
+          bool keep = false;
+          unsigned en = 0;
+          if (k < c) {
+            en = v.ent[k];
+            int j = static_cast<int>(en & NB_IDX_MASK);
+            double3 sh = v.S[en >> NB_SHIFT_BITS];
+            double4 pj = v.cur[j];
+            double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            keep = r2 < (j < v.nm ? cm2 : cf2);
+          }
+          unsigned long long m = __ballot(keep);
+          unsigned gm = static_cast<unsigned>((m >> gbase) & ((1ull << L) - 1ull));
+          if (keep) v.act[b + cnt + __popc(gm & ((1u << lane) - 1u))] = en;
+          cnt += __popc(gm);
+        }
+        if (lane == 0) v.alen[r0] = cnt;
+      }
+      for (int r0 = v.nm + tm.tid; r0 < v.nr; r0 += T) {
+        double4 pi = v.cur[r0];
+        int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
+        int cnt = 0;
+        for (int k = b; k < c; ++k) {
+          unsigned en = v.ent[k];
+          int j = static_cast<int>(en & NB_IDX_MASK);
+          double3 sh = v.S[en >> NB_SHIFT_BITS];
+          double4 pj = v.cur[j];
+          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
+          if (dx * dx + dy * dy + dz * dz < cf2) v.act[b + cnt++] = en;
+        }
+        v.alen[r0] = cnt;
+      }
+      __syncthreads();
+    }
+
     // EAM gradient of the mobile atoms at the current region positions (pass 1 rho/F' over M u S1,
     // pass 2 forces over M).  MIX: FP32 pair/spline arithmetic with FP64 accumulation.
-    template <int T, int L, bool MIX>
+    template <int T, int L, bool MIX, bool MULTI>
     __device__ unsigned region_gradient(BatchArgs const& A, SView const& v, Team<T> const& tm, double* g) {
+      using R = typename Seg<MIX>::R;
+      constexpr int NS = 16 / L;
+      static_assert(U % NS == 0, "a step must cover whole slot cycles");
       EAMTables const& Tb = A.T;
       int const nt = Tb.ntypes;
       unsigned e = 0;
       int const lane = tm.tid % L;
-      for (int r0 = tm.tid / L; r0 < v.nr; r0 += T / L) {
+      // Densities of U entries (FP64 values, zero where not used).
+      auto densities = [&](Quad<MIX> const& Q, int ti, double (&out)[U]) {
+        Seg<MIX> sg[U];
+        decltype(coef_of<MIX>(Tb, Tb.phi[0], sg[0])) cf[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+          int tj = MULTI ? v.type[Q.j[q]] : A.t0;
+          SplineDesc const& sd = Tb.phi[tj + nt * ti];
+          sg[q] = seg_of<MIX>(sd, Q.r[q], Q.in[q], e);
+          cf[q] = coef_of<MIX>(Tb, sd, sg[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) out[q] = sg[q].use ? static_cast<double>(poly0(cf[q], sg[q].t)) : 0.0;
+      };
+      // Mobile rows (long: every region neighbour): L lanes per row, 16-slot sums.
+      for (int r0 = tm.tid / L; r0 < v.nm; r0 += T / L) {
         double4 pi = v.cur[r0];
-        int ti = A.multi ? v.type[r0] : 0;
+        int ti = MULTI ? v.type[r0] : A.t0;
         Acc16<L> rho_acc;
-        int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
-        for (int k = b + lane, m = 0; k < c; k += L, ++m) {
-          unsigned en = v.ent[k];
-          int j = static_cast<int>(en & NB_IDX_MASK);
-          double3 sh = v.S[en >> NB_SHIFT_BITS];
-          double4 pj = v.cur[j];
-          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
-          double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < Tb.rc2) {
-            int tj = A.multi ? v.type[j] : 0;
-            if (MIX) {
-              float r2f = static_cast<float>(r2);
-              rho_acc.add(m, static_cast<double>(spl32f<0>(Tb, Tb.phi[tj + nt * ti], r2f * __frsqrt_rn(r2f), e)));
-            } else {
-              rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], sqrt(r2), e));
-            }
-          }
+        int b = v.rowptr[r0], c = b + v.alen[r0];
+        for (int k0 = b + lane; k0 < c; k0 += U * L) {
+          Quad<MIX> Q;
+          load_quad<MIX>(v, v.act, Tb.rc2, pi, k0, L, b, c, Q);
+          double d[U];
+          densities(Q, ti, d);
+#pragma unroll
+          for (int q = 0; q < U; ++q)
+            if (Q.in[q]) rho_acc.add_slot(q % NS, d[q]);
         }
         double rho = rho_acc.total();
-        if (lane == 0) {
-          if (r0 >= v.nm) rho += v.rhof[r0];
-          v.fp[r0] = spl64<1>(Tb, Tb.F[ti], rho, e);
+        if (lane == 0) v.cur[r0].w = spl64<1>(Tb, Tb.F[ti], rho, e);
+      }
+      // Frozen-shell rows (short: mobile neighbours only): one thread per row, entries summed in order,
+      // then the frozen part of the density.  Neither order depends on the workgroup shape.
+      for (int r0 = v.nm + tm.tid; r0 < v.nr; r0 += T) {
+        double4 pi = v.cur[r0];
+        int ti = MULTI ? v.type[r0] : A.t0;
+        int b = v.rowptr[r0], c = b + v.alen[r0];
+        double rho = 0.0;
+        for (int k0 = b; k0 < c; k0 += U) {
+          Quad<MIX> Q;
+          load_quad<MIX>(v, v.act, Tb.rc2, pi, k0, 1, b, c, Q);
+          double d[U];
+          densities(Q, ti, d);
+#pragma unroll
+          for (int q = 0; q < U; ++q)
+            if (Q.in[q]) rho = __dadd_rn(rho, d[q]);
         }
+        rho = __dadd_rn(rho, v.rhof[r0]);
+        v.cur[r0].w = spl64<1>(Tb, Tb.F[ti], rho, e);
       }
       __syncthreads();
       for (int r0 = tm.tid / L; r0 < v.nm; r0 += T / L) {
         double4 pi = v.cur[r0];
-        double fpi = v.fp[r0];
-        int ti = A.multi ? v.type[r0] : 0;
+        double fpi = pi.w;
+        int ti = MULTI ? v.type[r0] : A.t0;
         Acc16<L> gx, gy, gz;
-        int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
-        for (int k = b + lane, m = 0; k < c; k += L, ++m) {
-          unsigned en = v.ent[k];
-          int j = static_cast<int>(en & NB_IDX_MASK);
-          double3 sh = v.S[en >> NB_SHIFT_BITS];
-          double4 pj = v.cur[j];
-          double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
-          double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < Tb.rc2) {
-            int tj = A.multi ? v.type[j] : 0;
-            double fpj = v.fp[j];
-            double cf;
-            if (MIX) {
-              float r2f = static_cast<float>(r2);
-              float rinv = __frsqrt_rn(r2f);
-              float rf = r2f * rinv;
-              float vp = spl32f<1>(Tb, Tb.V[vix(nt, ti, tj)], rf, e);
-              float pa = spl32f<1>(Tb, Tb.phi[tj + nt * ti], rf, e);
-              float pb = A.multi ? spl32f<1>(Tb, Tb.phi[ti + nt * tj], rf, e) : pa;
-              float mag = vp + static_cast<float>(fpi) * pa + static_cast<float>(fpj) * pb;
-              cf = static_cast<double>(mag * rinv);
-            } else {
-              double r = sqrt(r2);
-              double vp = spl64<1>(Tb, Tb.V[vix(nt, ti, tj)], r, e);
-              double pa = spl64<1>(Tb, Tb.phi[tj + nt * ti], r, e);
-              double pb = A.multi ? spl64<1>(Tb, Tb.phi[ti + nt * tj], r, e) : pa;
-              cf = (vp + fpi * pa + fpj * pb) / r;
+        int b = v.rowptr[r0], c = b + v.alen[r0];
+        for (int k0 = b + lane; k0 < c; k0 += U * L) {
+          Quad<MIX> Q;
+          load_quad<MIX>(v, v.act, Tb.rc2, pi, k0, L, b, c, Q);
+          Seg<MIX> sv[U], sa[U], sb[U];
+          decltype(coef_of<MIX>(Tb, Tb.phi[0], sv[0])) cv[U], ca[U], cb[U];
+#pragma unroll
+          for (int q = 0; q < U; ++q) {
+            int tj = MULTI ? v.type[Q.j[q]] : A.t0;
+            SplineDesc const& dv = Tb.V[vix(nt, ti, tj)];
+            SplineDesc const& da = Tb.phi[tj + nt * ti];
+            sv[q] = seg_of<MIX>(dv, Q.r[q], Q.in[q], e);
+            sa[q] = seg_of<MIX>(da, Q.r[q], Q.in[q], e);
+            cv[q] = coef_of<MIX>(Tb, dv, sv[q]);
+            ca[q] = coef_of<MIX>(Tb, da, sa[q]);
+            if (MULTI) {
+              SplineDesc const& db = Tb.phi[ti + nt * tj];
+              sb[q] = seg_of<MIX>(db, Q.r[q], Q.in[q], e);
+              cb[q] = coef_of<MIX>(Tb, db, sb[q]);
             }
-            gx.add(m, -cf * dx);
-            gy.add(m, -cf * dy);
-            gz.add(m, -cf * dz);
+          }
+#pragma unroll
+          for (int q = 0; q < U; ++q) {
+            R vp = sv[q].use ? poly1(cv[q], sv[q].t) : static_cast<R>(0);
+            R pa = sa[q].use ? poly1(ca[q], sa[q].t) : static_cast<R>(0);
+            R pb = MULTI ? (sb[q].use ? poly1(cb[q], sb[q].t) : static_cast<R>(0)) : pa;
+            double cf;
+            if constexpr (MIX) {
+              float mag = vp + static_cast<float>(fpi) * pa + static_cast<float>(Q.w[q]) * pb;
+              cf = static_cast<double>(mag * Q.rinv[q]);
+            } else {
+              cf = (vp + fpi * pa + Q.w[q] * pb) / Q.r[q];
+            }
+            if (Q.in[q]) {
+              gx.add_slot(q % NS, -cf * Q.dx[q]);
+              gy.add_slot(q % NS, -cf * Q.dy[q]);
+              gz.add_slot(q % NS, -cf * Q.dz[q]);
+            }
           }
         }
         double tx = gx.total(), ty = gy.total(), tz = gz.total();
@@ -298,7 +509,7 @@ namespace flyg {
 
     // FP64 local energy (all terms that depend on mobile positions).  With base = true the densities
     // of the region come from the global base pass and S1's frozen density part is recorded.
-    template <int T, int L>
+    template <int T, int L, bool MULTI>
     __device__ double region_energy(BatchArgs const& A, SView const& v, Team<T> const& tm, bool base, unsigned& err) {
       EAMTables const& Tb = A.T;
       int const nt = Tb.ntypes;
@@ -307,7 +518,7 @@ namespace flyg {
       // Per-row energies go to the fp scratch (overwritten by the next gradient), then a fixed-order sum.
       for (int r0 = tm.tid / L; r0 < v.nr; r0 += T / L) {
         double4 pi = v.cur[r0];
-        int ti = A.multi ? v.type[r0] : 0;
+        int ti = MULTI ? v.type[r0] : A.t0;
         Acc16<L> rho_acc, pair_acc;
         bool mob = r0 < v.nm;
         int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
@@ -320,7 +531,7 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < Tb.rc2) {
             double r = sqrt(r2);
-            int tj = A.multi ? v.type[j] : 0;
+            int tj = MULTI ? v.type[j] : A.t0;
             rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], r, e));
             if (mob) pair_acc.add(m, (j < v.nm ? 0.5 : 1.0) * spl64<0>(Tb, Tb.V[vix(nt, ti, tj)], r, e));
           }
@@ -340,7 +551,12 @@ namespace flyg {
       }
       __syncthreads();
       double vv[Team<T>::NQ][4] = {};
-      for (int i = tm.tid, q = 0; i < v.nr; i += T, q = (q + 1) % Team<T>::NQ) vv[q][0] = __dadd_rn(vv[q][0], v.fp[i]);
+      for (int i0 = 0; i0 < v.nr; i0 += Team<T>::VS)
+#pragma unroll
+        for (int q = 0; q < Team<T>::NQ; ++q) {
+          int i = i0 + tm.tid + q * T;
+          if (i < v.nr) vv[q][0] = __dadd_rn(vv[q][0], v.fp[i]);
+        }
       double o[1];
       tm.template sum<1>(o, vv);
       err |= tm.any_or(e);
@@ -414,8 +630,10 @@ namespace flyg {
       }
     };
 
-    template <int T, int L, bool MIX>
-    __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S) {
+    template <int T, int L, bool MIX, bool MULTI>
+    __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
+                               double4* lds_cur) {
+      long const t_begin = static_cast<long>(wall_clock64());
       SearchHdr const h = A.hdr[sid];
       DimOpt const& o = A.o;
       SView v;
@@ -424,7 +642,8 @@ namespace flyg {
       v.n3 = 3 * h.nm;
       v.S = lds_S;
       v.srho = h.srho;
-      v.cur = A.cur + h.reg_off;
+      // Region positions (w = F' between the two passes of a gradient) live in LDS when they fit.
+      v.cur = h.nr <= lds_rows<T>() ? lds_cur : A.cur + h.reg_off;
       v.base = A.base + h.reg_off;
       v.type = A.rtype + h.reg_off;
       v.gidx = A.rgidx + h.reg_off;
@@ -432,6 +651,10 @@ namespace flyg {
       v.fp = A.fp + h.reg_off;
       v.rowptr = A.rowptr + h.row_off;
       v.ent = A.ent;
+      v.act = A.act;
+      v.alen = A.alen + h.reg_off;
+      v.act_f = fmin(ACT_SKIN_F, h.skin_f);
+      v.act_m = fmin(ACT_SKIN_M, h.skin_m);
       int const n3 = v.n3;
       double* W = A.ws + h.ws_off;
       auto vecp = [&](int k) { return W + static_cast<long>(k) * n3; };
@@ -445,6 +668,7 @@ namespace flyg {
       double* dgp = vecp(7);
       double* axp = vecp(8);
       double* xb = vecp(9);
+      double* xr = vecp(nvec(o.n, o.rn) - 1);   // positions at the last active-list refresh
       int const t0 = 10, r0v = 13 + 2 * o.n;
       Core<T> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
                  lds_core, lds_core + 32, o.n, 0, n3};
@@ -466,7 +690,7 @@ namespace flyg {
         v.cur[i] = b;
       }
       __syncthreads();
-      double e_base = region_energy<T, L>(A, v, tm, true, err);
+      double e_base = region_energy<T, L, MULTI>(A, v, tm, true, err);
       for (int i = tm.tid; i < n3; i += T) {
         int a = i / 3, c = i % 3;
         double4 b = v.base[a];
@@ -479,17 +703,27 @@ namespace flyg {
       int ret = 2;   // iter_max
       int steps = o.max_steps;
       double curv = 0;
-      double maxd = 0;
+      double maxd = 0, maxp = 0;
+      int refreshes = 0;
 
-      // Rows built at the base positions with cut-off r_c + skin stay complete while every pair that is
-      // missing from them stays beyond r_c: a mobile-frozen pair needs d_i < skin and a mobile-mobile
-      // pair d_i + d_j < skin, i.e. d1 + d2 < skin for the two largest mobile displacements d1 >= d2.
-      auto displacement_ok = [&]() {
+      // Rows built at the base positions stay complete while every pair that is missing from them stays
+      // beyond r_c: a mobile-frozen pair needs d_i < skin_f and a mobile-mobile pair d_i + d_j < skin_m,
+      // i.e. d1 < skin_f and d1 + d2 < skin_m for the two largest mobile displacements d1 >= d2 (plus
+      // the dimer separation, as the second image sits delta_r further along the axis).
+      long gticks = 0;
+      auto timed_gradient = [&](double* g) -> unsigned {
+        long t = static_cast<long>(wall_clock64());
+        unsigned r = region_gradient<T, L, MIX, MULTI>(A, v, tm, g);
+        gticks += static_cast<long>(wall_clock64()) - t;
+        return r;
+      };
+      // Two largest mobile displacements |x_i - ref_i| (a1 >= a2) over the workgroup.
+      auto top2 = [&](double const* ref, double& a1, double& a2) {
         double m1 = 0, m2 = 0;
         for (int a = tm.tid; a < v.nm; a += T) {
-          double dx = x[3 * a] - xb[3 * a], dy = x[3 * a + 1] - xb[3 * a + 1], dz = x[3 * a + 2] - xb[3 * a + 2];
+          double dx = x[3 * a] - ref[3 * a], dy = x[3 * a + 1] - ref[3 * a + 1], dz = x[3 * a + 2] - ref[3 * a + 2];
           double d = sqrt(dx * dx + dy * dy + dz * dz);
-          if (!(d <= m1)) {   // also takes NaN, which then fails the test below
+          if (!(d <= m1)) {   // also takes NaN, which then fails the tests of the callers
             m2 = m1;
             m1 = d;
           } else if (d > m2) {
@@ -507,7 +741,8 @@ namespace flyg {
           tm.red[4 * (tm.tid >> 6) + 1] = m2;
         }
         __syncthreads();
-        double a1 = 0, a2 = 0;
+        a1 = 0;
+        a2 = 0;
         for (int w = 0; w < T / 64; ++w) {
           double b1 = tm.red[4 * w], b2 = tm.red[4 * w + 1];
           double hi = fmax(a1, b1), lo = fmax(fmin(a1, b1), fmax(a2, b2));
@@ -515,17 +750,37 @@ namespace flyg {
           a2 = lo;
         }
         __syncthreads();
+      };
+      auto displacement_ok = [&]() {
+        double a1, a2;
+        top2(xb, a1, a2);
         maxd = fmax(maxd, a1);
-        return a1 + a2 < h.skin;
+        maxp = fmax(maxp, a1 + a2);
+        return a1 + dr < h.skin_f && a1 + a2 + 2.0 * dr < h.skin_m;
+      };
+      // Active lists built at xr stay complete for evaluations at x and x + dr * axis while the same
+      // condition holds with the active skins and displacements from xr.
+      bool active_built = false;
+      auto ensure_active = [&]() {
+        if (active_built) {
+          double a1, a2;
+          top2(xr, a1, a2);
+          if (a1 + dr < v.act_f && a1 + a2 + 2.0 * dr < v.act_m) return;
+        }
+        refresh_active<T, L>(A, v, tm);
+        tm.lin(xr, n3, x, 1.0);
+        active_built = true;
+        ++refreshes;
       };
 
       // Rotor::eff_gradient (rotor.cpp:49-164); returns false on a spline error.
       auto eff_gradient = [&](double& mag_g) -> bool {
         rc.k = 0;
         place(v, tm, x, 0.0, nullptr);
-        err |= region_gradient<T, L, MIX>(A, v, tm, g0);
+        ensure_active();
+        err |= timed_gradient(g0);
         place(v, tm, x, dr, ax);
-        err |= region_gradient<T, L, MIX>(A, v, tm, g1);
+        err |= timed_gradient(g1);
         grads += 2;
         if (err) return false;
         for (int i = 0;; ++i) {
@@ -549,7 +804,7 @@ namespace flyg {
           double ct = cos(theta_1), st = sin(theta_1);
           tm.lin(axp, n3, ax, ct, theta, st);
           place(v, tm, x, dr, axp);
-          err |= region_gradient<T, L, MIX>(A, v, tm, g1p);
+          err |= timed_gradient(g1p);
           ++grads;
           if (err) return false;
           tm.lin(dg, n3, g1p, 1.0, g0, -1.0);
@@ -631,7 +886,7 @@ namespace flyg {
       if (ret != EXIT_REGION && ret != EXIT_OOB) {
         place(v, tm, x, 0.0, nullptr);
         unsigned e2 = 0;
-        double e_sp = region_energy<T, L>(A, v, tm, false, e2);
+        double e_sp = region_energy<T, L, MULTI>(A, v, tm, false, e2);
         barrier = e_sp - e_base;
         if (e2) ret = EXIT_OOB;
       }
@@ -644,23 +899,27 @@ namespace flyg {
         r.exit_code = ret;
         r.steps = steps;
         r.rot_iters = rot_iters;
-        r.pad = 0;
+        r.refreshes = refreshes;
         r.grads = grads;
         r.curv = curv;
         r.barrier = barrier;
         r.e_base = e_base;
         r.max_disp = maxd;
+        r.max_pair = maxp;
+        r.ticks = static_cast<long>(wall_clock64()) - t_begin;
+        r.grad_ticks = gticks;
         A.out[sid] = r;
       }
       __syncthreads();
     }
 
-    template <int T, int L, bool MIX>
+    template <int T, int L, bool MIX, bool MULTI>
     __global__ __launch_bounds__(T) void k_search(BatchArgs A) {
       __shared__ double red[4 * T];
       __shared__ double lds_core[128];
       __shared__ int s_id;
       __shared__ double3 lds_S[27];
+      __shared__ double4 lds_cur[lds_rows<T>()];
       Team<T> tm{red, static_cast<int>(threadIdx.x)};
       while (true) {
         if (threadIdx.x == 0) s_id = static_cast<int>(atomicAdd(A.queue, 1u));
@@ -668,7 +927,7 @@ namespace flyg {
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX>(A, sid, tm, lds_core, lds_S);
+        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur);
       }
     }
 
@@ -765,6 +1024,7 @@ namespace flyg {
       DBuf<double> d_rho;
       DBuf<uint8_t> d_type;
       DBuf<double3> d_S;
+      int single_type = -1;           // the one species present, -1 if several
       long active = 0;                // searches queued or in flight
       bool retired = false;           // removal requested; freed once idle
       std::size_t dev_bytes() const { return d_base.cap * sizeof(double4) + d_rho.cap * 8 + d_type.cap + d_S.cap * sizeof(double3); }
@@ -789,6 +1049,12 @@ namespace flyg {
       require(n < MAX_ATOMS, "too many atoms");
       S->base.resize(3 * n);
       S->types.assign(types, types + n);
+      S->single_type = static_cast<int>(types[0]);
+      for (long i = 1; i < n; ++i)
+        if (types[i] != types[0]) {
+          S->single_type = -1;
+          break;
+        }
       std::vector<double4> b4(n);
       std::vector<uint8_t> t8(n);
       for (long i = 0; i < n; ++i) {
@@ -847,10 +1113,10 @@ namespace flyg {
       SysState* sys = nullptr;
       int64_t centre;
       double radius;
-      double skin;
+      double skin_f, skin_m;        // mobile-frozen / mobile-mobile Verlet skins
       std::vector<int> mobile;      // global indices (sorted)
       std::vector<double> x, ax;    // 3*nm start state (continuous with the base positions)
-      std::shared_ptr<Region> region;   // built for `skin`, reset when the skin grows
+      std::shared_ptr<Region> region;   // built for the skins, reset when they grow
     };
 
     struct Result {
@@ -872,8 +1138,8 @@ namespace flyg {
 
     // Queues one search of system S: mobile set around `centre`, start state continuous with the base.
     // Start positions/axis come either dense (3n arrays) or sparse (listed atoms only; the rest at base).
-    Pending make_pending(SysState& S, int64_t id, int64_t centre, double radius, double skin, double const* sp,
-                         double const* sa, int64_t const* kidx, long nk) {
+    Pending make_pending(SysState& S, double rc, int64_t id, int64_t centre, double radius, double skin,
+                         double const* sp, double const* sa, int64_t const* kidx, long nk) {
       require(centre >= 0 && centre < S.n, "centre out of range");
       require(radius > 0 && radius < 0.5 * std::fmin(S.box.ext[0], std::fmin(S.box.ext[1], S.box.ext[2])) - 1.0,
               "active radius must be positive and below half the box");
@@ -882,7 +1148,6 @@ namespace flyg {
       P.sys = &S;
       P.centre = centre;
       P.radius = radius;
-      P.skin = skin;
       mobile_set(S, centre, radius, P.mobile);
       int nm = static_cast<int>(P.mobile.size());
       P.x.resize(3 * nm);
@@ -907,18 +1172,38 @@ namespace flyg {
       } else {
         for (int i = 0; i < nm; ++i) put(P.mobile[i], sp + 3L * P.mobile[i], sa + 3L * P.mobile[i]);
       }
+      // Skins: the start state itself (kicked atoms) must sit well inside the rows.
+      double d1 = 0, d2 = 0;
+      for (int i = 0; i < nm; ++i) {
+        double dx = P.x[3 * i] - S.base[3 * P.mobile[i]], dy = P.x[3 * i + 1] - S.base[3 * P.mobile[i] + 1],
+               dz = P.x[3 * i + 2] - S.base[3 * P.mobile[i] + 2];
+        double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (d > d1) {
+          d2 = d1;
+          d1 = d;
+        } else if (d > d2) {
+          d2 = d;
+        }
+      }
+      double cap = S.skin_cap(rc);
+      P.skin_f = std::min(cap, std::max(skin, d1 + 1.0));
+      P.skin_m = std::min(cap, std::max(std::max(2.0 * skin, P.skin_f), d1 + d2 + 1.5));
       return P;
     }
 
-    void build_region(SysState const& S, Pending const& P, double rL, std::vector<int>& local_of, Region& g) {
+    // Region of a search: mobile atoms, then S1 (frozen atoms within rF = r_c + skin_f of a mobile atom),
+    // with rows of packed (shift, local index) entries: a mobile row lists mobile atoms within
+    // rM = r_c + skin_m and S1 atoms within rF; an S1 row lists the mobile atoms within rF.
+    void build_region(SysState const& S, Pending const& P, double rc, std::vector<int>& local_of, Region& g) {
+      double const rF = rc + P.skin_f, rM = rc + P.skin_m;
       int const nm = static_cast<int>(P.mobile.size());
       g.nm = nm;
       g.gidx = P.mobile;
       for (int i = 0; i < nm; ++i) local_of[P.mobile[i]] = i;
       std::vector<int> s1;
       for (int i = 0; i < nm; ++i)
-        S.grid.query(&S.base[3 * P.mobile[i]], rL, [&](int j, int, double r2) {
-          if (r2 < rL * rL && local_of[j] == -1) {
+        S.grid.query(&S.base[3 * P.mobile[i]], rF, [&](int j, int, double r2) {
+          if (r2 < rF * rF && local_of[j] == -1) {
             local_of[j] = -2;   // collected into S1
             s1.push_back(j);
           }
@@ -934,16 +1219,15 @@ namespace flyg {
         int gi = g.gidx[r];
         bool mob = r < nm;
         row.clear();
-        S.grid.query(&S.base[3 * gi], rL, [&](int j, int sc, double r2) {
-          if (!(r2 < rL * rL)) return;
+        S.grid.query(&S.base[3 * gi], mob ? rM : rF, [&](int j, int sc, double r2) {
           if (j == gi && sc == 13) return;
           int lj = local_of[j];
-          if (mob) {
-            if (lj < 0) return;   // cannot happen: S1 holds every frozen atom within r_L of a mobile atom
-            row.push_back((static_cast<unsigned>(sc) << NB_SHIFT_BITS) | static_cast<unsigned>(lj));
-          } else if (lj >= 0 && lj < nm) {
-            row.push_back((static_cast<unsigned>(sc) << NB_SHIFT_BITS) | static_cast<unsigned>(lj));
-          }
+          if (lj < 0) return;   // frozen beyond rF of every mobile atom
+          bool jm = lj < nm;
+          if (!mob && !jm) return;   // S1 rows hold mobile neighbours only
+          double rr = (mob && jm) ? rM : rF;
+          if (!(r2 < rr * rr)) return;
+          row.push_back((static_cast<unsigned>(sc) << NB_SHIFT_BITS) | static_cast<unsigned>(lj));
         });
         std::sort(row.begin(), row.end());
         g.rowlen[r] = static_cast<int>(row.size());
@@ -983,7 +1267,7 @@ namespace flyg {
         if (static_cast<long>(scratch[t].size()) < nmax) scratch[t].assign(nmax, -1);
         Pending& P = work[need[q]];
         auto g = std::make_shared<Region>();
-        build_region(*P.sys, P, rc + P.skin, scratch[t], *g);
+        build_region(*P.sys, P, rc, scratch[t], *g);
         P.region = std::move(g);
       });
     }
@@ -991,19 +1275,22 @@ namespace flyg {
     using KernelFn = void (*)(BatchArgs);
 
     template <int T, int L>
-    KernelFn pick(bool mix) {
-      return mix ? reinterpret_cast<KernelFn>(&k_search<T, L, true>) : reinterpret_cast<KernelFn>(&k_search<T, L, false>);
+    KernelFn pick(bool mix, bool multi) {
+      if (multi)
+        return mix ? reinterpret_cast<KernelFn>(&k_search<T, L, true, true>)
+                   : reinterpret_cast<KernelFn>(&k_search<T, L, false, true>);
+      return mix ? reinterpret_cast<KernelFn>(&k_search<T, L, true, false>)
+                 : reinterpret_cast<KernelFn>(&k_search<T, L, false, false>);
     }
 
-    KernelFn kernel_for(int T, int L, bool mix) {
-      if (T == 256 && L == 4) return pick<256, 4>(mix);
-      if (T == 256 && L == 8) return pick<256, 8>(mix);
-      if (T == 256 && L == 16) return pick<256, 16>(mix);
-      if (T == 512 && L == 4) return pick<512, 4>(mix);
-      if (T == 512 && L == 8) return pick<512, 8>(mix);
-      if (T == 512 && L == 16) return pick<512, 16>(mix);
-      if (T == 128 && L == 8) return pick<128, 8>(mix);
-      throw Error(FLYG_INVALID_ARG, "unsupported (block_threads, lanes_per_atom); use 128/256/512 and 4/8/16");
+    KernelFn kernel_for(int T, int L, bool mix, bool multi) {
+      if (T == 256 && L == 8) return pick<256, 8>(mix, multi);
+      if (T == 256 && L == 16) return pick<256, 16>(mix, multi);
+      if (T == 512 && L == 8) return pick<512, 8>(mix, multi);
+      if (T == 512 && L == 16) return pick<512, 16>(mix, multi);
+      if (T == 1024 && L == 8) return pick<1024, 8>(mix, multi);
+      if (T == 1024 && L == 16) return pick<1024, 16>(mix, multi);
+      throw Error(FLYG_INVALID_ARG, "unsupported (block_threads, lanes_per_atom); use 256, 512 or 1024 with 8 or 16");
     }
 
     DimOpt dim_opt(flyg_dimer_options const& o, flyg_rotor_options const& r) {
@@ -1107,7 +1394,7 @@ namespace flyg {
       std::size_t b = 4096;
       for (auto const& P : work) {
         Region const& g = *P.region;
-        b += sizeof(SearchHdr) + static_cast<std::size_t>(g.nr) * (4 + 4 + 32 + 32 + 1 + 8 + 8) + g.ent.size() * 4 +
+        b += sizeof(SearchHdr) + static_cast<std::size_t>(g.nr) * (4 + 4 + 4 + 32 + 32 + 1 + 8 + 8) + g.ent.size() * 8 +
              static_cast<std::size_t>(3 * g.nm) * 8 * (4 + nvec(dopt.n, dopt.rn)) + sizeof(SearchOut) + 3 * 256;
       }
       return b;
@@ -1133,7 +1420,8 @@ namespace flyg {
         h.row_off = nrow;
         h.v3_off = n3;
         h.ws_off = nws;
-        h.skin = P.skin;
+        h.skin_f = P.skin_f;
+        h.skin_m = P.skin_m;
         h.sbase = P.sys->d_base.get();
         h.srho = P.sys->d_rho.get();
         h.stype = P.sys->d_type.get();
@@ -1151,7 +1439,8 @@ namespace flyg {
                   o_ent = in.add<unsigned>(nent), o_sx = in.add<double>(n3), o_sax = in.add<double>(n3);
       Layout sc;
       std::size_t o_cur = sc.add<double4>(nreg), o_base = sc.add<double4>(nreg), o_type = sc.add<uint8_t>(nreg),
-                  o_rhof = sc.add<double>(nreg), o_fp = sc.add<double>(nreg), o_ws = sc.add<double>(nws),
+                  o_rhof = sc.add<double>(nreg), o_fp = sc.add<double>(nreg), o_act = sc.add<unsigned>(nent),
+                  o_alen = sc.add<int>(nreg), o_ws = sc.add<double>(nws),
                   o_q = sc.add<unsigned>(1);
       std::size_t zero_end = sc.size;
       F.o_out = sc.add<SearchOut>(ns);
@@ -1206,6 +1495,8 @@ namespace flyg {
       A.fp = reinterpret_cast<double*>(ds + o_fp);
       A.rowptr = reinterpret_cast<int const*>(di + o_row);
       A.ent = reinterpret_cast<unsigned const*>(di + o_ent);
+      A.act = reinterpret_cast<unsigned*>(ds + o_act);
+      A.alen = reinterpret_cast<int*>(ds + o_alen);
       A.ws = reinterpret_cast<double*>(ds + o_ws);
       A.start_x = reinterpret_cast<double const*>(di + o_sx);
       A.start_ax = reinterpret_cast<double const*>(di + o_sax);
@@ -1214,10 +1505,14 @@ namespace flyg {
       A.out = reinterpret_cast<SearchOut*>(ds + F.o_out);
       A.queue = reinterpret_cast<unsigned*>(ds + o_q);
       A.o = dopt;
-      A.multi = E.tab.ntypes > 1;
 
       unsigned grid = static_cast<unsigned>(std::max<long>(1, std::min<long>(slots, ns)));
-      KernelFn fn = kernel_for(T, L, mix);
+      // One species present in every system of the launch: species indices become kernel constants.
+      int t0 = F.work.empty() ? -1 : F.work[0].sys->single_type;
+      for (auto const& P : F.work)
+        if (P.sys->single_type != t0) t0 = -1;
+      A.t0 = t0 < 0 ? 0 : t0;
+      KernelFn fn = kernel_for(T, L, mix, E.tab.ntypes > 1 && t0 < 0);
       HIPCHK(hipEventRecord(F.e_a, F.st));
       hipLaunchKernelGGL(fn, dim3(grid), dim3(T), 0, F.st, A);
       HIPCHK(hipGetLastError());
@@ -1225,6 +1520,17 @@ namespace flyg {
       HIPCHK(hipMemcpyAsync(F.h_out.p, ds + F.o_out, F.out_bytes, hipMemcpyDeviceToHost, F.st));
       HIPCHK(hipEventRecord(F.e_done, F.st));
       F.busy = true;
+    }
+
+    double wall_clock_khz() {
+      static thread_local int khz = 0;
+      if (!khz) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) khz = 100000;
+      }
+      return static_cast<double>(khz);
     }
 
     // After F finished: finished searches go to `done`, searches that left their region (and may
@@ -1241,9 +1547,10 @@ namespace flyg {
         int nm = P.region->nm;
         ++P.launches;
         double cap = P.sys->skin_cap(rc);
-        if (o.exit_code == EXIT_REGION && P.skin < cap) {
+        if (o.exit_code == EXIT_REGION && (P.skin_f < cap || P.skin_m < cap)) {
           Pending q = std::move(P);
-          q.skin = std::min(cap, std::max(q.skin * 1.5, 2.0 * o.max_disp + 0.5));
+          q.skin_f = std::min(cap, std::max(q.skin_f * 1.5, o.max_disp + 1.0));
+          q.skin_m = std::min(cap, std::max(std::max(q.skin_m * 1.5, q.skin_f), o.max_pair + 1.5));
           q.region.reset();
           requeue.push_back(std::move(q));
           off += 3L * nm;
@@ -1254,11 +1561,14 @@ namespace flyg {
         R.st.exit_code = o.exit_code;
         R.st.steps = o.steps;
         R.st.grads = o.grads;
-        R.st.rebuilds = 0;
+        R.st.rebuilds = o.refreshes;
         R.st.curv = o.curv;
         R.st.energy = o.barrier;
         R.st.max_disp = o.max_disp;
         R.st.launches = P.launches;
+        R.st.rot_iters = o.rot_iters;
+        R.st.device_ms = static_cast<double>(o.ticks) / wall_clock_khz();
+        R.st.device_grad_ms = static_cast<double>(o.grad_ticks) / wall_clock_khz();
         R.mobile = P.mobile;
         R.x.assign(ox + off, ox + off + 3L * nm);
         R.ax.assign(oax + off, oax + off + 3L * nm);
@@ -1281,7 +1591,7 @@ namespace flyg {
       if (work.empty()) return;
       ensure_regions(work, E.tab.rc);
       d.max_steps = std::min(d.max_steps, 20);
-      int const cand[][2] = {{256, 4}, {256, 8}, {256, 16}, {512, 8}, {512, 16}, {128, 8}};
+      int const cand[][2] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {1024, 8}, {1024, 16}};
       double best = 1e300;
       for (auto const& c : cand) {
         std::vector<Pending> w = work;
@@ -1516,7 +1826,7 @@ int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const doub
     long const n = b->sys->n;
     for (int k = 0; k < count; ++k) {
       int64_t id = static_cast<int64_t>(b->results.size());
-      b->pending.push_back(make_pending(*b->sys, id, centre[k], radius[k], b->bopt.list_skin,
+      b->pending.push_back(make_pending(*b->sys, b->eam->tab.rc, id, centre[k], radius[k], b->bopt.list_skin,
                                         start_pos + 3 * n * static_cast<long>(k),
                                         start_axis + 3 * n * static_cast<long>(k), nullptr, 0));
       b->results.emplace_back();
@@ -1670,7 +1980,7 @@ int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* 
     for (int k = 0; k < count; ++k) {
       long a = kick_offsets[k], b = kick_offsets[k + 1];
       require(a >= 0 && b >= a, "kick offsets must be non-decreasing");
-      add.push_back(make_pending(S, s->next_id + k, centre[k], radius[k], s->sopt.list_skin, kick_pos + 3 * a,
+      add.push_back(make_pending(S, s->eam->tab.rc, s->next_id + k, centre[k], radius[k], s->sopt.list_skin, kick_pos + 3 * a,
                                  kick_axis + 3 * a, kick_atoms + a, b - a));
     }
     for (int k = 0; k < count; ++k) {

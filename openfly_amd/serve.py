@@ -24,7 +24,7 @@ from . import systems
 @dataclasses.dataclass
 class Request:
     t: float                 # arrival time (s after the start)
-    system: systems.System
+    system: systems.System   # the kMC system it came from (shared by that loop's requests)
     centres: np.ndarray
     radii: np.ndarray
     kicks: list              # per search: (atoms, positions (m, 3), axis (m, 3))
@@ -78,20 +78,33 @@ def sparse_kick(s: systems.System, centre, radius, rng, stddev=0.6, r_pert=4.0):
     return near.astype(np.int64), pos, ax
 
 
-def make_requests(count, rate=200.0, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), searches=(8, 64),
-                  radii=(4.0, 9.0)):
-    """Poisson arrivals at `rate`/s; per request a fresh bcc Fe system with N log-uniform in [n_min, n_max]
-    atoms and a defect fraction uniform in `defects` (half vacancies, half tetrahedral interstitials),
-    and a uniform number of searches in `searches` with radii uniform in `radii`."""
+def make_pool(count, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), relax=None):
+    """`count` kMC systems: bcc Fe with N log-uniform in [n_min, n_max] atoms and a defect fraction uniform
+    in `defects` (half vacancies, half tetrahedral interstitials).  relax(system) -> positions, if given,
+    turns each into its relaxed minimum (the state a kMC loop searches from)."""
     rng = np.random.default_rng(seed)
-    t = np.cumsum(rng.exponential(1.0 / rate, count))
-    reqs = []
+    pool = []
     for q in range(count):
         n_target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
         cells = max(10, int(round((n_target / 2.0) ** (1.0 / 3.0))))
         frac = rng.uniform(*defects)
         n_def = max(2, int(round(frac * 2 * cells ** 3)))
         s = systems.defect_system(cells, n_def // 2, n_def - n_def // 2, int(rng.integers(1 << 30)))
+        if relax is not None:
+            s.pos = relax(s)
+        s.pool_index = q
+        pool.append(s)
+    return pool
+
+
+def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0)):
+    """Poisson arrivals at `rate`/s, each from a uniformly chosen system of `pool`, asking for a uniform
+    number of searches in `searches` around defect-adjacent atoms with radii uniform in `radii`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    reqs = []
+    for q in range(count):
+        s = pool[int(rng.integers(len(pool)))]
         k = int(rng.integers(searches[0], searches[1] + 1))
         centres = sample_centres(s, k, rng)
         rad = rng.uniform(radii[0], radii[1], k)
@@ -100,17 +113,47 @@ def make_requests(count, rate=200.0, seed=4, n_min=2000, n_max=250000, defects=(
     return reqs
 
 
-def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_seconds=600.0):
-    """Replays `reqs` against `server` in real time (arrival times relative to the call).  Admission is
-    FIFO; a request whose system the memory budget cannot hold waits until running requests finish.
-    Returns a summary dict; per-search results land in each request's `results`."""
+def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_seconds=600.0, report=None):
+    """Replays `reqs` against `server` in real time (arrival times relative to the call).  A kMC system is
+    admitted on its first request and stays resident for later ones; admission is FIFO, and when the
+    memory budget is full the least recently used idle system is evicted, else the request waits for
+    running ones to finish.  Returns a summary dict; per-search results land in each request's
+    `results`.  `report(msg)`, if given, gets a progress line about every 10 s."""
     pending = collections.deque()
     owner = {}
+    resident = collections.OrderedDict()   # id(system) -> [sys_id, live requests], LRU order
     i = 0
     t0 = time.perf_counter()
     live = 0
+    finished = 0
+    admitted_systems = 0
+    next_report = 10.0
+
+    def admit(r):
+        nonlocal admitted_systems
+        key = id(r.system)
+        if key in resident:
+            resident.move_to_end(key)
+            return resident[key]
+        while True:
+            try:
+                sid = server.add_system(fg.Box(r.system.basis), r.system.pos, r.system.types)
+                break
+            except fg.Busy:
+                idle = [k for k, v in resident.items() if v[1] == 0]
+                if not idle:
+                    return None
+                server.remove_system(resident.pop(idle[0])[0])
+                server.progress()
+        admitted_systems += 1
+        resident[key] = [sid, 0]
+        return resident[key]
+
     while i < len(reqs) or pending or live:
         now = time.perf_counter() - t0
+        if report is not None and now >= next_report:
+            report(f"{now:.0f} s: {i} arrived, {finished} requests done, {live} running, {len(pending)} waiting")
+            next_report += 10.0
         if now > max_seconds:
             raise fg.FlyError(f"serving did not finish within {max_seconds} s")
         while i < len(reqs) and reqs[i].t <= now:
@@ -118,10 +161,11 @@ def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_second
             i += 1
         while pending:
             r = pending[0]
-            try:
-                r.sys_id = server.add_system(fg.Box(r.system.basis), r.system.pos, r.system.types)
-            except fg.Busy:
+            slot = admit(r)
+            if slot is None:
                 break
+            slot[1] += 1
+            r.sys_id = slot[0]
             ids = server.submit(r.sys_id, r.centres, r.radii, r.kicks)
             r.remaining = len(ids)
             r.admitted = time.perf_counter() - t0
@@ -138,9 +182,13 @@ def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_second
             r.remaining -= 1
             if r.remaining == 0:
                 r.done = time.perf_counter() - t0
-                server.remove_system(r.sys_id)
+                resident[id(r.system)][1] -= 1
                 live -= 1
+                finished += 1
     wall = time.perf_counter() - t0
+    for sid, _ in resident.values():
+        server.remove_system(sid)
+    server.progress()
     lat = np.array([r.done - r.t for r in reqs])
     res = [x for r in reqs for x in r.results.values()]
     return {
@@ -156,5 +204,6 @@ def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_second
         "mean_queue_wait_s": float(np.mean([r.admitted - r.t for r in reqs])),
         "offered_requests_per_s": len(reqs) / max(reqs[-1].t, 1e-9),
         "mean_atoms": float(np.mean([r.system.n for r in reqs])),
+        "systems_admitted": admitted_systems,
         "mean_searches_per_request": float(np.mean([len(r.centres) for r in reqs])),
     }

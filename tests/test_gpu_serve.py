@@ -163,14 +163,23 @@ def test_served_barriers_match_fp64_energy(ctx, wen):
     assert ok >= 2
 
 
-def test_serve_loop_small(ctx):
-    """A short Poisson replay with small systems (synthetic FS potential): every request completes and
-    every search reports an exit code."""
-    pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
-    reqs = serve.make_requests(6, rate=200.0, seed=1, n_min=2000, n_max=6000, searches=(4, 8))
+def test_serve_loop_small(ctx, wen):
+    """A short Poisson replay over three relaxed kMC systems: every request completes, systems are
+    admitted once and reused, and the server is empty afterwards."""
+    pot = wen["pot"]
+
+    def relax(s):
+        mn, _, failed, _ = fa.LBFGS(ctx, fa.Box(s.basis), f2norm=1e-4, iter_max=4000).minimise(
+            s.pos, s.types, np.zeros(s.n, np.uint8), pot)
+        assert not failed
+        return mn
+
+    pool = serve.make_pool(3, seed=1, n_min=2000, n_max=6000, relax=relax)
+    reqs = serve.make_requests(pool, 8, rate=200.0, seed=2, searches=(4, 8))
     srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
     out = serve.serve(srv, reqs)
     print(out)
-    assert out["requests"] == 6 and out["searches"] == sum(len(r.centres) for r in reqs)
+    assert out["requests"] == 8 and out["searches"] == sum(len(r.centres) for r in reqs)
+    assert out["systems_admitted"] <= 3
     assert all(r.done >= r.t for r in reqs)
     assert srv.status()["systems"] == 0
