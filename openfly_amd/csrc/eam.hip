@@ -92,6 +92,29 @@ namespace flyg {
     template <int L, bool MULTI, bool ENERGY, bool MIX>
     __global__ __launch_bounds__(NT) void k_density(EAMTables T, Shifts S, PairArgs A, double4* __restrict__ posf,
                                                     double* __restrict__ eatom, unsigned* __restrict__ err) {
+      __shared__ SplineDesc s_phi[MAX_TYPES * MAX_TYPES], s_V[MAX_TYPES * MAX_TYPES];
+      if (MULTI) {   // per-lane species pairs: table descriptors from LDS, not from the argument block
+        int t = threadIdx.x;
+        if (t < T.ntypes * T.ntypes) {
+          s_phi[t] = T.phi[t];
+          s_V[t] = T.V[t];
+        }
+        __syncthreads();
+      }
+      auto PHI = [&](int k) -> SplineDesc const& {
+        if constexpr (MULTI) {
+          return s_phi[k];
+        } else {
+          return T.phi[k];
+        }
+      };
+      auto VV = [&](int k) -> SplineDesc const& {
+        if constexpr (MULTI) {
+          return s_V[k];
+        } else {
+          return T.V[k];
+        }
+      };
       long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
@@ -111,11 +134,11 @@ namespace flyg {
             if (MIX && !ENERGY) {
               float r2f = static_cast<float>(p.r2);
               float rf = r2f * __frsqrt_rn(r2f);
-              rho.add_slot(q, static_cast<double>(spl32f<0>(T, T.phi[tj + T.ntypes * ti], rf, e)));
+              rho.add_slot(q, static_cast<double>(spl32f<0>(T, PHI(tj + T.ntypes * ti), rf, e)));
             } else {
               double r = sqrt(p.r2);
-              rho.add_slot(q, spl64<0>(T, T.phi[tj + T.ntypes * ti], r, e));
-              if (ENERGY) vs.add_slot(q, spl64<0>(T, T.V[vidx(T, ti, tj)], r, e));
+              rho.add_slot(q, spl64<0>(T, PHI(tj + T.ntypes * ti), r, e));
+              if (ENERGY) vs.add_slot(q, spl64<0>(T, VV(vidx(T, ti, tj)), r, e));
             }
           }
         });
@@ -134,6 +157,29 @@ namespace flyg {
     template <int L, bool MULTI, bool MIX>
     __global__ __launch_bounds__(NT) void k_force(EAMTables T, Shifts S, PairArgs A, double4 const* __restrict__ posf,
                                                   double* __restrict__ grad, unsigned* __restrict__ err) {
+      __shared__ SplineDesc s_phi[MAX_TYPES * MAX_TYPES], s_V[MAX_TYPES * MAX_TYPES];
+      if (MULTI) {   // per-lane species pairs: table descriptors from LDS, not from the argument block
+        int t = threadIdx.x;
+        if (t < T.ntypes * T.ntypes) {
+          s_phi[t] = T.phi[t];
+          s_V[t] = T.V[t];
+        }
+        __syncthreads();
+      }
+      auto PHI = [&](int k) -> SplineDesc const& {
+        if constexpr (MULTI) {
+          return s_phi[k];
+        } else {
+          return T.phi[k];
+        }
+      };
+      auto VV = [&](int k) -> SplineDesc const& {
+        if constexpr (MULTI) {
+          return s_V[k];
+        } else {
+          return T.V[k];
+        }
+      };
       long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
@@ -154,16 +200,16 @@ namespace flyg {
               float r2f = static_cast<float>(p.r2);
               float rinv = __frsqrt_rn(r2f);
               float rf = r2f * rinv;
-              float vp = spl32f<1>(T, T.V[vidx(T, ti, tj)], rf, e);
-              float pa = spl32f<1>(T, T.phi[tj + T.ntypes * ti], rf, e);
-              float pb = MULTI ? spl32f<1>(T, T.phi[ti + T.ntypes * tj], rf, e) : pa;
+              float vp = spl32f<1>(T, VV(vidx(T, ti, tj)), rf, e);
+              float pa = spl32f<1>(T, PHI(tj + T.ntypes * ti), rf, e);
+              float pb = MULTI ? spl32f<1>(T, PHI(ti + T.ntypes * tj), rf, e) : pa;
               float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(p.w) * pb;
               cf = static_cast<double>(mag * rinv);
             } else {
               double r = sqrt(p.r2);
-              double vp = spl64<1>(T, T.V[vidx(T, ti, tj)], r, e);
-              double pa = spl64<1>(T, T.phi[tj + T.ntypes * ti], r, e);
-              double pb = MULTI ? spl64<1>(T, T.phi[ti + T.ntypes * tj], r, e) : pa;
+              double vp = spl64<1>(T, VV(vidx(T, ti, tj)), r, e);
+              double pa = spl64<1>(T, PHI(tj + T.ntypes * ti), r, e);
+              double pb = MULTI ? spl64<1>(T, PHI(ti + T.ntypes * tj), r, e) : pa;
               cf = (vp + pi.w * pa + p.w * pb) / r;
             }
             gx.add_slot(q, -cf * p.dx);

@@ -83,7 +83,9 @@ struct flyg_list_s {
   flyg::DBuf<unsigned> eid;      // packed (shift, real) of ent
   flyg::DBuf<int> ecell;         // cell of ent
   flyg::DBuf<int> ccount, cstart, ccursor;
-  flyg::DBuf<double4> spos;      // ent sorted by cell
+  flyg::DBuf<double4> spos;      // ent sorted by cell (packed-id order inside a cell)
+  flyg::DBuf<unsigned> usid;     // packed ids scattered by cell, unordered inside a cell
+  flyg::DBuf<int> rslot;         // sorted slot of each real atom
   flyg::DBuf<unsigned> sid;
   flyg::DBuf<unsigned> nbr;      // n x K packed entries
   flyg::DBuf<int> ncnt;          // entries per atom

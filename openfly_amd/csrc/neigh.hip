@@ -99,142 +99,6 @@ namespace flyg {
       return (dx * dx + dy * dy) + dz * dz;
     }
 
-    __global__ void k_canon(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos) {
-      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (i >= n) return;
-      double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-      canon_image(b, x);
-      pos[i] = make_double4(x[0], x[1], x[2], 0.0);
-    }
-
-    __global__ void k_reals(double4 const* __restrict__ pos, long n, double4* __restrict__ ent, unsigned* __restrict__ eid) {
-      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (i >= n) return;
-      ent[i] = pos[i];
-      eid[i] = (13u << NB_SHIFT_BITS) | static_cast<unsigned>(i);
-    }
-
-    // make_ghosts for one real atom: the tree of chained images over the periodic axes.
-    template <bool WRITE>
-    __global__ void k_ghosts(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
-                             unsigned* __restrict__ eid, unsigned* __restrict__ counter, long cap) {
-      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (i >= n) return;
-      double cx[27][3];
-      int cs[27];
-      int m = 1;
-      double4 p = pos[i];
-      cx[0][0] = p.x;
-      cx[0][1] = p.y;
-      cx[0][2] = p.z;
-      cs[0] = 13;
-      int const pow3[3] = {1, 3, 9};
-      for (int ax = 0; ax < 3; ++ax) {
-        if (!b.per[ax]) continue;
-        int const end = m;
-        for (int sgn = 1; sgn >= -1; sgn -= 2) {
-          for (int c = 0; c < end; ++c) {
-            double im[3] = {cx[c][0], cx[c][1], cx[c][2]};
-            if (gen_image(b, rc, sgn, im, ax)) {
-              cx[m][0] = im[0];
-              cx[m][1] = im[1];
-              cx[m][2] = im[2];
-              cs[m] = cs[c] + sgn * pow3[ax];
-              ++m;
-            }
-          }
-        }
-      }
-      unsigned g = static_cast<unsigned>(m - 1);
-      if (g == 0) return;
-      unsigned base = atomicAdd(counter, g);
-      if (!WRITE) return;
-      for (int k = 1; k < m; ++k) {
-        long slot = n + static_cast<long>(base) + (k - 1);
-        if (slot >= cap) return;   // capacity is sized from the counting pass; never taken
-        ent[slot] = make_double4(cx[k][0], cx[k][1], cx[k][2], 0.0);
-        eid[slot] = (static_cast<unsigned>(cs[k]) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
-      }
-    }
-
-    __global__ void k_cell_count(GridDesc g, double4 const* __restrict__ ent, long m, int* __restrict__ ecell,
-                                 int* __restrict__ ccount) {
-      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (e >= m) return;
-      double4 p = ent[e];
-      double x[3] = {p.x, p.y, p.z};
-      int c3[3];
-      int c = cell_idx(g, x, c3);
-      ecell[e] = c;
-      atomicAdd(&ccount[c], 1);
-    }
-
-    __global__ void k_scatter(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
-                              int const* __restrict__ ecell, long m, int const* __restrict__ cstart,
-                              int* __restrict__ ccursor, double4* __restrict__ spos, unsigned* __restrict__ sid) {
-      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (e >= m) return;
-      int c = ecell[e];
-      int slot = cstart[c] + atomicAdd(&ccursor[c], 1);
-      spos[slot] = ent[e];
-      sid[slot] = eid[e];
-    }
-
-    // Orders each cell by packed id so that neighbour rows are independent of atomic ordering.
-    __global__ void k_cell_sort(int ncells, int const* __restrict__ cstart, int const* __restrict__ ccount,
-                                double4* __restrict__ spos, unsigned* __restrict__ sid) {
-      int c = blockIdx.x * blockDim.x + threadIdx.x;
-      if (c >= ncells) return;
-      int b = cstart[c], e = b + ccount[c];
-      for (int i = b + 1; i < e; ++i) {
-        unsigned key = sid[i];
-        double4 v = spos[i];
-        int j = i - 1;
-        while (j >= b && sid[j] > key) {
-          sid[j + 1] = sid[j];
-          spos[j + 1] = spos[j];
-          --j;
-        }
-        sid[j + 1] = key;
-        spos[j + 1] = v;
-      }
-    }
-
-    // List::build_neigh_list (list.cpp:134-165) for every real atom: own cell and its neighbours.
-    __global__ void k_build(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid, long m,
-                            int const* __restrict__ cstart, int const* __restrict__ ccount, unsigned* __restrict__ nbr,
-                            int* __restrict__ ncnt, int K, unsigned* __restrict__ maxcnt) {
-      long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (e >= m) return;
-      unsigned id = sid[e];
-      if ((id >> NB_SHIFT_BITS) != 13u) return;   // ghosts have no rows
-      long i = id & NB_IDX_MASK;
-      double4 p = spos[e];
-      double xi[3] = {p.x, p.y, p.z};
-      int c3[3];
-      cell_idx(g, xi, c3);
-      double const rc2 = g.rc * g.rc;
-      int cnt = 0;
-      unsigned* row = nbr + i * static_cast<long>(K);
-      for (int z = max(c3[2] - 1, 0); z < min(c3[2] + 2, g.shape[2]); ++z)
-        for (int y = max(c3[1] - 1, 0); y < min(c3[1] + 2, g.shape[1]); ++y)
-          for (int x = max(c3[0] - 1, 0); x < min(c3[0] + 2, g.shape[0]); ++x) {
-            int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
-            int b = cstart[cc], end = b + ccount[cc];
-            for (int s = b; s < end; ++s) {
-              if (s == e) continue;
-              double4 q = spos[s];
-              double xn[3] = {q.x, q.y, q.z};
-              if (dist2(xi, xn) < rc2) {
-                if (cnt < K) row[cnt] = sid[s];
-                ++cnt;
-              }
-            }
-          }
-      ncnt[i] = cnt;
-      atomicMax(maxcnt, static_cast<unsigned>(cnt));
-    }
-
     __global__ void k_update(double4* __restrict__ pos, double const* __restrict__ d, long n) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (i >= n) return;
@@ -291,8 +155,8 @@ namespace flyg {
     }
 
     // ---- single-sync rebuild path ----------------------------------------------------------------
-    // lstate[0] ghosts generated, [1] longest row, [2] largest staged cell block, [3] overflow bits.
-    enum : unsigned { OV_GHOST = 1u, OV_STAGE = 2u };
+    // lstate[0] ghosts generated, [1] longest row, [3] overflow bits.
+    enum : unsigned { OV_GHOST = 1u };
 
     __global__ void k_clear(int* __restrict__ a, long na, int* __restrict__ b, long nb, unsigned* __restrict__ st) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
@@ -374,126 +238,50 @@ namespace flyg {
       atomicAdd(&cnt[c], 1);
     }
 
-    __global__ void k_scatter_cap(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
-                                  int const* __restrict__ ecell, long n, long gcap, unsigned const* __restrict__ st,
-                                  int const* __restrict__ cstart, int* __restrict__ cursor, double4* __restrict__ spos,
-                                  unsigned* __restrict__ sid) {
+    // Counting-sort scatter of the packed ids only (order inside a cell follows the atomics).
+    __global__ void k_scatter_ids(unsigned const* __restrict__ eid, int const* __restrict__ ecell, long n, long gcap,
+                                  unsigned const* __restrict__ st, int const* __restrict__ cstart, int* __restrict__ cursor,
+                                  unsigned* __restrict__ usid) {
       long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (e >= live_entries(st, n, gcap)) return;
       int c = ecell[e];
-      int slot = cstart[c] + atomicAdd(&cursor[c], 1);
-      spos[slot] = ent[e];
-      sid[slot] = eid[e];
+      usid[cstart[c] + atomicAdd(&cursor[c], 1)] = eid[e];
     }
 
-    // List::build_neigh_list for all real atoms of one cell: the 3x3x3 block of cells around it is
-    // staged in LDS in z-major cell order, each cell sorted by packed id (ranks computed from the ids,
-    // so the rows do not depend on the atomic scatter order), then one thread per real atom of the
-    // cell tests every staged candidate.  Rows come out in the same order as cell-by-cell scanning.
-    constexpr int STAGE_CAP = 1024;
-    constexpr int BUILD_T = 128;
-
-    __global__ __launch_bounds__(BUILD_T) void k_build_lds(GridDesc g, double4 const* __restrict__ spos,
-                                                           unsigned const* __restrict__ sid,
-                                                           int const* __restrict__ cstart, unsigned* __restrict__ nbr,
-                                                           int* __restrict__ ncnt, int K, unsigned* __restrict__ st) {
-      __shared__ double4 lpos[STAGE_CAP];
-      __shared__ unsigned lid[STAGE_CAP];
-      __shared__ int cb[27], ce[27], lb[28];
-      __shared__ int ncell, own_slot;
-      int const c = blockIdx.x;
-      int const own_b = cstart[c], own_e = cstart[c + 1];
-      if (own_b == own_e) return;   // uniform: empty cell
-      int const tid = threadIdx.x;
-      if (tid == 0) {
-        int cz = c / g.prod[2], rem = c - cz * g.prod[2];
-        int cy = rem / g.prod[1], cx = rem - cy * g.prod[1];
-        int k = 0, tot = 0;
-        for (int z = max(cz - 1, 0); z < min(cz + 2, g.shape[2]); ++z)
-          for (int y = max(cy - 1, 0); y < min(cy + 2, g.shape[1]); ++y)
-            for (int x = max(cx - 1, 0); x < min(cx + 2, g.shape[0]); ++x) {
-              int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
-              if (cc == c) own_slot = k;
-              cb[k] = cstart[cc];
-              ce[k] = cstart[cc + 1];
-              lb[k] = tot;
-              tot += ce[k] - cb[k];
-              ++k;
-            }
-        lb[k] = tot;
-        ncell = k;
-        if (tot > STAGE_CAP) {
-          atomicOr(&st[3], static_cast<unsigned>(OV_STAGE));
-          atomicMax(&st[2], static_cast<unsigned>(tot));
-        }
-      }
-      __syncthreads();
-      int const total = lb[ncell];
-      if (total > STAGE_CAP) return;   // uniform: the host falls back to the unstaged build
-      for (int k = 0; k < ncell; ++k) {
-        int const b = cb[k], cnt = ce[k] - b;
-        for (int t = tid; t < cnt; t += BUILD_T) {
-          unsigned id = sid[b + t];
-          int rank = 0;
-          for (int u = 0; u < cnt; ++u) rank += sid[b + u] < id ? 1 : 0;
-          lpos[lb[k] + rank] = spos[b + t];
-          lid[lb[k] + rank] = id;
-        }
-      }
-      __syncthreads();
-      double const rc2 = g.rc * g.rc;
-      int const ob = lb[own_slot], oe = lb[own_slot + 1];
-      for (int t = ob + tid; t < oe; t += BUILD_T) {
-        unsigned id = lid[t];
-        if ((id >> NB_SHIFT_BITS) != 13u) continue;   // ghosts have no rows
-        long i = id & NB_IDX_MASK;
-        double4 p = lpos[t];
-        double xi[3] = {p.x, p.y, p.z};
-        unsigned* row = nbr + i * static_cast<long>(K);
-        int cnt = 0;
-        for (int s = 0; s < total; ++s) {
-          if (s == t) continue;
-          double4 q = lpos[s];
-          double xn[3] = {q.x, q.y, q.z};
-          if (dist2(xi, xn) < rc2) {
-            if (cnt < K) row[cnt] = lid[s];
-            ++cnt;
-          }
-        }
-        ncnt[i] = cnt;
-        atomicMax(&st[1], static_cast<unsigned>(cnt));
-      }
-    }
-
-    // Unstaged fallback (a 3x3x3 block with more than STAGE_CAP entries): cells sorted in place.
-    __global__ void k_cell_sort2(int ncells, int const* __restrict__ cstart, double4* __restrict__ spos,
-                                 unsigned* __restrict__ sid) {
-      int c = blockIdx.x * blockDim.x + threadIdx.x;
-      if (c >= ncells) return;
-      int b = cstart[c], e = cstart[c + 1];
-      for (int i = b + 1; i < e; ++i) {
-        unsigned key = sid[i];
-        double4 v = spos[i];
-        int j = i - 1;
-        while (j >= b && sid[j] > key) {
-          sid[j + 1] = sid[j];
-          spos[j + 1] = spos[j];
-          --j;
-        }
-        sid[j + 1] = key;
-        spos[j + 1] = v;
-      }
-    }
-
-    __global__ void k_build2(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid, long n,
-                             long gcap, unsigned const* __restrict__ st, int const* __restrict__ cstart,
-                             unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ mx) {
+    // Places every entry at cell start + its rank among the cell's packed ids: cells come out sorted by
+    // id whatever order the atomics took, so rows (and every sum over them) are reproducible.
+    __global__ void k_rank_place(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
+                                 int const* __restrict__ ecell, long n, long gcap, unsigned const* __restrict__ st,
+                                 int const* __restrict__ cstart, unsigned const* __restrict__ usid,
+                                 double4* __restrict__ spos, unsigned* __restrict__ sid, int* __restrict__ rslot) {
       long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (e >= live_entries(st, n, gcap)) return;
-      unsigned id = sid[e];
-      if ((id >> NB_SHIFT_BITS) != 13u) return;
-      long i = id & NB_IDX_MASK;
-      double4 p = spos[e];
+      int c = ecell[e];
+      unsigned id = eid[e];
+      int b = cstart[c], end = cstart[c + 1];
+      int rank = 0;
+      for (int q = b; q < end; ++q) rank += usid[q] < id ? 1 : 0;
+      spos[b + rank] = ent[e];
+      sid[b + rank] = id;
+      if (e < n) rslot[e] = b + rank;   // the first n entries are the reals, entry e = atom e
+    }
+
+    // List::build_neigh_list (list.cpp:134-165): BL lanes per real atom walk the 3x3x3 block of cells
+    // around it (z outermost, x innermost; each cell in packed-id order) with the strict r < r_c test
+    // of the reference; the lanes split each cell and write their hits in candidate order (ballot +
+    // prefix count), so a row is the same sequence a single thread would produce.
+    constexpr int BL = 8;
+
+    __global__ void k_build_rows(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid,
+                                 int const* __restrict__ cstart, int const* __restrict__ rslot, long n,
+                                 unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ st) {
+      long gid = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      long i = gid / BL;
+      int const lane = static_cast<int>(gid % BL);
+      if (i >= n) return;   // whole groups leave together (blockDim is a multiple of BL)
+      int const gbase = static_cast<int>(threadIdx.x & 63) - lane;
+      int const self = rslot[i];
+      double4 p = spos[self];
       double xi[3] = {p.x, p.y, p.z};
       int c3[3];
       cell_idx(g, xi, c3);
@@ -504,21 +292,30 @@ namespace flyg {
         for (int y = max(c3[1] - 1, 0); y < min(c3[1] + 2, g.shape[1]); ++y)
           for (int x = max(c3[0] - 1, 0); x < min(c3[0] + 2, g.shape[0]); ++x) {
             int cc = x * g.prod[0] + y * g.prod[1] + z * g.prod[2];
-            for (int s = cstart[cc]; s < cstart[cc + 1]; ++s) {
-              if (s == e) continue;
-              double4 q = spos[s];
-              double xn[3] = {q.x, q.y, q.z};
-              if (dist2(xi, xn) < rc2) {
-                if (cnt < K) row[cnt] = sid[s];
-                ++cnt;
+            int b = cstart[cc], end = cstart[cc + 1];
+            for (int s0 = b; s0 < end; s0 += BL) {
+              int s = s0 + lane;
+              bool hit = false;
+              if (s < end && s != self) {
+                double4 q = spos[s];
+                double xn[3] = {q.x, q.y, q.z};
+                hit = dist2(xi, xn) < rc2;
               }
+              unsigned long long m = __ballot(hit);
+              unsigned gm = static_cast<unsigned>((m >> gbase) & ((1ull << BL) - 1ull));
+              if (hit) {
+                int k = cnt + __popc(gm & ((1u << lane) - 1u));
+                if (k < K) row[k] = sid[s];
+              }
+              cnt += __popc(gm);
             }
           }
-      ncnt[i] = cnt;
-      atomicMax(mx, static_cast<unsigned>(cnt));
+      if (lane == 0) {
+        ncnt[i] = cnt;
+        atomicMax(&st[1], static_cast<unsigned>(cnt));
+      }
     }
 
-    // ---- exclusive scan (int), two levels per recursion --------------------------------------
     constexpr int SCAN_B = 1024;
 
     __global__ void k_scan_block(int* __restrict__ data, int n, int* __restrict__ sums) {
@@ -626,6 +423,7 @@ namespace flyg {
     nl.cstart.reserve(nc + 1);
     nl.ccursor.reserve(nc);
     nl.ncnt.reserve(n);
+    nl.rslot.reserve(n);
     unsigned* dst = nl.lstate.get();
 
     for (int attempt = 0; attempt < 3; ++attempt) {
@@ -635,6 +433,7 @@ namespace flyg {
       nl.ecell.reserve(cap);
       nl.spos.reserve(cap);
       nl.sid.reserve(cap);
+      nl.usid.reserve(cap);
       nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
       long nz = std::max<long>(nc + 1, 4);
       k_clear<<<nblk(nz, TPB), TPB, 0, st>>>(nl.cstart.get(), nc + 1, nl.ccursor.get(), nc, dst);
@@ -643,10 +442,13 @@ namespace flyg {
       k_cell_count_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), n, nl.gcap, dst, nl.ecell.get(),
                                                        nl.cstart.get());
       exclusive_scan(ctx, nl.cstart.get(), nc + 1, nl.scan_tmp);
-      k_scatter_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst,
-                                                    nl.cstart.get(), nl.ccursor.get(), nl.spos.get(), nl.sid.get());
-      k_build_lds<<<nc, BUILD_T, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), nl.cstart.get(), nl.nbr.get(),
-                                          nl.ncnt.get(), nl.K, dst);
+      k_scatter_ids<<<nblk(cap, TPB), TPB, 0, st>>>(nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst, nl.cstart.get(),
+                                                    nl.ccursor.get(), nl.usid.get());
+      k_rank_place<<<nblk(cap, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst,
+                                                   nl.cstart.get(), nl.usid.get(), nl.spos.get(), nl.sid.get(),
+                                                   nl.rslot.get());
+      k_build_rows<<<nblk(n * BL, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), nl.cstart.get(), nl.rslot.get(),
+                                                 n, nl.nbr.get(), nl.ncnt.get(), nl.K, dst);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(nl.hstate, dst, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -656,21 +458,6 @@ namespace flyg {
         continue;
       }
       nl.nghost = G;
-      if (flags & OV_STAGE) {   // a 3x3x3 cell block too large for LDS: unstaged build
-        k_cell_sort2<<<nblk(nc, TPB), TPB, 0, st>>>(nc, nl.cstart.get(), nl.spos.get(), nl.sid.get());
-        for (int a2 = 0; a2 < 2; ++a2) {
-          nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
-          HIPCHK(hipMemsetAsync(dst + 1, 0, sizeof(unsigned), st));
-          k_build2<<<nblk(n + nl.gcap, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), n, nl.gcap, dst,
-                                                          nl.cstart.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, dst + 1);
-          HIPCHK(hipMemcpyAsync(nl.hstate, dst, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
-          HIPCHK(hipStreamSynchronize(st));
-          if (static_cast<int>(nl.hstate[1]) <= nl.K) break;
-          nl.K = (static_cast<int>(nl.hstate[1]) + 15) / 16 * 16 + 16;
-        }
-        nl.built = true;
-        return;
-      }
       if (static_cast<int>(maxrow) > nl.K) {   // a row overflowed: widen the rows and build again
         nl.K = (static_cast<int>(maxrow) + 15) / 16 * 16 + 16;
         continue;
