@@ -125,6 +125,11 @@ namespace flyg {
       static constexpr int NQ = VS / T;
       static constexpr int VW = VS / 64;
 
+      // The partials alternate between two LDS buffers (red[0..63], red[64..127]): a thread can only
+      // write a buffer again after every thread has passed the barrier of the reduction in between,
+      // i.e. after every read of the previous use, so one barrier per reduction suffices.
+      mutable int par = 0;
+
       template <int K>
       __device__ void tree(double (&v)[NQ][4]) const {
 #pragma unroll
@@ -133,20 +138,21 @@ namespace flyg {
           for (int k = 0; k < K; ++k)
 #pragma unroll
             for (int bit = 32; bit >= 1; bit >>= 1) v[q][k] = __dadd_rn(v[q][k], __shfl_xor(v[q][k], bit, 64));
+        double* buf = red + 64 * par;
+        par ^= 1;
         if ((tid & 63) == 0) {
 #pragma unroll
           for (int q = 0; q < NQ; ++q)
 #pragma unroll
-            for (int k = 0; k < K; ++k) red[4 * ((tid >> 6) + q * (T / 64)) + k] = v[q][k];
+            for (int k = 0; k < K; ++k) buf[4 * ((tid >> 6) + q * (T / 64)) + k] = v[q][k];
         }
         __syncthreads();
         double t[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) t[k] = red[k];
+        for (int k = 0; k < K; ++k) t[k] = buf[k];
         for (int w = 1; w < VW; ++w)
 #pragma unroll
-          for (int k = 0; k < K; ++k) t[k] = __dadd_rn(t[k], red[4 * w + k]);
-        __syncthreads();
+          for (int k = 0; k < K; ++k) t[k] = __dadd_rn(t[k], buf[4 * w + k]);
 #pragma unroll
         for (int k = 0; k < K; ++k) v[0][k] = t[k];
       }
@@ -338,6 +344,10 @@ namespace flyg {
     // Region rows staged in LDS per workgroup size (96 KB at 1024 threads: one workgroup per CU anyway).
     template <int T>
     __host__ __device__ constexpr int lds_rows() { return T >= 1024 ? 3072 : (T >= 512 ? 1536 : 1024); }
+    // LDS doubles for the rotor's workspace vectors (the rest of the 160 KB after the reduction buffer
+    // and the region rows).
+    template <int T>
+    __host__ __device__ constexpr int lds_ws() { return T >= 1024 ? 3000 : (T >= 512 ? 11000 : 14000); }
 
     // Rebuilds the active lists from the candidate rows at the current region positions (stable:
     // an active row is the ordered subsequence of its candidate row, whatever the workgroup shape).
@@ -632,7 +642,7 @@ namespace flyg {
 
     template <int T, int L, bool MIX, bool MULTI>
     __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
-                               double4* lds_cur) {
+                               double4* lds_cur, double* lds_ws_buf) {
       long const t_begin = static_cast<long>(wall_clock64());
       SearchHdr const h = A.hdr[sid];
       DimOpt const& o = A.o;
@@ -656,8 +666,12 @@ namespace flyg {
       v.act_f = fmin(ACT_SKIN_F, h.skin_f);
       v.act_m = fmin(ACT_SKIN_M, h.skin_m);
       int const n3 = v.n3;
+      // Workspace: the vectors the rotor touches every iteration (10 named ones and the rotation core)
+      // come first and live in LDS when they fit; the translation core and xr stay in global memory.
       double* W = A.ws + h.ws_off;
-      auto vecp = [&](int k) { return W + static_cast<long>(k) * n3; };
+      int const hot = 13 + 2 * o.rn;
+      double* Wh = hot * n3 <= lds_ws<T>() ? lds_ws_buf : W;
+      auto vecp = [&](int k) { return (k < hot ? Wh : W) + static_cast<long>(k) * n3; };
       double* x = vecp(0);
       double* ax = vecp(1);
       double* eff = vecp(2);
@@ -669,7 +683,7 @@ namespace flyg {
       double* axp = vecp(8);
       double* xb = vecp(9);
       double* xr = vecp(nvec(o.n, o.rn) - 1);   // positions at the last active-list refresh
-      int const t0 = 10, r0v = 13 + 2 * o.n;
+      int const r0v = 10, t0 = 13 + 2 * o.rn;
       Core<T> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
                  lds_core, lds_core + 32, o.n, 0, n3};
       Core<T> rc{vecp(r0v), vecp(r0v + o.rn), vecp(r0v + 2 * o.rn), vecp(r0v + 2 * o.rn + 1), vecp(r0v + 2 * o.rn + 2),
@@ -736,15 +750,16 @@ namespace flyg {
           m1 = hi;
           m2 = lo;
         }
+        double* buf = tm.red + 128;   // clear of the two reduction buffers
         if ((tm.tid & 63) == 0) {
-          tm.red[4 * (tm.tid >> 6)] = m1;
-          tm.red[4 * (tm.tid >> 6) + 1] = m2;
+          buf[2 * (tm.tid >> 6)] = m1;
+          buf[2 * (tm.tid >> 6) + 1] = m2;
         }
         __syncthreads();
         a1 = 0;
         a2 = 0;
         for (int w = 0; w < T / 64; ++w) {
-          double b1 = tm.red[4 * w], b2 = tm.red[4 * w + 1];
+          double b1 = buf[2 * w], b2 = buf[2 * w + 1];
           double hi = fmax(a1, b1), lo = fmax(fmin(a1, b1), fmax(a2, b2));
           a1 = hi;
           a2 = lo;
@@ -920,6 +935,7 @@ namespace flyg {
       __shared__ int s_id;
       __shared__ double3 lds_S[27];
       __shared__ double4 lds_cur[lds_rows<T>()];
+      __shared__ double lds_wsb[lds_ws<T>()];
       Team<T> tm{red, static_cast<int>(threadIdx.x)};
       while (true) {
         if (threadIdx.x == 0) s_id = static_cast<int>(atomicAdd(A.queue, 1u));
@@ -927,7 +943,7 @@ namespace flyg {
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur);
+        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb);
       }
     }
 
