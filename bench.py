@@ -197,6 +197,66 @@ def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
     return out
 
 
+def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
+    """Config 5: bcc Fe 80^3 cells (1,028,096 atoms) with 0.1 % vacancies in clusters of 1-6 and 0.5 %
+    interstitials, relaxed (min0); 1024 concurrent searches (radius U(4, 9) A, kick 0.6 A, |g| < 1e-3 eV/A)
+    in one launch of the autotuned persistent kernel; converged barriers re-evaluated as full-system FP64
+    energy differences E(sp) - E(min0) on a sample."""
+    from openfly_amd import serve
+    t0 = time.perf_counter()
+    s = systems.config5(5)
+    pot = fa.EAM(ctx, WEN)
+    s.pos = relax(ctx, fa, pot, s)
+    t_setup = time.perf_counter() - t0
+    rng = np.random.default_rng(55)
+    centres = serve.sample_centres(s, n_search, rng)
+    radii = rng.uniform(4.0, 9.0, n_search)
+    kicks = [serve.sparse_kick(s, int(c), float(R), rng) for c, R in zip(centres, radii)]
+    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3), max_batch=n_search, streams=1)
+    box = fa.Box(s.basis)
+    t0 = time.perf_counter()
+    sysid = srv.add_system(box, s.pos, s.types)
+    t_admit = time.perf_counter() - t0
+
+    def drain(ids):
+        want, got = set(int(i) for i in ids), {}
+        while len(got) < len(want):
+            for d in srv.progress(wait_ms=50.0):
+                got[int(d)] = srv.result(d, positions=True, release=True)
+        return [got[int(i)] for i in ids]
+
+    log("config 5 leg: autotune on 64 searches")
+    drain(srv.submit(sysid, centres[:64], radii[:64], kicks[:64]))
+    log(f"config 5 leg: {n_search} searches")
+    t0 = time.perf_counter()
+    res = drain(srv.submit(sysid, centres, radii, kicks))
+    el = time.perf_counter() - t0
+    st = srv.status()
+    conv = [r for r in res if r["exit"] == 0]
+    # FP64 re-evaluation of a sample of converged barriers (full system, fresh list each time).
+    log(f"config 5 leg: FP64 re-evaluation of {min(n_check, len(conv))} barriers")
+    nl = fa.List(ctx, box, pot.r_cut())
+    frozen = np.zeros(s.n, np.uint8)
+    nl.rebuild(s.pos)
+    e0 = pot.energy(s.types, frozen, nl)
+    errs = []
+    for r in conv[:n_check]:
+        p = s.pos.copy()
+        p[r["atoms"]] = r["pos"]
+        nl.rebuild(p)
+        errs.append(abs((pot.energy(s.types, frozen, nl) - e0) - r["barrier"]))
+    ms = np.array([r["device_ms"] for r in res])
+    return {"workload": f"bcc Fe 80^3 cells ({s.n} atoms), 0.1% vacancies in clusters of 1-6, 0.5% interstitials, "
+                        f"relaxed; {n_search} concurrent searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, "
+                        "FP32 pair / FP64 accumulation, wen.eam.fs",
+            "atoms": int(s.n), "setup_seconds": t_setup, "admit_seconds": t_admit, "seconds": el,
+            "searches_per_s": n_search / el, "converged": len(conv), "converged_per_s": len(conv) / el,
+            "exits": np.bincount([r["exit"] for r in res], minlength=7).tolist(),
+            "block_threads": st["block_threads"], "lanes_per_atom": st["lanes_per_atom"],
+            "search_device_ms_p50": float(np.percentile(ms, 50)), "search_device_ms_p99": float(np.percentile(ms, 99)),
+            "fp64_reevaluated": len(errs), "max_abs_barrier_error_eV": float(max(errs)) if errs else None}
+
+
 def cpu_search_baseline(data_args, cpu_sample=16):
     """FP64 oracle, one search per thread over the host cores (find.cpp:697-732 with threads = 1), on the
     first cpu_sample starts of the batched leg; also the agreement of the device's mixed-precision
@@ -251,6 +311,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-search", action="store_true", help="skip the batched saddle-search leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served saddle-search leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 1M-atom, 1024-search leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box of this build has a "
@@ -380,6 +441,10 @@ def main():
     if not args.no_serve:
         out["serving"] = serve_bench(ctx, fa, systems)
         log("serving leg done")
+    if not args.no_large:
+        log("config 5 leg: building and relaxing the 1M-atom system")
+        out["large"] = large_bench(ctx, fa, systems)
+        log("config 5 leg done")
     if not args.no_cpu:
         log("gradient leg: CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(s, path)

@@ -35,22 +35,45 @@ class Request:
     results: dict = dataclasses.field(default_factory=dict)
 
 
+class Near:
+    """Periodic fixed-radius neighbour queries on a system's positions (k-d tree over the box)."""
+
+    def __init__(self, s: systems.System):
+        from scipy.spatial import cKDTree
+        self.L = np.diag(s.basis).copy()
+        w = s.pos - self.L * np.floor(s.pos / self.L)
+        w = np.where(w >= self.L, w - self.L, w)
+        self.tree = cKDTree(w, boxsize=self.L)
+        self.pos = s.pos
+
+    def within(self, p, r):
+        """Sorted indices of the atoms within r of point p (minimum image)."""
+        q = np.asarray(p, float) - self.L * np.floor(np.asarray(p, float) / self.L)
+        q = np.where(q >= self.L, q - self.L, q)
+        return np.array(sorted(self.tree.query_ball_point(q, r)), dtype=np.int64)
+
+
+def _near_of(s):
+    nr = getattr(s, "_near", None)
+    if nr is None:
+        nr = Near(s)
+        s._near = nr
+    return nr
+
+
 def sample_centres(s: systems.System, k, rng, r=3.0):
     """k random defect-adjacent atoms: pick a random defect (vacancy site or interstitial), then a random
     atom within r of it."""
-    L = np.diag(s.basis)
     pts = []
     if getattr(s, "vacancies", None) is not None and len(s.vacancies):
         pts.append(np.asarray(s.vacancies))
     if getattr(s, "interstitial_ids", None) is not None and len(s.interstitial_ids):
         pts.append(s.pos[s.interstitial_ids])
     pts = np.concatenate(pts) if pts else s.pos
+    nr = _near_of(s)
     out = np.empty(k, np.int64)
     for q in range(k):
-        p = pts[rng.integers(len(pts))]
-        d = s.pos - p
-        d -= L * np.round(d / L)
-        near = np.nonzero(np.einsum("ij,ij->i", d, d) < r * r)[0]
+        near = nr.within(pts[rng.integers(len(pts))], r)
         out[q] = rng.choice(near) if len(near) else rng.integers(s.n)
     return out
 
@@ -61,13 +84,16 @@ def sparse_kick(s: systems.System, centre, radius, rng, stddev=0.6, r_pert=4.0):
     negative) scaled by (1 - r / r_pert) (the centre: 1), and a standard-normal axis component; the axis
     is normalised.  Returns (atoms, positions, axis)."""
     L = np.diag(s.basis)
-    d = s.pos - s.pos[centre]
+    cand = _near_of(s).within(s.pos[centre], max(r_pert, 0.0) + 1e-9)
+    d = s.pos[cand] - s.pos[centre]
     d -= L * np.round(d / L)
-    r = np.sqrt(np.einsum("ij,ij->i", d, d))
+    rr = np.sqrt(np.einsum("ij,ij->i", d, d))
     sd = -1.0
     while sd < 0:
         sd = rng.normal(stddev, stddev / 3.0)
-    near = np.nonzero((r < r_pert) & (r <= radius))[0]
+    keep = (rr < r_pert) & (rr <= radius)
+    near = cand[keep]
+    r = dict(zip(cand.tolist(), rr.tolist()))
     pos = s.pos[near].copy()
     ax = np.zeros((len(near), 3))
     for q, j in enumerate(near):

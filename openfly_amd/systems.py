@@ -190,14 +190,16 @@ def defect_system(cells, n_vac, n_int, seed, clusters=False):
     sites = bcc_sites(A_FE, cells)
     keep = np.ones(len(sites), bool)
     if clusters:
+        from scipy.spatial import cKDTree
         L = cells * A_FE
+        tree = cKDTree(sites - L * np.floor(sites / L), boxsize=L)
         removed = 0
         while removed < n_vac:
             size = int(min(rng.integers(1, 7), n_vac - removed))
             c = sites[rng.integers(len(sites))]
-            d = sites - c
-            d -= L * np.round(d / L)
-            order = np.argsort(np.einsum("ij,ij->i", d, d))
+            # the nearest still-present sites (ties broken by index, as a stable sort of distances would)
+            dist, order = tree.query(c - L * np.floor(c / L), k=64)
+            order = order[np.lexsort((order, np.round(dist, 9)))]
             picked = [k for k in order if keep[k]][:size]
             keep[picked] = False
             removed += len(picked)
