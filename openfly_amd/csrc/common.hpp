@@ -188,6 +188,20 @@ namespace flyg {
   }
 
   // ----------------------------------------------------------------------------------------------
+  // XCD-aware block order.  The dispatcher hands workgroup b to XCD b mod 8, and each XCD has its own
+  // L2; renumbering so that XCD x processes one contiguous range of blocks keeps the atoms (and so
+  // the neighbour gathers) of one XCD spatially together instead of spreading every L2 over the
+  // whole system.  A bijection on [0, nb).
+  // ----------------------------------------------------------------------------------------------
+  constexpr unsigned NUM_XCD = 8;
+
+  __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+    unsigned const per = nb / NUM_XCD, rem = nb % NUM_XCD;
+    unsigned const x = b % NUM_XCD, k = b / NUM_XCD;
+    return x * per + (x < rem ? x : rem) + k;
+  }
+
+  // ----------------------------------------------------------------------------------------------
   // Wavefront helpers (64 lanes on CDNA).
   // ----------------------------------------------------------------------------------------------
 

@@ -115,7 +115,7 @@ namespace flyg {
           return T.V[k];
         }
       };
-      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool act = i < A.n;
@@ -180,7 +180,7 @@ namespace flyg {
           return T.V[k];
         }
       };
-      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool inr = i < A.n;
@@ -234,7 +234,7 @@ namespace flyg {
     template <int L, bool MULTI>
     __global__ __launch_bounds__(NT) void k_hvp_density(EAMTables T, Shifts S, PairArgs A, double const* __restrict__ v,
                                                         double2* __restrict__ dfp, unsigned* __restrict__ err) {
-      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool act = i < A.n;
@@ -282,7 +282,7 @@ namespace flyg {
     __global__ __launch_bounds__(NT) void k_hvp_force(EAMTables T, Shifts S, PairArgs A, double const* __restrict__ v,
                                                       double2 const* __restrict__ dfp, double* __restrict__ hv,
                                                       unsigned* __restrict__ err) {
-      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool inr = i < A.n;
@@ -345,7 +345,7 @@ namespace flyg {
     template <int L, bool MULTI>
     __global__ __launch_bounds__(NT) void k_rho(EAMTables T, Shifts S, PairArgs A, double* __restrict__ rho_out,
                                                 unsigned* __restrict__ err) {
-      long gid = blockIdx.x * static_cast<long>(NT) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
       long i = gid / L;
       int lane = static_cast<int>(gid % L);
       bool act = i < A.n;

@@ -275,7 +275,7 @@ namespace flyg {
     __global__ void k_build_rows(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid,
                                  int const* __restrict__ cstart, int const* __restrict__ rslot, long n,
                                  unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ st) {
-      long gid = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(blockDim.x) + threadIdx.x;
       long i = gid / BL;
       int const lane = static_cast<int>(gid % BL);
       if (i >= n) return;   // whole groups leave together (blockDim is a multiple of BL)
