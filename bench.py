@@ -29,6 +29,13 @@ R_CUT = 5.3
 SKIN = (1.1 ** (1.0 / 3.0) - 1.0) * R_CUT   # LBFGS::minimise skin policy (lbfgs.cpp:39), CPU baseline list
 LIST_SKIN = 0.6   # Verlet skin of the timed update/rebuild loop (rebuild once the accumulated move > skin / 2)
 
+# HBM traffic of one gradient (density + force pass) at the default workload, as measured with rocprofv3
+# (profiles/eam_gradient_config2.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
+# reading for a copy kernel of known size (python bench.py --copy-calibration: a 64 MiB copy reads
+# FETCH_SIZE 32780.9 KiB and WRITE_SIZE 65536 KiB).
+GRADIENT_TRAFFIC = {"fetch_kib": 12732.5 + 13335.5, "write_kib": 1704.0 + 1278.1,
+                    "fetch_scale": 65536.0 / 32780.9, "write_scale": 1.0}
+
 
 _T0 = time.perf_counter()
 
@@ -187,6 +194,14 @@ def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
     reqs = serve.make_requests(pool, n_req, rate=rate, seed=42)
     log("serving leg: replay")
     out = serve.serve(srv, reqs, report=log)
+    # Latency below saturation: the same pool at half the request rate just measured as sustained.
+    rate2 = 0.5 * out["requests"] / out["wall_s"]
+    log(f"serving leg: replay at {rate2:.1f} requests/s (half the sustained rate)")
+    reqs2 = serve.make_requests(pool, 40, rate=rate2, seed=43)
+    light = serve.serve(srv, reqs2, report=log)
+    out["below_saturation"] = {k: light[k] for k in ("requests", "searches", "converged", "wall_s", "searches_per_s",
+                                                     "latency_p50_s", "latency_p99_s", "latency_max_s",
+                                                     "offered_requests_per_s", "mean_searches_per_request")}
     st = srv.status()
     out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, from {pool_size} relaxed kMC systems (bcc Fe, N "
                         "log-uniform [2000, 250000], 0.5-2% vacancies + interstitials), 8-64 searches per request, "
@@ -239,12 +254,14 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
     frozen = np.zeros(s.n, np.uint8)
     nl.rebuild(s.pos)
     e0 = pot.energy(s.types, frozen, nl)
-    errs = []
+    errs, pairs = [], []
     for r in conv[:n_check]:
         p = s.pos.copy()
         p[r["atoms"]] = r["pos"]
         nl.rebuild(p)
-        errs.append(abs((pot.energy(s.types, frozen, nl) - e0) - r["barrier"]))
+        de = pot.energy(s.types, frozen, nl) - e0
+        errs.append(abs(de - r["barrier"]))
+        pairs.append((r["barrier"], de, float(np.abs(r["pos"] - s.pos[r["atoms"]]).max())))
     ms = np.array([r["device_ms"] for r in res])
     return {"workload": f"bcc Fe 80^3 cells ({s.n} atoms), 0.1% vacancies in clusters of 1-6, 0.5% interstitials, "
                         f"relaxed; {n_search} concurrent searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, "
@@ -254,7 +271,8 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
             "exits": np.bincount([r["exit"] for r in res], minlength=7).tolist(),
             "block_threads": st["block_threads"], "lanes_per_atom": st["lanes_per_atom"],
             "search_device_ms_p50": float(np.percentile(ms, 50)), "search_device_ms_p99": float(np.percentile(ms, 99)),
-            "fp64_reevaluated": len(errs), "max_abs_barrier_error_eV": float(max(errs)) if errs else None}
+            "fp64_reevaluated": len(errs), "max_abs_barrier_error_eV": float(max(errs)) if errs else None,
+            "sample_barrier_fp64_maxdisp": [[round(a, 6), round(b, 6), round(c, 3)] for a, b, c in pairs[:5]]}
 
 
 def cpu_search_baseline(data_args, cpu_sample=16):
@@ -312,7 +330,19 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip the batched saddle-search leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served saddle-search leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 1M-atom, 1024-search leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 64 MiB device copies (the known-size kernel that scales FETCH_SIZE / "
+                         "WRITE_SIZE readings to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box of this build has a "
               "single MI355X and the path is not sharded across GPUs.", file=sys.stderr)
@@ -425,7 +455,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (1024.0 * (GRADIENT_TRAFFIC["fetch_kib"] * GRADIENT_TRAFFIC["fetch_scale"]
+                                  + GRADIENT_TRAFFIC["write_kib"] * GRADIENT_TRAFFIC["write_scale"])
+                        if (args.cells == 30 and args.skin == LIST_SKIN and args.precision == "mixed") else None),
             "kernel": "EAM gradient (density pass + force pass)",
             "kernel_ms": kern_ms,
             "algorithmic_bytes_per_launch": bytes_per_grad,

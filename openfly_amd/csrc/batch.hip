@@ -96,6 +96,7 @@ namespace flyg {
       double* out_ax;
       SearchOut* out;
       unsigned* queue;
+      int* done;                 // per-search completion flags (host-visible memory)
       DimOpt o;
       int t0;   // the one species present when the kernel is instantiated with MULTI = false
     };
@@ -925,6 +926,13 @@ namespace flyg {
         r.grad_ticks = gticks;
         A.out[sid] = r;
       }
+      // Results (host-visible memory) first, then the flag: the host may read a search as soon as its
+      // flag is set, while the kernel still runs other searches.
+      __threadfence_system();
+      __syncthreads();
+      if (tm.tid == 0) {
+        __hip_atomic_store(&A.done[sid], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
       __syncthreads();
     }
 
@@ -1359,8 +1367,9 @@ namespace flyg {
       }
     };
 
-    // One packed launch of the persistent search kernel on its own stream: uploads in one copy, the
-    // kernel, and the results back into pinned memory in one copy, all asynchronous.
+    // One packed launch of the persistent search kernel on its own stream: uploads in one copy, then the
+    // kernel, which writes each search's results and a done flag straight into pinned host memory, so
+    // searches can be harvested one by one while the kernel still runs others.
     struct Flight {
       hipStream_t st = nullptr;
       bool own_stream = false;
@@ -1370,7 +1379,10 @@ namespace flyg {
       long n3 = 0;
       DBuf<char> d_in, d_scr;
       Pinned h_in, h_out;
-      std::size_t o_out = 0, o_ox = 0, o_oax = 0, out_bytes = 0;
+      std::size_t o_out = 0, o_ox = 0, o_oax = 0, out_bytes = 0;   // offsets in h_out (flags at 0)
+      std::vector<long> v3off;      // per search: first element of its 3*nm output vectors
+      std::vector<char> harvested;  // per search
+      long left = 0;                // searches not yet harvested
       Flight() = default;
       Flight(Flight const&) = delete;
       Flight& operator=(Flight const&) = delete;
@@ -1459,10 +1471,12 @@ namespace flyg {
                   o_alen = sc.add<int>(nreg), o_ws = sc.add<double>(nws),
                   o_q = sc.add<unsigned>(1);
       std::size_t zero_end = sc.size;
-      F.o_out = sc.add<SearchOut>(ns);
-      F.o_ox = sc.add<double>(n3);
-      F.o_oax = sc.add<double>(n3);
-      F.out_bytes = sc.size - F.o_out;
+      Layout ho;
+      std::size_t o_flags = ho.add<int>(ns);
+      F.o_out = ho.add<SearchOut>(ns);
+      F.o_ox = ho.add<double>(n3);
+      F.o_oax = ho.add<double>(n3);
+      F.out_bytes = ho.size;
       // Grow with slack: freeing device or pinned memory synchronises the whole device, which would
       // stall the other streams' launches.
       auto grow = [](std::size_t need, std::size_t cap) { return need > cap ? need + need / 2 : need; };
@@ -1516,9 +1530,16 @@ namespace flyg {
       A.ws = reinterpret_cast<double*>(ds + o_ws);
       A.start_x = reinterpret_cast<double const*>(di + o_sx);
       A.start_ax = reinterpret_cast<double const*>(di + o_sax);
-      A.out_x = reinterpret_cast<double*>(ds + F.o_ox);
-      A.out_ax = reinterpret_cast<double*>(ds + F.o_oax);
-      A.out = reinterpret_cast<SearchOut*>(ds + F.o_out);
+      char* hb_out = F.h_out.p;
+      std::memset(hb_out + o_flags, 0, ns * sizeof(int));
+      A.done = reinterpret_cast<int*>(hb_out + o_flags);
+      A.out_x = reinterpret_cast<double*>(hb_out + F.o_ox);
+      A.out_ax = reinterpret_cast<double*>(hb_out + F.o_oax);
+      A.out = reinterpret_cast<SearchOut*>(hb_out + F.o_out);
+      F.v3off.resize(ns);
+      for (long k = 0; k < ns; ++k) F.v3off[k] = hdr[k].v3_off;
+      F.harvested.assign(ns, 0);
+      F.left = ns;
       A.queue = reinterpret_cast<unsigned*>(ds + o_q);
       A.o = dopt;
 
@@ -1533,7 +1554,6 @@ namespace flyg {
       hipLaunchKernelGGL(fn, dim3(grid), dim3(T), 0, F.st, A);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(F.e_b, F.st));
-      HIPCHK(hipMemcpyAsync(F.h_out.p, ds + F.o_out, F.out_bytes, hipMemcpyDeviceToHost, F.st));
       HIPCHK(hipEventRecord(F.e_done, F.st));
       F.busy = true;
     }
@@ -1549,27 +1569,34 @@ namespace flyg {
       return static_cast<double>(khz);
     }
 
-    // After F finished: finished searches go to `done`, searches that left their region (and may
-    // still grow it) come back in `requeue` with a larger skin.
+    // Harvests the searches of F whose done flags are set (all of them once the kernel has finished):
+    // finished searches go to `done`, searches that left their region (and may still grow it) come back
+    // in `requeue` with larger skins.  The flight becomes free when the kernel has finished and every
+    // search is harvested.
     template <typename Done>
     void flight_harvest(Flight& F, double rc, Done&& done, std::vector<Pending>& requeue) {
-      SearchOut const* outs = reinterpret_cast<SearchOut const*>(F.h_out.p);
-      double const* ox = reinterpret_cast<double const*>(F.h_out.p + (F.o_ox - F.o_out));
-      double const* oax = reinterpret_cast<double const*>(F.h_out.p + (F.o_oax - F.o_out));
-      long off = 0;
+      bool const finished = F.finished();
+      int const* flags = reinterpret_cast<int const*>(F.h_out.p);
+      SearchOut const* outs = reinterpret_cast<SearchOut const*>(F.h_out.p + F.o_out);
+      double const* ox = reinterpret_cast<double const*>(F.h_out.p + F.o_ox);
+      double const* oax = reinterpret_cast<double const*>(F.h_out.p + F.o_oax);
       for (long k = 0; k < static_cast<long>(F.work.size()); ++k) {
+        if (F.harvested[k]) continue;
+        if (!finished && __atomic_load_n(&flags[k], __ATOMIC_ACQUIRE) == 0) continue;
+        F.harvested[k] = 1;
+        --F.left;
         Pending& P = F.work[k];
         SearchOut const& o = outs[k];
         int nm = P.region->nm;
+        long off = F.v3off[k];
         ++P.launches;
         double cap = P.sys->skin_cap(rc);
         if (o.exit_code == EXIT_REGION && (P.skin_f < cap || P.skin_m < cap)) {
-          Pending q = std::move(P);
+          Pending q = P;
           q.skin_f = std::min(cap, std::max(q.skin_f * 1.5, o.max_disp + 1.0));
           q.skin_m = std::min(cap, std::max(std::max(q.skin_m * 1.5, q.skin_f), o.max_pair + 1.5));
           q.region.reset();
           requeue.push_back(std::move(q));
-          off += 3L * nm;
           continue;
         }
         Result R;
@@ -1589,11 +1616,12 @@ namespace flyg {
         R.x.assign(ox + off, ox + off + 3L * nm);
         R.ax.assign(oax + off, oax + off + 3L * nm);
         R.nr = P.region->nr;
-        off += 3L * nm;
         done(P, std::move(R));
       }
-      F.work.clear();
-      F.busy = false;
+      if (finished && F.left == 0) {
+        F.work.clear();
+        F.busy = false;
+      }
     }
 
     // Times a 20-step prefix of up to 64 searches with each candidate shape on F (synchronously).
@@ -1692,8 +1720,7 @@ namespace flyg {
       double const rc = S.eam->tab.rc;
       for (auto& fp : S.flights) {
         Flight& F = *fp;
-        if (!F.busy || !F.finished()) continue;
-        S.kernel_ms += F.kernel_ms();
+        if (!F.busy) continue;
         std::vector<Pending> requeue;
         flight_harvest(F, rc, [&](Pending& P, Result&& R) {
           --P.sys->active;
@@ -1702,6 +1729,7 @@ namespace flyg {
           ++S.searches_done;
           ++fin;
         }, requeue);
+        if (!F.busy) S.kernel_ms += F.kernel_ms();
         for (auto it = requeue.rbegin(); it != requeue.rend(); ++it) S.queue.push_front(std::move(*it));
       }
       for (auto it = S.systems.begin(); it != S.systems.end();) {

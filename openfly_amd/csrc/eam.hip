@@ -125,12 +125,12 @@ namespace flyg {
       int ti = 0;
       if (act) {
         pi = A.pos[i];
-        ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, A.pos, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
+            int tj = MULTI ? static_cast<int>(A.types[p.j]) : A.t0;
             if (MIX && !ENERGY) {
               float r2f = static_cast<float>(p.r2);
               float rf = r2f * __frsqrt_rn(r2f);
@@ -189,12 +189,12 @@ namespace flyg {
       unsigned e = 0;
       if (act) {
         double4 pi = posf[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, posf, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[p.j]) : 0;
+            int tj = MULTI ? static_cast<int>(A.types[p.j]) : A.t0;
             double cf;
             if (MIX) {
               float r2f = static_cast<float>(p.r2);
@@ -243,7 +243,7 @@ namespace flyg {
       int ti = 0;
       if (act) {
         double4 pi = A.pos[i];
-        ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
         bool fi = A.frozen[i];
         double vix = fi ? 0 : v[3 * i], viy = fi ? 0 : v[3 * i + 1], viz = fi ? 0 : v[3 * i + 2];
         int const c = min(A.cnt[i], A.K);
@@ -257,7 +257,7 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
             double r = sqrt(r2);
-            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
             double f, fp;
             spl64_ffp(T, T.phi[tj + T.ntypes * ti], r, f, fp, e);
             rho += f;
@@ -292,7 +292,7 @@ namespace flyg {
       if (act) {
         double4 pi = A.pos[i];
         double2 di = dfp[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
         double vix = v[3 * i], viy = v[3 * i + 1], viz = v[3 * i + 2];
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
@@ -305,7 +305,7 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
             double r = sqrt(r2);
-            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
             double2 dj = dfp[j];
             bool fj = A.frozen[j];
             double vjx = fj ? 0 : v[3 * j], vjy = fj ? 0 : v[3 * j + 1], vjz = fj ? 0 : v[3 * j + 2];
@@ -353,7 +353,7 @@ namespace flyg {
       unsigned e = 0;
       if (act) {
         double4 pi = A.pos[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : 0;
+        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
@@ -364,7 +364,7 @@ namespace flyg {
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[j]) : 0;
+            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
             rho += spl64<0>(T, T.phi[tj + T.ntypes * ti], sqrt(r2), e);
           }
         }
@@ -391,8 +391,11 @@ namespace flyg {
       return s;
     }
 
-    PairArgs pair_args(flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
-      return PairArgs{nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, nl.n, types, frozen};
+    // t0 >= 0 selects the single-species kernels (constant table indices): always with one species
+    // tabulated, and with several when the staged host types hold one species only.
+    PairArgs pair_args(flyg_eam_s const& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
+      int t0 = E.tab.ntypes == 1 ? 0 : (types == E.types.get() ? E.staged_single : -1);
+      return PairArgs{nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, nl.n, types, frozen, t0};
     }
 
     // Lanes per atom for the full-system kernels (a row of ~60 entries split 8 ways keeps ~8 pairs
@@ -448,7 +451,7 @@ namespace flyg {
     void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy, int precision = FLYG_FP64) {
       hipStream_t st = E.ctx->stream;
       unsigned* err = E.ctx->dflags.get();
-      bool multi = E.tab.ntypes > 1;
+      bool multi = A.t0 < 0;
       bool mix = precision == FLYG_MIXED;
       unsigned g = grid_for(A.n, L);
       double4* pf = E.posf.get();
@@ -473,7 +476,7 @@ namespace flyg {
     void launch_force(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* grad, int precision) {
       hipStream_t st = E.ctx->stream;
       unsigned* err = E.ctx->dflags.get();
-      bool multi = E.tab.ntypes > 1;
+      bool multi = A.t0 < 0;
       bool mix = precision == FLYG_MIXED;
       unsigned g = grid_for(A.n, L);
       if (multi) {
@@ -495,7 +498,7 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.posf.reserve(nl.n);
     E.eatom.reserve(nl.n);
-    launch_density<LANES>(E, shifts_of(nl.box), pair_args(nl, types, frozen), true);
+    launch_density<LANES>(E, shifts_of(nl.box), pair_args(E, nl, types, frozen), true);
     HIPCHK(hipGetLastError());
     dev_sum(*E.ctx, E.eatom.get(), nl.n, E.ctx->dscal.get());
     HIPCHK(hipMemcpyAsync(E.ctx->hscal, E.ctx->dscal.get(), sizeof(double), hipMemcpyDeviceToHost, E.ctx->stream));
@@ -559,7 +562,7 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.posf.reserve(nl.n);
     Shifts S = shifts_of(nl.box);
-    PairArgs A = pair_args(nl, types, frozen);
+    PairArgs A = pair_args(E, nl, types, frozen);
     int L = tuned_lanes(E, S, A, d_grad, precision);
     gradient_with(E, S, A, d_grad, precision, L);
     HIPCHK(hipGetLastError());
@@ -571,11 +574,11 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.dfp.reserve(nl.n);
     Shifts S = shifts_of(nl.box);
-    PairArgs A = pair_args(nl, types, frozen);
+    PairArgs A = pair_args(E, nl, types, frozen);
     hipStream_t st = E.ctx->stream;
     unsigned* err = E.ctx->dflags.get();
     unsigned g = grid_for(nl.n, LANES);
-    if (E.tab.ntypes > 1) {
+    if (A.t0 < 0) {
       k_hvp_density<LANES, true><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), err);
       k_hvp_force<LANES, true><<<g, NT, 0, st>>>(E.tab, S, A, d_v, E.dfp.get(), d_hv, err);
     } else {
@@ -588,9 +591,9 @@ namespace flyg {
   void eam_density(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* d_types, double* d_rho) {
     require(nl.built, "List used before rebuild");
     Shifts S = shifts_of(nl.box);
-    PairArgs A = pair_args(nl, d_types, nullptr);
+    PairArgs A = pair_args(E, nl, d_types, nullptr);
     unsigned g = grid_for(nl.n, LANES);
-    if (E.tab.ntypes > 1) {
+    if (A.t0 < 0) {
       k_rho<LANES, true><<<g, NT, 0, E.ctx->stream>>>(E.tab, S, A, d_rho, E.ctx->dflags.get());
     } else {
       k_rho<LANES, false><<<g, NT, 0, E.ctx->stream>>>(E.tab, S, A, d_rho, E.ctx->dflags.get());
@@ -629,7 +632,11 @@ namespace flyg {
     bool same_f = static_cast<long>(E.last_frozen.size()) == n &&
                   std::memcmp(E.last_frozen.data(), frozen, n * sizeof(uint8_t)) == 0;
     if (!same_t) {
-      for (long i = 0; i < n; ++i) require(types[i] < static_cast<uint32_t>(E.tab.ntypes), "TypeID out of range");
+      E.staged_single = n > 0 ? static_cast<int>(types[0]) : -1;
+      for (long i = 0; i < n; ++i) {
+        require(types[i] < static_cast<uint32_t>(E.tab.ntypes), "TypeID out of range");
+        if (types[i] != types[0]) E.staged_single = -1;
+      }
       E.last_types.assign(types, types + n);
       HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, E.ctx->stream));
     }

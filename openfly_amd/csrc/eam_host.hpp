@@ -66,6 +66,7 @@ struct flyg_eam_s {
   std::vector<uint32_t> last_types;
   std::vector<uint8_t> last_frozen;
   int lanes = 0;                                   // lanes per atom row, 0 = timed choice
+  int staged_single = -1;                          // one species in the staged host types, else -1
   std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes
 };
 
@@ -117,6 +118,7 @@ namespace flyg {
     long n;
     uint32_t const* types;
     uint8_t const* frozen;
+    int t0;   // the one species present (-1: several or unknown); read by the single-species kernels
   };
   void eam_setup_tables(flyg_eam_s& e);
   double eam_energy(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen);
