@@ -1678,7 +1678,8 @@ struct flyg_batch_s {
   std::vector<flyg::Pending> pending;
   std::vector<flyg::Result> results;
   int tuned_T = 0, tuned_L = 0;
-  flyg::Flight flight;
+  flyg::Flight flight;                                   // on the context's stream
+  std::vector<std::unique_ptr<flyg::Flight>> extra;      // relaunches, on streams of their own
 };
 
 struct flyg_server_s {
@@ -1891,21 +1892,63 @@ int flyg_batch_run(flyg_batch b) {
     } else if (!b->tuned_T) {
       autotune_shape(*b->eam, b->flight, b->pending, d, mix, slots, b->tuned_T, b->tuned_L);
     }
+    // All queued searches go out in one launch; a search that leaves its region is relaunched with
+    // larger skins on another stream as soon as its flag shows it finished, overlapping the rest.
+    constexpr int MAX_LAUNCHES = 6;
     std::vector<Pending> todo = std::move(b->pending);
     b->pending.clear();
-    for (int round = 0; round < 6 && !todo.empty(); ++round) {
-      ensure_regions(todo, b->eam->tab.rc);
-      flight_launch(*b->eam, b->flight, std::move(todo), b->tuned_T, b->tuned_L, d, mix, slots);
-      HIPCHK(hipEventSynchronize(b->flight.e_done));
-      todo.clear();
-      flight_harvest(b->flight, b->eam->tab.rc, [&](Pending& P, Result&& R) { b->results[P.id] = std::move(R); }, todo);
-    }
-    for (auto& P : todo) {   // still outside the largest region the box allows
+    double const rc = b->eam->tab.rc;
+    auto give_up = [&](Pending& P) {   // still outside the largest region the box allows
       auto& R = b->results[P.id];
       R.done = 1;
       R.st.exit_code = EXIT_REGION;
       R.st.launches = P.launches;
       R.mobile = P.mobile;
+    };
+    auto free_flight = [&]() -> Flight* {
+      if (!b->flight.busy) return &b->flight;
+      for (auto& f : b->extra)
+        if (!f->busy) return f.get();
+      if (b->extra.size() >= 3) return nullptr;
+      hipStream_t st;
+      HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      b->extra.push_back(std::make_unique<Flight>());
+      b->extra.back()->init(st, true);
+      return b->extra.back().get();
+    };
+    while (true) {
+      bool launched = false;
+      if (!todo.empty()) {
+        std::vector<Pending> go;
+        for (auto& P : todo) {
+          if (P.launches >= MAX_LAUNCHES) {
+            give_up(P);
+          } else {
+            go.push_back(std::move(P));
+          }
+        }
+        todo.clear();
+        if (!go.empty()) {
+          Flight* F = free_flight();
+          if (F) {
+            ensure_regions(go, rc);
+            flight_launch(*b->eam, *F, std::move(go), b->tuned_T, b->tuned_L, d, mix, slots);
+            launched = true;
+          } else {
+            todo = std::move(go);
+          }
+        }
+      }
+      bool busy = false;
+      auto harvest = [&](Flight& F) {
+        if (!F.busy) return;
+        flight_harvest(F, rc, [&](Pending& P, Result&& R) { b->results[P.id] = std::move(R); }, todo);
+        busy = busy || F.busy;
+      };
+      harvest(b->flight);
+      for (auto& f : b->extra) harvest(*f);
+      if (!busy && todo.empty()) break;
+      if (!launched) std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
   });
 }
