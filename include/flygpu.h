@@ -248,6 +248,38 @@ int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double
 int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region);
 
 /* ---------------------------------------------------------------------------------------------------
+ * Mechanisms: Master::do_adj_min + the pathway checks of Master::saddle_2_mech (find.cpp:757-837,
+ * 1014-1090) on the device, right after a search converges: relax from the saddle nudged by
+ * +-nudge_frac |sp - min| along the axis (LBFGS::minimise with `minimiser`), call the minimum nearer
+ * the initial one rev, remove the centroid drift of rev (over all atoms), and require
+ * |rev - min| <= basin_tol, |fwd - rev| >= basin_tol, |rev - sp| and |fwd - sp| >= stationary_tol, the
+ * atom moving most from rev to fwd to be the search's centre, and E(sp) > E(rev).
+ * ------------------------------------------------------------------------------------------------- */
+typedef struct {
+  int enabled;                   /* 0 (default): no mechanism step */
+  double nudge_frac;             /* Master::Options::nudge_frac (find.hpp:184), 0.01 */
+  double basin_tol;              /* Master::Options::basin_tol (find.hpp:112), 0.05 A */
+  double stationary_tol;         /* Master::Options::stationary_tol (find.hpp:120), basin_tol / 2 */
+  flyg_lbfgs_options minimiser;  /* LBFGS::Options of the two relaxations (history length: the dimer's) */
+} flyg_mech_options;
+
+typedef struct {
+  int code;          /* 0 pathway; 1 a minimisation failed; 2 unlinked (rev away from the initial minimum);
+                        3 converged back to the initial minimum; 4 min->sp or 5 sp->min displacement below
+                        stationary_tol; 6 centred on another atom; 7 E(sp) <= E(rev); -1 not checked */
+  int min_iters;     /* L-BFGS iterations of the two relaxations */
+  double barrier;    /* E(sp) - E(rev), FP64 */
+  double delta;      /* E(fwd) - E(rev), FP64 */
+  double drift[3];   /* centroid drift of rev from the initial minimum; the pathway is rev, sp, fwd - drift */
+} flyg_mechanism;
+
+void flyg_mech_default_options(flyg_mech_options* opt);
+int flyg_batch_set_mechanisms(flyg_batch b, const flyg_mech_options* opt);
+/* Mechanism of search `id` (stats.exit_code 0 only); rev_pos / fwd_pos: full 3n positions of the two
+ * minima before the drift correction (either may be NULL). */
+int flyg_batch_mechanism(flyg_batch b, int64_t id, flyg_mechanism* mech, double* rev_pos, double* fwd_pos);
+
+/* ---------------------------------------------------------------------------------------------------
  * Serving: continuous batching of saddle-search requests from many systems (a kMC loop's catalogue
  * misses, Master::find_mechs -> find_batch, find.cpp:211-312,684-755, one system per request stream).
  * Each admitted system stays resident on the device (positions, base densities, types) until removed
@@ -300,6 +332,9 @@ int flyg_server_result(flyg_server s, int64_t id, flyg_search_stats* stats, int*
                        double* pos, double* axis);
 /* Forget a finished search's result. */
 int flyg_server_release(flyg_server s, int64_t id);
+int flyg_server_set_mechanisms(flyg_server s, const flyg_mech_options* opt);
+/* Mechanism of a finished search; rev / fwd: 3 per mobile atom (the order of flyg_server_result's atoms). */
+int flyg_server_mechanism(flyg_server s, int64_t id, flyg_mechanism* mech, double* rev, double* fwd);
 int flyg_server_status(flyg_server s, flyg_server_state* state);
 
 #ifdef __cplusplus

@@ -59,6 +59,8 @@ namespace flyg {
       double const* srho;     // FP64 base densities
       uint8_t const* stype;   // type ids
       double3 const* sS;      // the 27 image shifts of the box, by shift code
+      long natoms;            // atoms in the system (centroids run over all of them)
+      int centre_local;       // mobile index of the search's centre atom
     };
 
     struct SearchOut {
@@ -66,6 +68,9 @@ namespace flyg {
       long grads;
       double curv, barrier, e_base, max_disp;
       double max_pair;   // largest d1 + d2 seen
+      int mech_code, min_iters;      // mechanism check (0 = a min->sp->min pathway), minimiser iterations
+      double mech_barrier, mech_delta;   // E(sp) - E(rev), E(fwd) - E(rev)
+      double drift[3];                   // centroid drift of rev from the initial minimum (all atoms)
       long ticks;        // device wall-clock ticks the workgroup spent on this search
       long grad_ticks;   // ... of which inside region gradients
     };
@@ -73,6 +78,10 @@ namespace flyg {
     struct DimOpt {
       int n, max_steps, convex_max, rn, iter_max_rot, relax_in_convex;
       double f2norm, proj_tol, max_trust, min_trust, grow_trust, shrink_trust, delta_r, theta_tol;
+      // Mechanisms (Master::do_adj_min / saddle_2_mech): 0 = off.
+      int mech, min_iter_max;
+      double nudge_frac, basin_tol, stationary_tol;
+      double min_f2norm, min_proj_tol, min_max_trust, min_min_trust, min_grow_trust, min_shrink_trust;
     };
 
     struct BatchArgs {
@@ -94,6 +103,8 @@ namespace flyg {
       double const* start_ax;
       double* out_x;
       double* out_ax;
+      double* out_rev;           // mechanism minima (when mechanisms are on)
+      double* out_fwd;
       SearchOut* out;
       unsigned* queue;
       int* done;                 // per-search completion flags (host-visible memory)
@@ -906,12 +917,191 @@ namespace flyg {
         barrier = e_sp - e_base;
         if (e2) ret = EXIT_OOB;
       }
+      // Master::do_adj_min + saddle_2_mech (find.cpp:757-837, 1014-1090): relax from the saddle nudged
+      // by +-nudge_frac |sp - min| along the axis (LBFGS::minimise, lbfgs.cpp:29-115), call the minimum
+      // nearer the initial one rev, remove the centroid drift of rev, and check the pathway.
+      int mech_code = -1, min_iters = 0;
+      double mech_barrier = 0, mech_delta = 0, drift[3] = {0, 0, 0};
+      double* xsp = axp;    // free after the search
+      double* xfw = g1p;
+      if (o.mech && ret == 0) {
+        auto minimise = [&](int& iters) -> int {   // 0 converged, 1 iter_max, EXIT_REGION, EXIT_OOB
+          tc.k = 0;
+          place(v, tm, x, 0.0, nullptr);
+          if (!displacement_ok()) return EXIT_REGION;
+          ensure_active();
+          err |= timed_gradient(g0);
+          ++grads;
+          if (err) return EXIT_OOB;
+          double mag = tm.dot(g0, g0, n3);
+          double trust = o.min_min_trust;
+          for (int i = 0; i < o.min_iter_max; ++i) {
+            iters = i;
+            if (mag < o.min_f2norm * o.min_f2norm) return 0;
+            double* q = tc.newton_step(tm, x, g0);
+            double nrm = sqrt(tm.dot(q, q, n3));
+            double scale = fmin(1.0, trust / nrm);
+            tm.lin(q, n3, q, scale);
+            tm.lin(x, n3, x, 1.0, q, -1.0);
+            __syncthreads();
+            if (!displacement_ok()) return EXIT_REGION;
+            place(v, tm, x, 0.0, nullptr);
+            ensure_active();
+            err |= timed_gradient(g0);
+            ++grads;
+            if (err) return EXIT_OOB;
+            double proj;
+            tm.dot2(g0, q, g0, g0, n3, proj, mag);
+            if (proj < -o.min_proj_tol) {
+              trust = fmax(o.min_min_trust, o.min_shrink_trust * trust);
+            } else if (proj > o.min_proj_tol) {
+              trust = fmin(o.min_max_trust, o.min_grow_trust * trust);
+            }
+          }
+          iters = o.min_iter_max;
+          return 1;
+        };
+        tm.lin(xsp, n3, x, 1.0);
+        tm.lin(dg, n3, x, 1.0, xb, -1.0);
+        double disp = sqrt(tm.dot(dg, dg, n3));
+        int it_f = 0, it_r = 0;
+        tm.lin(x, n3, xsp, 1.0, ax, disp * o.nudge_frac);
+        int rf = minimise(it_f);
+        tm.lin(xfw, n3, x, 1.0);
+        int rr = rf == 0 ? (tm.lin(x, n3, xsp, 1.0, ax, -disp * o.nudge_frac), minimise(it_r)) : rf;
+        min_iters = it_f + it_r;
+        if (rf == EXIT_REGION || rr == EXIT_REGION) {
+          ret = EXIT_REGION;   // re-run with larger skins (the minima lie outside the rows)
+        } else if (rf == EXIT_OOB || rr == EXIT_OOB) {
+          ret = EXIT_OOB;
+        } else if (rf != 0 || rr != 0) {
+          mech_code = 1;       // a minimisation failed
+        } else {
+          double* xrv = x;
+          double df, dr2;
+          tm.lin(dg, n3, xfw, 1.0, xb, -1.0);
+          tm.lin(dgp, n3, xrv, 1.0, xb, -1.0);
+          tm.dot2(dg, dg, dgp, dgp, n3, df, dr2);
+          if (dr2 > df) {      // forward went to the initial minimum: swap
+            double* t = xrv;
+            xrv = xfw;
+            xfw = t;
+          }
+          // Drift of the centroid over all natoms (frozen atoms do not move).
+          double sdx = 0, sdy = 0, sdz = 0;
+          {
+            double vv[Team<T>::NQ][4] = {};
+            for (int i0 = 0; i0 < v.nm; i0 += Team<T>::VS)
+#pragma unroll
+              for (int q = 0; q < Team<T>::NQ; ++q) {
+                int a = i0 + tm.tid + q * T;
+                if (a < v.nm) {
+                  vv[q][0] = __dadd_rn(vv[q][0], xrv[3 * a] - xb[3 * a]);
+                  vv[q][1] = __dadd_rn(vv[q][1], xrv[3 * a + 1] - xb[3 * a + 1]);
+                  vv[q][2] = __dadd_rn(vv[q][2], xrv[3 * a + 2] - xb[3 * a + 2]);
+                }
+              }
+            double o3[3];
+            tm.template sum<3>(o3, vv);
+            sdx = o3[0];
+            sdy = o3[1];
+            sdz = o3[2];
+          }
+          double const Nd = static_cast<double>(h.natoms);
+          drift[0] = sdx / Nd;
+          drift[1] = sdy / Nd;
+          drift[2] = sdz / Nd;
+          // |rev - drift - in| over all atoms: mobile terms + the frozen atoms shifted by -drift.
+          for (int i = tm.tid; i < n3; i += T) dg[i] = xrv[i] - drift[i % 3] - xb[i];
+          double i2r2 = tm.dot(dg, dg, n3) + (Nd - v.nm) * (drift[0] * drift[0] + drift[1] * drift[1] + drift[2] * drift[2]);
+          double r2f2, r2w2, w2f2;
+          tm.lin(dg, n3, xfw, 1.0, xrv, -1.0);
+          tm.lin(dgp, n3, xrv, 1.0, xsp, -1.0);
+          tm.dot2(dg, dg, dgp, dgp, n3, r2f2, r2w2);
+          tm.lin(dgp, n3, xfw, 1.0, xsp, -1.0);
+          w2f2 = tm.dot(dgp, dgp, n3);
+          // Atom that moves most from rev to fwd (first of equals in atom order) must be the centre.
+          double best = 0;
+          int arg = 0;
+          for (int a = tm.tid; a < v.nm; a += T) {
+            double ddx = xfw[3 * a] - xrv[3 * a], ddy = xfw[3 * a + 1] - xrv[3 * a + 1], ddz = xfw[3 * a + 2] - xrv[3 * a + 2];
+            double r = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (r > best) {
+              best = r;
+              arg = a;
+            }
+          }
+          for (int off = 32; off > 0; off >>= 1) {
+            double ob = __shfl_xor(best, off, 64);
+            int oa = __shfl_xor(arg, off, 64);
+            if (ob > best || (ob == best && oa < arg)) {
+              best = ob;
+              arg = oa;
+            }
+          }
+          __shared__ double mb[16];
+          __shared__ int ma[16];
+          if ((tm.tid & 63) == 0) {
+            mb[tm.tid >> 6] = best;
+            ma[tm.tid >> 6] = arg;
+          }
+          __syncthreads();
+          best = mb[0];
+          arg = ma[0];
+          for (int w = 1; w < T / 64; ++w)
+            if (mb[w] > best || (mb[w] == best && ma[w] < arg)) {
+              best = mb[w];
+              arg = ma[w];
+            }
+          __syncthreads();
+          // Energies (local FP64: the frozen terms cancel in the differences).
+          unsigned e3 = 0;
+          place(v, tm, xrv, 0.0, nullptr);
+          double e_rev = region_energy<T, L, MULTI>(A, v, tm, false, e3);
+          place(v, tm, xfw, 0.0, nullptr);
+          double e_fwd = region_energy<T, L, MULTI>(A, v, tm, false, e3);
+          double e_sp = barrier + e_base;
+          mech_barrier = e_sp - e_rev;
+          mech_delta = e_fwd - e_rev;
+          double const bt = o.basin_tol, st = o.stationary_tol;
+          if (e3) {
+            ret = EXIT_OOB;
+          } else if (i2r2 > bt * bt) {
+            mech_code = 2;     // starts away from the initial minimum: unlinked
+          } else if (r2f2 < bt * bt) {
+            mech_code = 3;     // converged back to the initial minimum
+          } else if (r2w2 < st * st) {
+            mech_code = 4;     // min -> sp displacement below the stationary tolerance
+          } else if (w2f2 < st * st) {
+            mech_code = 5;     // sp -> min displacement below the stationary tolerance
+          } else if (arg != h.centre_local) {
+            mech_code = 6;     // the mechanism is centred elsewhere
+          } else if (!(e_sp > e_rev)) {
+            mech_code = 7;     // saddle not above its minimum
+          } else {
+            mech_code = 0;
+          }
+          for (int i = tm.tid; i < n3; i += T) {
+            A.out_rev[h.v3_off + i] = xrv[i];
+            A.out_fwd[h.v3_off + i] = xfw[i];
+          }
+        }
+        tm.lin(x, n3, xsp, 1.0);   // the saddle goes out as the search's positions
+        __syncthreads();
+      }
       for (int i = tm.tid; i < n3; i += T) {
         A.out_x[h.v3_off + i] = x[i];
         A.out_ax[h.v3_off + i] = ax[i];
       }
       if (tm.tid == 0) {
         SearchOut r;
+        r.mech_code = mech_code;
+        r.min_iters = min_iters;
+        r.mech_barrier = mech_barrier;
+        r.mech_delta = mech_delta;
+        r.drift[0] = drift[0];
+        r.drift[1] = drift[1];
+        r.drift[2] = drift[2];
         r.exit_code = ret;
         r.steps = steps;
         r.rot_iters = rot_iters;
@@ -1149,6 +1339,8 @@ namespace flyg {
       std::vector<int> mobile;
       std::vector<double> x, ax;
       int nr = 0;
+      flyg_mechanism mech{-1, 0, 0.0, 0.0, {0.0, 0.0, 0.0}};
+      std::vector<double> rev, fwd;   // mechanism minima (mobile atoms), when checked
     };
 
     double min_image(double d, double L) { return d - L * std::round(d / L); }
@@ -1162,8 +1354,10 @@ namespace flyg {
 
     // Queues one search of system S: mobile set around `centre`, start state continuous with the base.
     // Start positions/axis come either dense (3n arrays) or sparse (listed atoms only; the rest at base).
+    // With mechanisms on, the rows must also hold the two adjacent minima (a hop moves one atom by a
+    // near-neighbour distance), so the skins start at mech_f / mech_m.
     Pending make_pending(SysState& S, double rc, int64_t id, int64_t centre, double radius, double skin,
-                         double const* sp, double const* sa, int64_t const* kidx, long nk) {
+                         double const* sp, double const* sa, int64_t const* kidx, long nk, bool mech = false) {
       require(centre >= 0 && centre < S.n, "centre out of range");
       require(radius > 0 && radius < 0.5 * std::fmin(S.box.ext[0], std::fmin(S.box.ext[1], S.box.ext[2])) - 1.0,
               "active radius must be positive and below half the box");
@@ -1210,8 +1404,9 @@ namespace flyg {
         }
       }
       double cap = S.skin_cap(rc);
-      P.skin_f = std::min(cap, std::max(skin, d1 + 1.0));
-      P.skin_m = std::min(cap, std::max(std::max(2.0 * skin, P.skin_f), d1 + d2 + 1.5));
+      double const mech_f = mech ? 3.0 : 0.0, mech_m = mech ? 4.5 : 0.0;
+      P.skin_f = std::min(cap, std::max(std::max(skin, mech_f), d1 + 1.0));
+      P.skin_m = std::min(cap, std::max(std::max(std::max(2.0 * skin, mech_m), P.skin_f), d1 + d2 + 1.5));
       return P;
     }
 
@@ -1317,8 +1512,21 @@ namespace flyg {
       throw Error(FLYG_INVALID_ARG, "unsupported (block_threads, lanes_per_atom); use 256, 512 or 1024 with 8 or 16");
     }
 
-    DimOpt dim_opt(flyg_dimer_options const& o, flyg_rotor_options const& r) {
+    DimOpt dim_opt(flyg_dimer_options const& o, flyg_rotor_options const& r, flyg_mech_options const* m = nullptr) {
       DimOpt d{};
+      if (m && m->enabled) {
+        d.mech = 1;
+        d.nudge_frac = m->nudge_frac;
+        d.basin_tol = m->basin_tol;
+        d.stationary_tol = m->stationary_tol;
+        d.min_iter_max = m->minimiser.iter_max;
+        d.min_f2norm = m->minimiser.f2norm;
+        d.min_proj_tol = m->minimiser.proj_tol;
+        d.min_max_trust = m->minimiser.max_trust;
+        d.min_min_trust = m->minimiser.min_trust;
+        d.min_grow_trust = m->minimiser.grow_trust;
+        d.min_shrink_trust = m->minimiser.shrink_trust;
+      }
       d.n = o.n;
       d.max_steps = o.max_steps;
       d.convex_max = o.convex_max;
@@ -1379,7 +1587,8 @@ namespace flyg {
       long n3 = 0;
       DBuf<char> d_in, d_scr;
       Pinned h_in, h_out;
-      std::size_t o_out = 0, o_ox = 0, o_oax = 0, out_bytes = 0;   // offsets in h_out (flags at 0)
+      std::size_t o_out = 0, o_ox = 0, o_oax = 0, o_rev = 0, o_fwd = 0, out_bytes = 0;   // offsets in h_out
+      int mech = 0;
       std::vector<long> v3off;      // per search: first element of its 3*nm output vectors
       std::vector<char> harvested;  // per search
       long left = 0;                // searches not yet harvested
@@ -1454,6 +1663,9 @@ namespace flyg {
         h.srho = P.sys->d_rho.get();
         h.stype = P.sys->d_type.get();
         h.sS = P.sys->d_S.get();
+        h.natoms = P.sys->n;
+        h.centre_local = static_cast<int>(std::lower_bound(P.mobile.begin(), P.mobile.end(), static_cast<int>(P.centre)) -
+                                          P.mobile.begin());
         nreg += g.nr;
         nrow += g.nr + 1;
         nent += static_cast<long>(g.ent.size());
@@ -1476,6 +1688,8 @@ namespace flyg {
       F.o_out = ho.add<SearchOut>(ns);
       F.o_ox = ho.add<double>(n3);
       F.o_oax = ho.add<double>(n3);
+      F.o_rev = ho.add<double>(dopt.mech ? n3 : 0);
+      F.o_fwd = ho.add<double>(dopt.mech ? n3 : 0);
       F.out_bytes = ho.size;
       // Grow with slack: freeing device or pinned memory synchronises the whole device, which would
       // stall the other streams' launches.
@@ -1535,6 +1749,9 @@ namespace flyg {
       A.done = reinterpret_cast<int*>(hb_out + o_flags);
       A.out_x = reinterpret_cast<double*>(hb_out + F.o_ox);
       A.out_ax = reinterpret_cast<double*>(hb_out + F.o_oax);
+      A.out_rev = reinterpret_cast<double*>(hb_out + F.o_rev);
+      A.out_fwd = reinterpret_cast<double*>(hb_out + F.o_fwd);
+      F.mech = dopt.mech;
       A.out = reinterpret_cast<SearchOut*>(hb_out + F.o_out);
       F.v3off.resize(ns);
       for (long k = 0; k < ns; ++k) F.v3off[k] = hdr[k].v3_off;
@@ -1580,6 +1797,8 @@ namespace flyg {
       SearchOut const* outs = reinterpret_cast<SearchOut const*>(F.h_out.p + F.o_out);
       double const* ox = reinterpret_cast<double const*>(F.h_out.p + F.o_ox);
       double const* oax = reinterpret_cast<double const*>(F.h_out.p + F.o_oax);
+      double const* orv = reinterpret_cast<double const*>(F.h_out.p + F.o_rev);
+      double const* ofw = reinterpret_cast<double const*>(F.h_out.p + F.o_fwd);
       for (long k = 0; k < static_cast<long>(F.work.size()); ++k) {
         if (F.harvested[k]) continue;
         if (!finished && __atomic_load_n(&flags[k], __ATOMIC_ACQUIRE) == 0) continue;
@@ -1615,6 +1834,15 @@ namespace flyg {
         R.mobile = P.mobile;
         R.x.assign(ox + off, ox + off + 3L * nm);
         R.ax.assign(oax + off, oax + off + 3L * nm);
+        R.mech.code = o.mech_code;
+        R.mech.min_iters = o.min_iters;
+        R.mech.barrier = o.mech_barrier;
+        R.mech.delta = o.mech_delta;
+        for (int c = 0; c < 3; ++c) R.mech.drift[c] = o.drift[c];
+        if (F.mech && o.mech_code >= 0 && o.mech_code != 1) {
+          R.rev.assign(orv + off, orv + off + 3L * nm);
+          R.fwd.assign(ofw + off, ofw + off + 3L * nm);
+        }
         R.nr = P.region->nr;
         done(P, std::move(R));
       }
@@ -1677,6 +1905,7 @@ struct flyg_batch_s {
   std::unique_ptr<flyg::SysState> sys;
   std::vector<flyg::Pending> pending;
   std::vector<flyg::Result> results;
+  flyg_mech_options mopt{};                              // mechanisms off unless set
   int tuned_T = 0, tuned_L = 0;
   flyg::Flight flight;                                   // on the context's stream
   std::vector<std::unique_ptr<flyg::Flight>> extra;      // relaunches, on streams of their own
@@ -1688,6 +1917,7 @@ struct flyg_server_s {
   flyg_dimer_options opt;
   flyg_rotor_options ropt;
   flyg_server_options sopt;
+  flyg_mech_options mopt{};
   std::map<int64_t, std::unique_ptr<flyg::SysState>> systems;
   int64_t next_sys = 0;
   std::deque<flyg::Pending> queue;
@@ -1741,7 +1971,7 @@ namespace flyg {
         }
       }
       if (S.queue.empty()) return fin;
-      DimOpt d = dim_opt(S.opt, S.ropt);
+      DimOpt d = dim_opt(S.opt, S.ropt, &S.mopt);
       bool mix = S.sopt.precision == FLYG_MIXED;
       int slots = slots_of(*S.ctx, S.sopt.max_concurrent);
       if (!S.tuned_T) {
@@ -1873,7 +2103,7 @@ int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const doub
       int64_t id = static_cast<int64_t>(b->results.size());
       b->pending.push_back(make_pending(*b->sys, b->eam->tab.rc, id, centre[k], radius[k], b->bopt.list_skin,
                                         start_pos + 3 * n * static_cast<long>(k),
-                                        start_axis + 3 * n * static_cast<long>(k), nullptr, 0));
+                                        start_axis + 3 * n * static_cast<long>(k), nullptr, 0, b->mopt.enabled != 0));
       b->results.emplace_back();
       if (ids) ids[k] = id;
     }
@@ -1883,7 +2113,7 @@ int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const doub
 int flyg_batch_run(flyg_batch b) {
   return bguard(b->ctx, [&] {
     if (b->pending.empty()) return;
-    DimOpt d = dim_opt(b->opt, b->ropt);
+    DimOpt d = dim_opt(b->opt, b->ropt, &b->mopt);
     bool mix = b->bopt.precision == FLYG_MIXED;
     int slots = slots_of(*b->ctx, b->bopt.max_concurrent);
     if (b->bopt.block_threads > 0 && b->bopt.lanes_per_atom > 0) {
@@ -1960,6 +2190,49 @@ int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double
     require(R.done, "search has not run yet");
     if (stats) *stats = R.st;
     fill_positions(*b->sys, R, pos, axis);
+  });
+}
+
+void flyg_mech_default_options(flyg_mech_options* o) {
+  flyg_mech_options m{};
+  m.enabled = 0;
+  m.nudge_frac = 0.01;
+  m.basin_tol = 0.05;
+  m.stationary_tol = 0.025;
+  flyg_lbfgs_default_options(&m.minimiser);
+  *o = m;
+}
+
+namespace {
+  void check_mech(flyg_mech_options const* opt) {
+    require(opt != nullptr, "null argument");
+    require(opt->nudge_frac > 0 && opt->basin_tol > 0 && opt->stationary_tol > 0, "mechanism tolerances must be positive");
+    require(opt->minimiser.iter_max > 0 && opt->minimiser.f2norm > 0, "bad minimiser options");
+  }
+}  // namespace
+
+int flyg_batch_set_mechanisms(flyg_batch b, const flyg_mech_options* opt) {
+  return bguard(b->ctx, [&] {
+    check_mech(opt);
+    b->mopt = *opt;
+  });
+}
+
+int flyg_batch_mechanism(flyg_batch b, int64_t id, flyg_mechanism* mech, double* rev_pos, double* fwd_pos) {
+  return bguard(b->ctx, [&] {
+    require(id >= 0 && id < static_cast<int64_t>(b->results.size()), "unknown search id");
+    auto const& R = b->results[id];
+    require(R.done, "search has not run yet");
+    if (mech) *mech = R.mech;
+    auto fill = [&](std::vector<double> const& v, double* out) {
+      if (!out) return;
+      std::memcpy(out, b->sys->base.data(), 3 * b->sys->n * sizeof(double));
+      for (std::size_t i = 0; i < R.mobile.size() && 3 * i + 2 < v.size(); ++i)
+        for (int c = 0; c < 3; ++c) out[3 * R.mobile[i] + c] = v[3 * i + c];
+    };
+    require((!rev_pos && !fwd_pos) || !R.rev.empty(), "no mechanism minima for this search");
+    fill(R.rev, rev_pos);
+    fill(R.fwd, fwd_pos);
   });
 }
 
@@ -2068,7 +2341,7 @@ int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* 
       long a = kick_offsets[k], b = kick_offsets[k + 1];
       require(a >= 0 && b >= a, "kick offsets must be non-decreasing");
       add.push_back(make_pending(S, s->eam->tab.rc, s->next_id + k, centre[k], radius[k], s->sopt.list_skin, kick_pos + 3 * a,
-                                 kick_axis + 3 * a, kick_atoms + a, b - a));
+                                 kick_axis + 3 * a, kick_atoms + a, b - a, s->mopt.enabled != 0));
     }
     for (int k = 0; k < count; ++k) {
       int64_t id = s->next_id++;
@@ -2125,6 +2398,25 @@ int flyg_server_release(flyg_server s, int64_t id) {
   return bguard(s->ctx, [&] {
     s->results.erase(id);
     s->owner.erase(id);
+  });
+}
+
+int flyg_server_set_mechanisms(flyg_server s, const flyg_mech_options* opt) {
+  return bguard(s->ctx, [&] {
+    check_mech(opt);
+    s->mopt = *opt;
+  });
+}
+
+int flyg_server_mechanism(flyg_server s, int64_t id, flyg_mechanism* mech, double* rev, double* fwd) {
+  return bguard(s->ctx, [&] {
+    auto it = s->results.find(id);
+    require(it != s->results.end() && it->second.done, "search has not finished (or unknown id)");
+    Result const& R = it->second;
+    if (mech) *mech = R.mech;
+    require((!rev && !fwd) || !R.rev.empty(), "no mechanism minima for this search");
+    if (rev) std::memcpy(rev, R.rev.data(), R.rev.size() * sizeof(double));
+    if (fwd) std::memcpy(fwd, R.fwd.data(), R.fwd.size() * sizeof(double));
   });
 }
 

@@ -86,6 +86,24 @@ class BatchOptions(C.Structure):
                 ("list_skin", C.c_double), ("max_concurrent", C.c_int)]
 
 
+class MechOptions(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("nudge_frac", C.c_double), ("basin_tol", C.c_double),
+                ("stationary_tol", C.c_double), ("minimiser", LBFGSOptions)]
+
+
+class Mechanism(C.Structure):
+    _fields_ = [("code", C.c_int), ("min_iters", C.c_int), ("barrier", C.c_double), ("delta", C.c_double),
+                ("drift", C.c_double * 3)]
+
+    def as_dict(self):
+        return {"code": self.code, "min_iters": self.min_iters, "barrier": self.barrier, "delta": self.delta,
+                "drift": np.array(self.drift[:])}
+
+
+MECH_NAMES = {0: "pathway", 1: "minimisation failed", 2: "unlinked", 3: "back to initial", 4: "min-sp stationary",
+              5: "sp-min stationary", 6: "centred elsewhere", 7: "saddle below minimum", -1: "not checked"}
+
+
 class ServerOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("block_threads", C.c_int), ("lanes_per_atom", C.c_int),
                 ("list_skin", C.c_double), ("max_concurrent", C.c_int), ("streams", C.c_int), ("max_batch", C.c_int),
@@ -152,6 +170,11 @@ _SIGS = {
     "flyg_batch_run": [_vp],
     "flyg_batch_result": [_vp, C.c_int64, C.POINTER(SearchStats), _d, _d],
     "flyg_batch_region": [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "flyg_mech_default_options": [C.POINTER(MechOptions)],
+    "flyg_batch_set_mechanisms": [_vp, C.POINTER(MechOptions)],
+    "flyg_batch_mechanism": [_vp, C.c_int64, C.POINTER(Mechanism), _d, _d],
+    "flyg_server_set_mechanisms": [_vp, C.POINTER(MechOptions)],
+    "flyg_server_mechanism": [_vp, C.c_int64, C.POINTER(Mechanism), _d, _d],
     "flyg_server_default_options": [C.POINTER(ServerOptions)],
     "flyg_server_create": [_vp, _vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), C.POINTER(ServerOptions),
                            C.POINTER(_vp)],
@@ -165,7 +188,7 @@ _SIGS = {
     "flyg_server_status": [_vp, C.POINTER(ServerState)],
 }
 _VOID = {"flyg_lbfgs_default_options", "flyg_dimer_default_options", "flyg_rotor_default_options",
-         "flyg_batch_default_options", "flyg_server_default_options"}
+         "flyg_batch_default_options", "flyg_server_default_options", "flyg_mech_default_options"}
 
 
 def lib():
@@ -558,6 +581,23 @@ class Batch:
     def run(self):
         self.ctx.check(lib().flyg_batch_run(self.h))
 
+    def set_mechanisms(self, **kw):
+        """Check every converged saddle as a min->sp->min mechanism (Master::do_adj_min / saddle_2_mech)."""
+        self.mopt = mech_options(**kw)
+        self.ctx.check(lib().flyg_batch_set_mechanisms(self.h, C.byref(self.mopt)))
+
+    def mechanism(self, sid, positions=False):
+        m = Mechanism()
+        rev = np.zeros((self.n, 3)) if positions else None
+        fwd = np.zeros((self.n, 3)) if positions else None
+        self.ctx.check(lib().flyg_batch_mechanism(self.h, int(sid), C.byref(m),
+                                                  rev.ctypes.data_as(_d) if positions else None,
+                                                  fwd.ctypes.data_as(_d) if positions else None))
+        out = m.as_dict()
+        if positions:
+            out.update(rev=rev, fwd=fwd)
+        return out
+
     def result(self, sid, positions=False):
         st = SearchStats()
         pos = np.zeros((self.n, 3)) if positions else None
@@ -577,6 +617,18 @@ class Batch:
         out["n_mobile"] = nm.value
         out["n_region"] = nr.value
         return out
+
+
+def mech_options(minimiser=None, **kw):
+    """flyg_mech_options with enabled = 1; minimiser: dict of LBFGS option overrides."""
+    o = MechOptions()
+    lib().flyg_mech_default_options(C.byref(o))
+    o.enabled = 1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    for k, v in (minimiser or {}).items():
+        setattr(o.minimiser, k, v)
+    return o
 
 
 def server_options(**kw):
@@ -660,6 +712,25 @@ class Server:
             out.update(atoms=atoms, pos=pos, axis=ax)
         if release:
             self.ctx.check(lib().flyg_server_release(self.h, int(sid)))
+        return out
+
+    def set_mechanisms(self, **kw):
+        self.mopt = mech_options(**kw)
+        self.ctx.check(lib().flyg_server_set_mechanisms(self.h, C.byref(self.mopt)))
+
+    def mechanism(self, sid, n_mobile=None, positions=False):
+        """Mechanism of a finished search (call before release); rev/fwd per mobile atom."""
+        m = Mechanism()
+        rev = fwd = None
+        if positions:
+            rev = np.zeros((n_mobile, 3))
+            fwd = np.zeros((n_mobile, 3))
+        self.ctx.check(lib().flyg_server_mechanism(self.h, int(sid), C.byref(m),
+                                                   rev.ctypes.data_as(_d) if positions else None,
+                                                   fwd.ctypes.data_as(_d) if positions else None))
+        out = m.as_dict()
+        if positions:
+            out.update(rev=rev, fwd=fwd)
         return out
 
     def status(self):
