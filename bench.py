@@ -159,6 +159,20 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     out["batched_mixed"]["matching_fp64_within_1e-3eV"] = good
     out["batched_mixed"]["searches_per_s_dE_1e-3"] = good / el
     out["batched_mixed"]["max_abs_dE_vs_fp64"] = float(max(agree)) if agree else None
+    # The same mixed batch with every converged saddle turned into a checked min->sp->min mechanism.
+    log("search leg: batched_mixed with mechanisms")
+    bm = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=fa.MIXED)
+    bm.set_mechanisms()
+    idm = bm.submit(centres, radii, starts, axes)
+    t0 = time.perf_counter()
+    bm.run()
+    el = time.perf_counter() - t0
+    rm = [bm.result(i) for i in idm]
+    mm = [bm.mechanism(i) for i, r in zip(idm, rm) if r["exit"] == 0]
+    codes = np.bincount([m["code"] for m in mm if m["code"] >= 0], minlength=8).tolist()
+    out["batched_mixed_mechanisms"] = {"seconds": el, "searches_per_s": n_search / el,
+                                       "converged": sum(1 for r in rm if r["exit"] == 0), "pathways": codes[0],
+                                       "pathways_per_s": codes[0] / el, "mechanism_codes": codes}
     # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
     log(f"search leg: plain FP64 sample of {plain_sample}")
     dim = fa.Dimer(ctx, box, dimer=dimer)
