@@ -115,6 +115,8 @@ namespace flyg {
     // ---------------------------------------------------------------------------------------------
     // Workgroup primitives.  Element i of a 3*nm vector is always handled by thread i % T, so
     // elementwise chains need no barrier; reductions are fixed-order trees (bitwise reproducible).
+    // Every device function of the search kernel is force-inlined: an out-of-line call spills the
+    // caller's live registers around it and pins the kernel argument block in per-lane scratch.
     // ---------------------------------------------------------------------------------------------
 
     template <int T>
@@ -122,7 +124,7 @@ namespace flyg {
       double* red;   // LDS, 4 * T
       int tid;
 
-      __device__ double wave(double v) const {
+      __device__ __forceinline__ double wave(double v) const {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         return v;
@@ -143,7 +145,7 @@ namespace flyg {
       mutable int par = 0;
 
       template <int K>
-      __device__ void tree(double (&v)[NQ][4]) const {
+      __device__ __forceinline__ void tree(double (&v)[NQ][4]) const {
 #pragma unroll
         for (int q = 0; q < NQ; ++q)
 #pragma unroll
@@ -170,13 +172,13 @@ namespace flyg {
       }
 
       template <int K>
-      __device__ void sum(double* out, double (&v)[NQ][4]) const {
+      __device__ __forceinline__ void sum(double* out, double (&v)[NQ][4]) const {
         tree<K>(v);
 #pragma unroll
         for (int k = 0; k < K; ++k) out[k] = v[0][k];
       }
 
-      __device__ double dot(double const* a, double const* b, int n) const {
+      __device__ __forceinline__ double dot(double const* a, double const* b, int n) const {
         double v[NQ][4] = {};
         for (int i0 = 0; i0 < n; i0 += VS)
 #pragma unroll
@@ -189,7 +191,7 @@ namespace flyg {
         return o[0];
       }
 
-      __device__ void dot2(double const* a, double const* b, double const* c, double const* d, int n, double& x,
+      __device__ __forceinline__ void dot2(double const* a, double const* b, double const* c, double const* d, int n, double& x,
                            double& y) const {
         double v[NQ][4] = {};
         for (int i0 = 0; i0 < n; i0 += VS)
@@ -208,7 +210,7 @@ namespace flyg {
       }
 
       // Workgroup sum of per-thread values that are already in a fixed per-row form (region energy).
-      __device__ double sum1_rows(double x) const {
+      __device__ __forceinline__ double sum1_rows(double x) const {
         double v[NQ][4] = {};
         v[0][0] = x;
         double o[1];
@@ -217,7 +219,7 @@ namespace flyg {
       }
 
       // o = ca a + cb b + cc c (b, c may be null); o may alias a.
-      __device__ void lin(double* o, int n, double const* a, double ca, double const* b = nullptr, double cb = 0,
+      __device__ __forceinline__ void lin(double* o, int n, double const* a, double ca, double const* b = nullptr, double cb = 0,
                           double const* c = nullptr, double cc = 0) const {
         for (int i = tid; i < n; i += T) {
           double v = ca * a[i];
@@ -227,7 +229,7 @@ namespace flyg {
         }
       }
 
-      __device__ unsigned any_or(unsigned e) const {
+      __device__ __forceinline__ unsigned any_or(unsigned e) const {
         __shared__ unsigned flag;
         if (tid == 0) flag = 0;
         __syncthreads();
@@ -243,8 +245,18 @@ namespace flyg {
     // Per-search view
     // ---------------------------------------------------------------------------------------------
 
+    // Spline descriptors of the potential, staged in LDS at kernel start.  The kernels index them by
+    // atom types; indexing the by-value kernel argument instead would place the whole argument block
+    // in per-lane scratch.
+    struct SplineSet {
+      SplineDesc F[MAX_TYPES];
+      SplineDesc phi[MAX_TYPES * MAX_TYPES];
+      SplineDesc V[MAX_TYPES * MAX_TYPES];
+    };
+
     struct SView {
       int nm, nr, n3;
+      SplineSet const* spl; // LDS copy of the spline descriptors
       double3 const* S;     // image shifts (LDS copy)
       double const* srho;   // system base densities
       double4* cur;
@@ -364,7 +376,7 @@ namespace flyg {
     // Rebuilds the active lists from the candidate rows at the current region positions (stable:
     // an active row is the ordered subsequence of its candidate row, whatever the workgroup shape).
     template <int T, int L>
-    __device__ void refresh_active(BatchArgs const& A, SView const& v, Team<T> const& tm) {
+    __device__ __forceinline__ void refresh_active(BatchArgs const& A, SView const& v, Team<T> const& tm) {
       double const cf = A.T.rc + v.act_f, cm = A.T.rc + v.act_m;
       double const cf2 = cf * cf, cm2 = cm * cm;
       int const lane = tm.tid % L;
@@ -413,22 +425,35 @@ namespace flyg {
     // EAM gradient of the mobile atoms at the current region positions (pass 1 rho/F' over M u S1,
     // pass 2 forces over M).  MIX: FP32 pair/spline arithmetic with FP64 accumulation.
     template <int T, int L, bool MIX, bool MULTI>
-    __device__ unsigned region_gradient(BatchArgs const& A, SView const& v, Team<T> const& tm, double* g) {
+    __device__ __forceinline__ unsigned region_gradient(BatchArgs const& A, SView const& v, Team<T> const& tm, double* g) {
       using R = typename Seg<MIX>::R;
       constexpr int NS = 16 / L;
       static_assert(U % NS == 0, "a step must cover whole slot cycles");
       EAMTables const& Tb = A.T;
       int const nt = Tb.ntypes;
+      SplineSet const& SS = *v.spl;
+      // One species: its three descriptors, read once.
+      int const t0 = MULTI ? 0 : A.t0;
+      SplineDesc const phi0 = SS.phi[t0 + nt * t0], V0 = SS.V[t0 + nt * t0], F0 = SS.F[t0];
+      auto PHI = [&](int tj, int ti) -> SplineDesc const& {
+        if constexpr (MULTI) return SS.phi[tj + nt * ti]; else return phi0;
+      };
+      auto VV = [&](int ti, int tj) -> SplineDesc const& {
+        if constexpr (MULTI) return SS.V[vix(nt, ti, tj)]; else return V0;
+      };
+      auto FF = [&](int ti) -> SplineDesc const& {
+        if constexpr (MULTI) return SS.F[ti]; else return F0;
+      };
       unsigned e = 0;
       int const lane = tm.tid % L;
       // Densities of U entries (FP64 values, zero where not used).
       auto densities = [&](Quad<MIX> const& Q, int ti, double (&out)[U]) {
         Seg<MIX> sg[U];
-        decltype(coef_of<MIX>(Tb, Tb.phi[0], sg[0])) cf[U];
+        decltype(coef_of<MIX>(Tb, phi0, sg[0])) cf[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
-          int tj = MULTI ? v.type[Q.j[q]] : A.t0;
-          SplineDesc const& sd = Tb.phi[tj + nt * ti];
+          int tj = MULTI ? v.type[Q.j[q]] : t0;
+          SplineDesc const& sd = PHI(tj, ti);
           sg[q] = seg_of<MIX>(sd, Q.r[q], Q.in[q], e);
           cf[q] = coef_of<MIX>(Tb, sd, sg[q]);
         }
@@ -438,7 +463,7 @@ namespace flyg {
       // Mobile rows (long: every region neighbour): L lanes per row, 16-slot sums.
       for (int r0 = tm.tid / L; r0 < v.nm; r0 += T / L) {
         double4 pi = v.cur[r0];
-        int ti = MULTI ? v.type[r0] : A.t0;
+        int ti = MULTI ? v.type[r0] : t0;
         Acc16<L> rho_acc;
         int b = v.rowptr[r0], c = b + v.alen[r0];
         for (int k0 = b + lane; k0 < c; k0 += U * L) {
@@ -451,13 +476,13 @@ namespace flyg {
             if (Q.in[q]) rho_acc.add_slot(q % NS, d[q]);
         }
         double rho = rho_acc.total();
-        if (lane == 0) v.cur[r0].w = spl64<1>(Tb, Tb.F[ti], rho, e);
+        if (lane == 0) v.cur[r0].w = spl64<1>(Tb, FF(ti), rho, e);
       }
       // Frozen-shell rows (short: mobile neighbours only): one thread per row, entries summed in order,
       // then the frozen part of the density.  Neither order depends on the workgroup shape.
       for (int r0 = v.nm + tm.tid; r0 < v.nr; r0 += T) {
         double4 pi = v.cur[r0];
-        int ti = MULTI ? v.type[r0] : A.t0;
+        int ti = MULTI ? v.type[r0] : t0;
         int b = v.rowptr[r0], c = b + v.alen[r0];
         double rho = 0.0;
         for (int k0 = b; k0 < c; k0 += U) {
@@ -470,31 +495,31 @@ namespace flyg {
             if (Q.in[q]) rho = __dadd_rn(rho, d[q]);
         }
         rho = __dadd_rn(rho, v.rhof[r0]);
-        v.cur[r0].w = spl64<1>(Tb, Tb.F[ti], rho, e);
+        v.cur[r0].w = spl64<1>(Tb, FF(ti), rho, e);
       }
       __syncthreads();
       for (int r0 = tm.tid / L; r0 < v.nm; r0 += T / L) {
         double4 pi = v.cur[r0];
         double fpi = pi.w;
-        int ti = MULTI ? v.type[r0] : A.t0;
+        int ti = MULTI ? v.type[r0] : t0;
         Acc16<L> gx, gy, gz;
         int b = v.rowptr[r0], c = b + v.alen[r0];
         for (int k0 = b + lane; k0 < c; k0 += U * L) {
           Quad<MIX> Q;
           load_quad<MIX>(v, v.act, Tb.rc2, pi, k0, L, b, c, Q);
           Seg<MIX> sv[U], sa[U], sb[U];
-          decltype(coef_of<MIX>(Tb, Tb.phi[0], sv[0])) cv[U], ca[U], cb[U];
+          decltype(coef_of<MIX>(Tb, phi0, sv[0])) cv[U], ca[U], cb[U];
 #pragma unroll
           for (int q = 0; q < U; ++q) {
-            int tj = MULTI ? v.type[Q.j[q]] : A.t0;
-            SplineDesc const& dv = Tb.V[vix(nt, ti, tj)];
-            SplineDesc const& da = Tb.phi[tj + nt * ti];
+            int tj = MULTI ? v.type[Q.j[q]] : t0;
+            SplineDesc const& dv = VV(ti, tj);
+            SplineDesc const& da = PHI(tj, ti);
             sv[q] = seg_of<MIX>(dv, Q.r[q], Q.in[q], e);
             sa[q] = seg_of<MIX>(da, Q.r[q], Q.in[q], e);
             cv[q] = coef_of<MIX>(Tb, dv, sv[q]);
             ca[q] = coef_of<MIX>(Tb, da, sa[q]);
             if (MULTI) {
-              SplineDesc const& db = Tb.phi[ti + nt * tj];
+              SplineDesc const& db = PHI(ti, tj);
               sb[q] = seg_of<MIX>(db, Q.r[q], Q.in[q], e);
               cb[q] = coef_of<MIX>(Tb, db, sb[q]);
             }
@@ -532,15 +557,17 @@ namespace flyg {
     // FP64 local energy (all terms that depend on mobile positions).  With base = true the densities
     // of the region come from the global base pass and S1's frozen density part is recorded.
     template <int T, int L, bool MULTI>
-    __device__ double region_energy(BatchArgs const& A, SView const& v, Team<T> const& tm, bool base, unsigned& err) {
+    __device__ __forceinline__ double region_energy(BatchArgs const& A, SView const& v, Team<T> const& tm, bool base, unsigned& err) {
       EAMTables const& Tb = A.T;
       int const nt = Tb.ntypes;
+      SplineSet const& SS = *v.spl;
+      int const t0 = MULTI ? 0 : A.t0;
       unsigned e = 0;
       int const lane = tm.tid % L;
       // Per-row energies go to the fp scratch (overwritten by the next gradient), then a fixed-order sum.
       for (int r0 = tm.tid / L; r0 < v.nr; r0 += T / L) {
         double4 pi = v.cur[r0];
-        int ti = MULTI ? v.type[r0] : A.t0;
+        int ti = MULTI ? v.type[r0] : t0;
         Acc16<L> rho_acc, pair_acc;
         bool mob = r0 < v.nm;
         int b = v.rowptr[r0], c = v.rowptr[r0 + 1];
@@ -553,9 +580,9 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < Tb.rc2) {
             double r = sqrt(r2);
-            int tj = MULTI ? v.type[j] : A.t0;
-            rho_acc.add(m, spl64<0>(Tb, Tb.phi[tj + nt * ti], r, e));
-            if (mob) pair_acc.add(m, (j < v.nm ? 0.5 : 1.0) * spl64<0>(Tb, Tb.V[vix(nt, ti, tj)], r, e));
+            int tj = MULTI ? v.type[j] : t0;
+            rho_acc.add(m, spl64<0>(Tb, SS.phi[tj + nt * ti], r, e));
+            if (mob) pair_acc.add(m, (j < v.nm ? 0.5 : 1.0) * spl64<0>(Tb, SS.V[vix(nt, ti, tj)], r, e));
           }
         }
         double rho = rho_acc.total();
@@ -568,7 +595,7 @@ namespace flyg {
           } else {
             rt = mob ? rho : rho + v.rhof[r0];
           }
-          v.fp[r0] = spl64<0>(Tb, Tb.F[ti], rt, e) + pair;
+          v.fp[r0] = spl64<0>(Tb, SS.F[ti], rt, e) + pair;
         }
       }
       __syncthreads();
@@ -587,7 +614,7 @@ namespace flyg {
 
     // cur[i] = x + c * d for the mobile atoms (d may be null).
     template <int T>
-    __device__ void place(SView const& v, Team<T> const& tm, double const* x, double c, double const* d) {
+    __device__ __forceinline__ void place(SView const& v, Team<T> const& tm, double const* x, double c, double const* d) {
       __syncthreads();
       for (int i = tm.tid; i < v.nm; i += T) {
         double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
@@ -613,7 +640,7 @@ namespace flyg {
       double* alpha;  // LDS, m
       int m, k, n3;
 
-      __device__ double* newton_step(Team<T> const& tm, double const* x, double const* g) {
+      __device__ __forceinline__ double* newton_step(Team<T> const& tm, double const* x, double const* g) {
         int prev = k > 0 ? (k - 1) % m : 0;
         if (k > 0) {
           double* sp = s + static_cast<long>(prev) * n3;
@@ -653,8 +680,8 @@ namespace flyg {
     };
 
     template <int T, int L, bool MIX, bool MULTI>
-    __device__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
-                               double4* lds_cur, double* lds_ws_buf) {
+    __device__ __forceinline__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
+                               double4* lds_cur, double* lds_ws_buf, SplineSet const* lds_spl) {
       long const t_begin = static_cast<long>(wall_clock64());
       SearchHdr const h = A.hdr[sid];
       DimOpt const& o = A.o;
@@ -663,6 +690,7 @@ namespace flyg {
       v.nr = h.nr;
       v.n3 = 3 * h.nm;
       v.S = lds_S;
+      v.spl = lds_spl;
       v.srho = h.srho;
       // Region positions (w = F' between the two passes of a gradient) live in LDS when they fit.
       v.cur = h.nr <= lds_rows<T>() ? lds_cur : A.cur + h.reg_off;
@@ -1012,7 +1040,10 @@ namespace flyg {
           drift[1] = sdy / Nd;
           drift[2] = sdz / Nd;
           // |rev - drift - in| over all atoms: mobile terms + the frozen atoms shifted by -drift.
-          for (int i = tm.tid; i < n3; i += T) dg[i] = xrv[i] - drift[i % 3] - xb[i];
+          for (int i = tm.tid; i < n3; i += T) {
+            int const c = i % 3;
+            dg[i] = xrv[i] - (c == 0 ? drift[0] : (c == 1 ? drift[1] : drift[2])) - xb[i];
+          }
           double i2r2 = tm.dot(dg, dg, n3) + (Nd - v.nm) * (drift[0] * drift[0] + drift[1] * drift[1] + drift[2] * drift[2]);
           double r2f2, r2w2, w2f2;
           tm.lin(dg, n3, xfw, 1.0, xrv, -1.0);
@@ -1134,14 +1165,25 @@ namespace flyg {
       __shared__ double3 lds_S[27];
       __shared__ double4 lds_cur[lds_rows<T>()];
       __shared__ double lds_wsb[lds_ws<T>()];
+      __shared__ SplineSet lds_spl;
       Team<T> tm{red, static_cast<int>(threadIdx.x)};
+      if (threadIdx.x == 0) {   // constant indices only
+#pragma unroll
+        for (int i = 0; i < MAX_TYPES; ++i) lds_spl.F[i] = A.T.F[i];
+#pragma unroll
+        for (int i = 0; i < MAX_TYPES * MAX_TYPES; ++i) {
+          lds_spl.phi[i] = A.T.phi[i];
+          lds_spl.V[i] = A.T.V[i];
+        }
+      }
+      __syncthreads();
       while (true) {
         if (threadIdx.x == 0) s_id = static_cast<int>(atomicAdd(A.queue, 1u));
         __syncthreads();
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb);
+        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb, &lds_spl);
       }
     }
 
@@ -2052,7 +2094,7 @@ namespace {
     } else {
       flyg_rotor_default_options(&r);
     }
-    require(o.n > 0 && o.n <= 32 && r.n > 0 && r.n <= 32, "batched searches support L-BFGS histories up to 32");
+    require(o.n > 0 && o.n <= 32 && r.n > 0 && r.n <= 32, "batched searches support L-BFGS histories up to 12");
   }
 }  // namespace
 
