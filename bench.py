@@ -28,6 +28,7 @@ HBM_PEAK_GBS = 8000.0
 R_CUT = 5.3
 SKIN = (1.1 ** (1.0 / 3.0) - 1.0) * R_CUT   # LBFGS::minimise skin policy (lbfgs.cpp:39), CPU baseline list
 LIST_SKIN = 0.6   # Verlet skin of the timed update/rebuild loop (rebuild once the accumulated move > skin / 2)
+MECH_F2NORM = 1e-4   # |g| tolerance of the mixed form's mechanism relaxations (its FP32 force noise is ~1e-5 eV/A)
 
 # HBM traffic of one gradient (density + force pass) at the default workload, as measured with rocprofv3
 # (profiles/eam_gradient_config2.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
@@ -162,7 +163,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     # The same mixed batch with every converged saddle turned into a checked min->sp->min mechanism.
     log("search leg: batched_mixed with mechanisms")
     bm = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=fa.MIXED)
-    bm.set_mechanisms()
+    bm.set_mechanisms(minimiser=dict(f2norm=MECH_F2NORM))
     idm = bm.submit(centres, radii, starts, axes)
     t0 = time.perf_counter()
     bm.run()
@@ -172,7 +173,8 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     codes = np.bincount([m["code"] for m in mm if m["code"] >= 0], minlength=8).tolist()
     out["batched_mixed_mechanisms"] = {"seconds": el, "searches_per_s": n_search / el,
                                        "converged": sum(1 for r in rm if r["exit"] == 0), "pathways": codes[0],
-                                       "pathways_per_s": codes[0] / el, "mechanism_codes": codes}
+                                       "pathways_per_s": codes[0] / el, "mechanism_codes": codes,
+                                       "minimiser_f2norm": MECH_F2NORM}
     # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
     log(f"search leg: plain FP64 sample of {plain_sample}")
     dim = fa.Dimer(ctx, box, dimer=dimer)

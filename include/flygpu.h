@@ -260,7 +260,10 @@ typedef struct {
   double nudge_frac;             /* Master::Options::nudge_frac (find.hpp:184), 0.01 */
   double basin_tol;              /* Master::Options::basin_tol (find.hpp:112), 0.05 A */
   double stationary_tol;         /* Master::Options::stationary_tol (find.hpp:120), basin_tol / 2 */
-  flyg_lbfgs_options minimiser;  /* LBFGS::Options of the two relaxations (history length: the dimer's) */
+  flyg_lbfgs_options minimiser;  /* LBFGS::Options of the two relaxations (history length: the dimer's).
+                                    The relaxations use the batch's precision: FP32 pair forces carry
+                                    rounding noise near 1e-5 eV/A, so FLYG_MIXED requires f2norm >= 1e-4
+                                    (the reference default 1e-6 is for FP64) and is refused otherwise. */
 } flyg_mech_options;
 
 typedef struct {

@@ -2246,16 +2246,23 @@ void flyg_mech_default_options(flyg_mech_options* o) {
 }
 
 namespace {
-  void check_mech(flyg_mech_options const* opt) {
+  // Smallest minimiser tolerance |g| the mixed form can reach: its FP32 pair forces carry rounding
+  // noise of order 1e-5 eV/A over a few hundred mobile components.
+  constexpr double MIXED_MIN_F2NORM = 1e-4;
+
+  void check_mech(flyg_mech_options const* opt, bool mixed) {
     require(opt != nullptr, "null argument");
     require(opt->nudge_frac > 0 && opt->basin_tol > 0 && opt->stationary_tol > 0, "mechanism tolerances must be positive");
     require(opt->minimiser.iter_max > 0 && opt->minimiser.f2norm > 0, "bad minimiser options");
+    require(!opt->enabled || !mixed || opt->minimiser.f2norm >= MIXED_MIN_F2NORM,
+            "mixed precision: the mechanism relaxations need minimiser.f2norm >= 1e-4 (FP32 pair forces carry "
+            "rounding noise near 1e-5 eV/A); use FP64 precision for tighter tolerances");
   }
 }  // namespace
 
 int flyg_batch_set_mechanisms(flyg_batch b, const flyg_mech_options* opt) {
   return bguard(b->ctx, [&] {
-    check_mech(opt);
+    check_mech(opt, b->bopt.precision == FLYG_MIXED);
     b->mopt = *opt;
   });
 }
@@ -2445,7 +2452,7 @@ int flyg_server_release(flyg_server s, int64_t id) {
 
 int flyg_server_set_mechanisms(flyg_server s, const flyg_mech_options* opt) {
   return bguard(s->ctx, [&] {
-    check_mech(opt);
+    check_mech(opt, s->sopt.precision == FLYG_MIXED);
     s->mopt = *opt;
   });
 }

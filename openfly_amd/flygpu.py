@@ -582,7 +582,8 @@ class Batch:
         self.ctx.check(lib().flyg_batch_run(self.h))
 
     def set_mechanisms(self, **kw):
-        """Check every converged saddle as a min->sp->min mechanism (Master::do_adj_min / saddle_2_mech)."""
+        """Check every converged saddle as a min->sp->min mechanism (Master::do_adj_min / saddle_2_mech).
+        The mixed form needs minimiser=dict(f2norm=1e-4) or looser (FlyError otherwise)."""
         self.mopt = mech_options(**kw)
         self.ctx.check(lib().flyg_batch_set_mechanisms(self.h, C.byref(self.mopt)))
 

@@ -4,7 +4,8 @@ Bar: for a converged FP64 batched search with mechanisms on, the two minima are 
 the FP64 oracle's energy (|g| < 1e-5 eV/A on the mobile atoms), equal the oracle's LBFGS::minimise from
 the same nudged saddle within 1e-4 A, and the reported barrier / delta equal the oracle's full-system
 energy differences within 1e-6 eV; the classification follows the reference's tolerances; the server
-reports the same mechanisms as the batch API.  Potential: the reference's wen.eam.fs (Fe vacancy hop).
+reports the same mechanisms as the batch API; the mixed form refuses a minimiser tolerance below its
+FP32 force noise.  Potential: the reference's wen.eam.fs (Fe vacancy hop).
 """
 import numpy as np
 import pytest
@@ -136,12 +137,12 @@ def test_server_mechanisms_equal_batch(ctx, setup):
     st = starts(S, 4, 9)
     b = fa.Batch(ctx, S["pot"], fa.Box(S["s"].basis), S["s"].pos, S["s"].types, dimer=dict(f2norm=1e-4),
                  block_threads=256, lanes_per_atom=8)
-    b.set_mechanisms()
+    b.set_mechanisms(minimiser=dict(f2norm=1e-4))
     ids = b.submit([c for c, _, _, _ in st], [RADIUS] * len(st), np.stack([x for _, x, _, _ in st]),
                    np.stack([a for _, _, a, _ in st]))
     b.run()
     srv = fa.Server(ctx, S["pot"], dimer=dict(f2norm=1e-4), block_threads=256, lanes_per_atom=8)
-    srv.set_mechanisms()
+    srv.set_mechanisms(minimiser=dict(f2norm=1e-4))
     sid = srv.add_system(fa.Box(S["s"].basis), S["s"].pos, S["s"].types)
     sids = srv.submit(sid, [c for c, _, _, _ in st], [RADIUS] * len(st), [k for _, _, _, k in st])
     got = {}
@@ -154,5 +155,21 @@ def test_server_mechanisms_equal_batch(ctx, setup):
     for i, j in zip(ids, sids):
         mb = b.mechanism(i)
         r, ms = got[int(j)]
+        print(f"search {i}: exit {r['exit']} mechanism {fa.MECH_NAMES[mb['code']]}", flush=True)
         assert r["exit"] == b.result(i)["exit"]
         assert mb["code"] == ms["code"] and mb["barrier"] == ms["barrier"] and mb["delta"] == ms["delta"]
+        if r["exit"] == 0:
+            assert mb["code"] != 1   # the mixed relaxations reach 1e-4 eV/A
+
+
+def test_mixed_mechanisms_refuse_unreachable_tolerance(ctx, setup):
+    S = setup
+    b = fa.Batch(ctx, S["pot"], fa.Box(S["s"].basis), S["s"].pos, S["s"].types, precision=fa.MIXED)
+    with pytest.raises(fa.FlyError, match="f2norm"):
+        b.set_mechanisms()                      # the reference default 1e-6 is below the FP32 noise
+    b.set_mechanisms(minimiser=dict(f2norm=1e-4))
+    b64 = fa.Batch(ctx, S["pot"], fa.Box(S["s"].basis), S["s"].pos, S["s"].types, precision=fa.FP64)
+    b64.set_mechanisms()
+    srv = fa.Server(ctx, S["pot"])
+    with pytest.raises(fa.FlyError, match="f2norm"):
+        srv.set_mechanisms()
