@@ -283,42 +283,6 @@ namespace flyg {
 
     constexpr int U = 4;
 
-    // Spline segment of x (reference Spline::fetch); `use` is false for entries that do not count.
-    template <bool MIX>
-    struct Seg {
-      using R = typename std::conditional<MIX, float, double>::type;
-      int i;
-      R t;
-      bool use;
-    };
-
-    template <bool MIX>
-    __device__ inline Seg<MIX> seg_of(SplineDesc const& s, typename Seg<MIX>::R x, bool in, unsigned& e) {
-      using R = typename Seg<MIX>::R;
-      R q = x * (MIX ? static_cast<R>(s.inv_dxf) : static_cast<R>(s.inv_dx));
-      bool ok = (q < static_cast<R>(s.count)) && !(q <= static_cast<R>(-1));
-      if (in && !ok) e |= ERR_SPLINE_OOB;
-      Seg<MIX> g;
-      g.use = in && ok;
-      g.i = g.use ? static_cast<int>(q) : 0;
-      g.t = g.use ? x - static_cast<R>(g.i) * (MIX ? static_cast<R>(s.dxf) : static_cast<R>(s.dx)) : static_cast<R>(0);
-      return g;
-    }
-
-    template <bool MIX>
-    __device__ inline auto coef_of(EAMTables const& T, SplineDesc const& s, Seg<MIX> const& g) {
-      if constexpr (MIX) {
-        return T.c32[s.off + g.i];
-      } else {
-        return T.c64[s.off + g.i];
-      }
-    }
-
-    template <typename C, typename R>
-    __device__ inline R poly0(C const& c, R t) { return c.x + t * (c.y + t * (c.z + t * c.w)); }
-    template <typename C, typename R>
-    __device__ inline R poly1(C const& c, R t) { return c.y + t * (static_cast<R>(2) * c.z + t * static_cast<R>(3) * c.w); }
-
     // Geometry of U entries of one row: entry k_q = k0 + q * stride (valid while < c).
     template <bool MIX>
     struct Quad {

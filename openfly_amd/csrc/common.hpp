@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -186,6 +187,45 @@ namespace flyg {
     if (D == 1) return c.y + t * (2.0f * c.z + t * 3.0f * c.w);
     return 2.0f * c.z + 6.0f * t * c.w;
   }
+
+  // Branch-free spline segment of x (reference Spline::fetch, spline.hpp:81-154): `use` is false for
+  // entries that do not count (and for x out of range, which sets ERR_SPLINE_OOB), so a caller can
+  // compute every segment of a group of entries, issue all their coefficient loads, and only then do
+  // the arithmetic; the value is the same as spl64 / spl32f bit for bit.
+  template <bool MIX>
+  struct Seg {
+    using R = typename std::conditional<MIX, float, double>::type;
+    int i;
+    R t;
+    bool use;
+  };
+
+  template <bool MIX>
+  __device__ inline Seg<MIX> seg_of(SplineDesc const& s, typename Seg<MIX>::R x, bool in, unsigned& e) {
+    using R = typename Seg<MIX>::R;
+    R q = x * (MIX ? static_cast<R>(s.inv_dxf) : static_cast<R>(s.inv_dx));
+    bool ok = (q < static_cast<R>(s.count)) && !(q <= static_cast<R>(-1));
+    if (in && !ok) e |= ERR_SPLINE_OOB;
+    Seg<MIX> g;
+    g.use = in && ok;
+    g.i = g.use ? static_cast<int>(q) : 0;
+    g.t = g.use ? x - static_cast<R>(g.i) * (MIX ? static_cast<R>(s.dxf) : static_cast<R>(s.dx)) : static_cast<R>(0);
+    return g;
+  }
+
+  template <bool MIX>
+  __device__ inline auto coef_of(EAMTables const& T, SplineDesc const& s, Seg<MIX> const& g) {
+    if constexpr (MIX) {
+      return T.c32[s.off + g.i];
+    } else {
+      return T.c64[s.off + g.i];
+    }
+  }
+
+  template <typename C, typename R>
+  __device__ inline R poly0(C const& c, R t) { return c.x + t * (c.y + t * (c.z + t * c.w)); }
+  template <typename C, typename R>
+  __device__ inline R poly1(C const& c, R t) { return c.y + t * (static_cast<R>(2) * c.z + t * static_cast<R>(3) * c.w); }
 
   // ----------------------------------------------------------------------------------------------
   // XCD-aware block order.  The dispatcher hands workgroup b to XCD b mod 8, and each XCD has its own

@@ -202,14 +202,15 @@ namespace flyg {
               float rf = r2f * rinv;
               float vp = spl32f<1>(T, VV(vidx(T, ti, tj)), rf, e);
               float pa = spl32f<1>(T, PHI(tj + T.ntypes * ti), rf, e);
-              float pb = MULTI ? spl32f<1>(T, PHI(ti + T.ntypes * tj), rf, e) : pa;
+              // phi_{g->a} is phi_{a->g} for a same-species pair: no second table load.
+              float pb = MULTI && ti != tj ? spl32f<1>(T, PHI(ti + T.ntypes * tj), rf, e) : pa;
               float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(p.w) * pb;
               cf = static_cast<double>(mag * rinv);
             } else {
               double r = sqrt(p.r2);
               double vp = spl64<1>(T, VV(vidx(T, ti, tj)), r, e);
               double pa = spl64<1>(T, PHI(tj + T.ntypes * ti), r, e);
-              double pb = MULTI ? spl64<1>(T, PHI(ti + T.ntypes * tj), r, e) : pa;
+              double pb = MULTI && ti != tj ? spl64<1>(T, PHI(ti + T.ntypes * tj), r, e) : pa;
               cf = (vp + pi.w * pa + p.w * pb) / r;
             }
             gx.add_slot(q, -cf * p.dx);
