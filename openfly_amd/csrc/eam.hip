@@ -42,6 +42,18 @@ namespace flyg {
 
     __device__ inline int vidx(EAMTables const& T, int a, int b) { return a > b ? a + T.ntypes * b : b + T.ntypes * a; }
 
+    // Mixed-form pass-1 output: the force pass uses F' only in FP32, so the w slot of (x, y, z, w) carries
+    // F' as a float in its low 32 bits and the species in its high 32 bits, and a neighbour's species
+    // arrives with its position (one gather less per pair).
+    __device__ inline double pack_fp_type(float fp, int t) {
+      return __longlong_as_double((static_cast<long long>(t) << 32) |
+                                  static_cast<long long>(static_cast<unsigned>(__float_as_int(fp))));
+    }
+    __device__ inline float packed_fp(double w) {
+      return __int_as_float(static_cast<int>(static_cast<unsigned>(__double_as_longlong(w) & 0xffffffffll)));
+    }
+    __device__ inline int packed_type(double w) { return static_cast<int>(__double_as_longlong(w) >> 32); }
+
     // One neighbour entry: image position difference and squared distance.
     struct PairGeo {
       double dx, dy, dz, r2, w;
@@ -148,7 +160,8 @@ namespace flyg {
       if (act && lane == 0) {
         double f, fp;
         spl64_ffp(T, T.F[ti], rt, f, fp, e);
-        posf[i] = make_double4(pi.x, pi.y, pi.z, fp);
+        double const w = MIX && !ENERGY ? pack_fp_type(static_cast<float>(fp), ti) : fp;
+        posf[i] = make_double4(pi.x, pi.y, pi.z, w);
         if (ENERGY) eatom[i] = 0.5 * vt + f;
       }
       if (e) atomicOr(err, e);
@@ -194,7 +207,7 @@ namespace flyg {
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, posf, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[p.j]) : A.t0;
+            int tj = MULTI ? (MIX ? packed_type(p.w) : static_cast<int>(A.types[p.j])) : A.t0;
             double cf;
             if (MIX) {
               float r2f = static_cast<float>(p.r2);
@@ -204,7 +217,7 @@ namespace flyg {
               float pa = spl32f<1>(T, PHI(tj + T.ntypes * ti), rf, e);
               // phi_{g->a} is phi_{a->g} for a same-species pair: no second table load.
               float pb = MULTI && ti != tj ? spl32f<1>(T, PHI(ti + T.ntypes * tj), rf, e) : pa;
-              float mag = vp + static_cast<float>(pi.w) * pa + static_cast<float>(p.w) * pb;
+              float mag = vp + packed_fp(pi.w) * pa + packed_fp(p.w) * pb;
               cf = static_cast<double>(mag * rinv);
             } else {
               double r = sqrt(p.r2);
