@@ -87,6 +87,12 @@ int flyg_eam_set_precision(flyg_eam eam, int precision);
 /* Lanes of a wavefront that share one atom's neighbour row in the gradient kernels: 4, 8 or 16, or 0
  * (default) to time the three once per system size.  Results are bitwise identical for every choice. */
 int flyg_eam_set_lanes(flyg_eam eam, int lanes);
+/* Kernel timing of gradient calls (no reference counterpart; for benchmarks): while on, each gradient
+ * call records a HIP event pair on the context's stream around its density + force kernels.
+ * set_timing(1) also clears the record; flyg_eam_timing synchronises the stream and returns the summed
+ * event time (ms) and the number of timed calls since. */
+int flyg_eam_set_timing(flyg_eam eam, int on);
+int flyg_eam_timing(flyg_eam eam, double* ms_total, int64_t* calls);
 /* Evaluates one table on the device at m host points: kind 0 = F_a, 1 = phi(a,b), 2 = V(a,b);
  * deriv 0/1/2 = f/f'/f'' (DataEAM::f/phi/v + Spline::f/fp/fpp, data.hpp:73-116, spline.hpp:81-123). */
 int flyg_eam_table_eval(flyg_eam eam, int kind, int a, int b, const double* x, int64_t m, int deriv, double* out);

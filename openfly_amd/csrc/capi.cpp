@@ -189,6 +189,28 @@ int flyg_eam_set_lanes(flyg_eam eam, int lanes) {
   });
 }
 
+int flyg_eam_set_timing(flyg_eam eam, int on) {
+  return guard(eam->ctx, [&] {
+    eam->timing = on != 0;
+    eam->events_used = 0;
+  });
+}
+
+int flyg_eam_timing(flyg_eam eam, double* ms_total, int64_t* calls) {
+  return guard(eam->ctx, [&] {
+    require(ms_total && calls, "null argument");
+    HIPCHK(hipStreamSynchronize(eam->ctx->stream));
+    double t = 0;
+    for (std::size_t k = 0; k < eam->events_used; ++k) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, eam->events[k].first, eam->events[k].second));
+      t += ms;
+    }
+    *ms_total = t;
+    *calls = static_cast<int64_t>(eam->events_used);
+  });
+}
+
 int flyg_eam_table_eval(flyg_eam eam, int kind, int a, int b, const double* x, int64_t m, int deriv, double* out) {
   return guard(eam->ctx, [&] {
     require(kind >= 0 && kind <= 2, "kind must be 0 (F), 1 (phi) or 2 (V)");

@@ -578,7 +578,17 @@ namespace flyg {
     Shifts S = shifts_of(nl.box);
     PairArgs A = pair_args(E, nl, types, frozen);
     int L = tuned_lanes(E, S, A, d_grad, precision);
+    if (E.timing) {
+      if (E.events_used == E.events.size()) {
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        E.events.emplace_back(a, b);
+      }
+      HIPCHK(hipEventRecord(E.events[E.events_used].first, E.ctx->stream));
+    }
     gradient_with(E, S, A, d_grad, precision, L);
+    if (E.timing) HIPCHK(hipEventRecord(E.events[E.events_used++].second, E.ctx->stream));
     HIPCHK(hipGetLastError());
   }
 

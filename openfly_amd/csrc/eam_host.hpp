@@ -68,6 +68,16 @@ struct flyg_eam_s {
   int lanes = 0;                                   // lanes per atom row, 0 = timed choice
   int staged_single = -1;                          // one species in the staged host types, else -1
   std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes
+  // Optional kernel timing of gradient calls: one event pair around each call's kernels.
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  std::size_t events_used = 0;
+  ~flyg_eam_s() {
+    for (auto& e : events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
 };
 
 struct flyg_list_s {

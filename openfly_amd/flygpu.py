@@ -133,6 +133,8 @@ _SIGS = {
     "flyg_eam_info": [_vp, C.POINTER(C.c_int), _d, _d, C.c_char_p],
     "flyg_eam_set_precision": [_vp, C.c_int],
     "flyg_eam_set_lanes": [_vp, C.c_int],
+    "flyg_eam_set_timing": [_vp, C.c_int],
+    "flyg_eam_timing": [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)],
     "flyg_eam_table_eval": [_vp, C.c_int, C.c_int, C.c_int, _d, C.c_int64, C.c_int, _d],
     "flyg_list_create": [_vp, _d, C.POINTER(C.c_int), C.c_double, C.POINTER(_vp)],
     "flyg_list_destroy": [_vp],
@@ -317,6 +319,16 @@ class EAM:
 
     def set_lanes(self, lanes):
         self.ctx.check(lib().flyg_eam_set_lanes(self.h, lanes))
+
+    def set_timing(self, on=True):
+        """Record HIP events around each gradient call's kernels (clears the record)."""
+        self.ctx.check(lib().flyg_eam_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        """(summed kernel ms, timed gradient calls) since set_timing(True); synchronises."""
+        ms, n = C.c_double(), C.c_int64()
+        self.ctx.check(lib().flyg_eam_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def table(self, kind, a, b, x, deriv=0):
         x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
