@@ -191,8 +191,8 @@ int flyg_eam_set_lanes(flyg_eam eam, int lanes) {
 
 int flyg_eam_set_timing(flyg_eam eam, int on) {
   return guard(eam->ctx, [&] {
+    if (on) eam->events_used = 0;   // a new record; turning timing off keeps the last one readable
     eam->timing = on != 0;
-    eam->events_used = 0;
   });
 }
 

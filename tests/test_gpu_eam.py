@@ -160,6 +160,26 @@ def test_gradient_deterministic(ctx):
     assert np.array_equal(g1, g2)
 
 
+def test_gradient_timing_counts_calls(ctx):
+    # flyg_eam_set_timing / flyg_eam_timing: one event pair per gradient call while on; the timed
+    # gradient equals an untimed one bit for bit.
+    pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
+    s = systems.config1(seed=5)
+    nl = fa.List(ctx, fa.Box(s.basis), 5.3)
+    nl.rebuild(s.pos)
+    g0 = pot.gradient(s.types, s.frozen, nl)
+    pot.set_timing(True)
+    assert pot.timing() == (0.0, 0)
+    for _ in range(3):
+        g = pot.gradient(s.types, s.frozen, nl)
+    ms, calls = pot.timing()
+    pot.set_timing(False)
+    pot.gradient(s.types, s.frozen, nl)
+    assert calls == 3 and 0.0 < ms < 1e3
+    assert pot.timing()[1] == 3          # calls while off are not recorded
+    assert np.array_equal(g, g0)
+
+
 def test_spline_out_of_bounds_raises(ctx, wen_gpu):
     # 40 atoms within 2 A: the density exceeds the F table (rho > (Nrho + 4) drho ~ 200), which must raise
     # Spline::OutOfBounds (spline.hpp:141-143); phi_Fe(0.05 A) ~ 185 in wen.eam.fs.
