@@ -31,7 +31,7 @@ LIST_SKIN = 0.6   # Verlet skin of the timed update/rebuild loop (rebuild once t
 MECH_F2NORM = 1e-4   # |g| tolerance of the mixed form's mechanism relaxations (its FP32 force noise is ~1e-5 eV/A)
 
 # HBM traffic of one gradient (density + force pass) at the default workload, as measured with rocprofv3
-# (profiles/eam_gradient_config2.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
+# (profiles/eam_gradient_fe30_x_alloy.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
 # reading for a copy kernel of known size (python bench.py --copy-calibration: a 64 MiB copy reads
 # FETCH_SIZE 32780.9 KiB and WRITE_SIZE 65536 KiB).
 GRADIENT_TRAFFIC = {"fetch_kib": 12744.2 + 13129.4, "write_kib": 1704.0 + 1278.6,
@@ -110,10 +110,10 @@ def relax(ctx, fa, pot, s):
 
 
 def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
-    """Config 3: 256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials
+    """256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials
     (relaxed: min0), active radius U(4, 9) A around defect-adjacent atoms, kick 0.6 A, |g| < 1e-3 eV/A,
     FP32 pair arithmetic / FP64 accumulation, curvature, dots and barriers; potential wen.eam.fs."""
-    s = systems.config3(3)
+    s = systems.fe_defects_20(3)
     pot = fa.EAM(ctx, WEN)
     t0 = time.perf_counter()
     s.pos = relax(ctx, fa, pot, s)
@@ -125,7 +125,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     starts, axes = perturbed_starts(s, centres, radii, seed=34)
     box = fa.Box(s.basis)
     dimer = dict(f2norm=1e-3)
-    out = {"workload": f"bcc Fe 20^3 + 20 vacancies + 40 interstitials (config 3, {s.n} atoms) relaxed to min0, "
+    out = {"workload": f"bcc Fe 20^3 + 20 vacancies + 40 interstitials ({s.n} atoms) relaxed to min0, "
                        f"{n_search} searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, potential wen.eam.fs",
            "relax_seconds": t_relax}
     res = {}
@@ -194,7 +194,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
 
 
 def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
-    """Config 4: Poisson arrivals at 200/s of requests from 12 kMC systems (bcc Fe, N log-uniform in
+    """Serving: Poisson arrivals at 200/s of requests from 12 kMC systems (bcc Fe, N log-uniform in
     [2000, 250000], 0.5-2 % vacancies + interstitials, each relaxed), 8-64 searches each, radii U(4, 9) A,
     served with continuous batching; latency = arrival to last search done."""
     from openfly_amd import serve
@@ -229,13 +229,13 @@ def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
 
 
 def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
-    """Config 5: bcc Fe 80^3 cells (1,028,096 atoms) with 0.1 % vacancies in clusters of 1-6 and 0.5 %
+    """Large system: bcc Fe 80^3 cells (1,028,096 atoms) with 0.1 % vacancies in clusters of 1-6 and 0.5 %
     interstitials, relaxed (min0); 1024 concurrent searches (radius U(4, 9) A, kick 0.6 A, |g| < 1e-3 eV/A)
     in one launch of the autotuned persistent kernel; converged barriers re-evaluated as full-system FP64
     energy differences E(sp) - E(min0) on a sample."""
     from openfly_amd import serve
     t0 = time.perf_counter()
-    s = systems.config5(5)
+    s = systems.fe_clusters_80(5)
     pot = fa.EAM(ctx, WEN)
     s.pos = relax(ctx, fa, pot, s)
     t_setup = time.perf_counter() - t0
@@ -256,16 +256,16 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
                 got[int(d)] = srv.result(d, positions=True, release=True)
         return [got[int(i)] for i in ids]
 
-    log("config 5 leg: autotune on 64 searches")
+    log("1M-atom leg: autotune on 64 searches")
     drain(srv.submit(sysid, centres[:64], radii[:64], kicks[:64]))
-    log(f"config 5 leg: {n_search} searches")
+    log(f"1M-atom leg: {n_search} searches")
     t0 = time.perf_counter()
     res = drain(srv.submit(sysid, centres, radii, kicks))
     el = time.perf_counter() - t0
     st = srv.status()
     conv = [r for r in res if r["exit"] == 0]
     # FP64 re-evaluation of a sample of converged barriers (full system, fresh list each time).
-    log(f"config 5 leg: FP64 re-evaluation of {min(n_check, len(conv))} barriers")
+    log(f"1M-atom leg: FP64 re-evaluation of {min(n_check, len(conv))} barriers")
     nl = fa.List(ctx, box, pot.r_cut())
     frozen = np.zeros(s.n, np.uint8)
     nl.rebuild(s.pos)
@@ -368,7 +368,7 @@ def main():
     import openfly_amd as fa
     from openfly_amd import systems
 
-    s = systems.config2(seed=2, cells=args.cells)
+    s = systems.fe_x_alloy(seed=2, cells=args.cells)
     path = systems.potential_path(("Fe", "X"))
     ctx = fa.Context(0)
     pot = fa.EAM(ctx, path)
@@ -453,8 +453,8 @@ def main():
         "dtype": "f32 pair arithmetic, f64 accumulation" if args.precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"bcc Fe {args.cells}^3 cells + 1% vacancies + 2% X tetrahedral interstitials "
-                        "(config 2): random displacement <= 0.05 A, list update/rebuild, EAM gradient",
+            "workload": f"bcc Fe {args.cells}^3 cells + 1% vacancies + 2% X tetrahedral interstitials, "
+                        "random displacement <= 0.05 A per step, list update/rebuild, EAM gradient",
             "atoms": s.n,
             "mean_neighbours": float(counts.mean()),
             "rebuilds_in_timed_steps": state["rebuilds"],
@@ -486,9 +486,9 @@ def main():
         out["serving"] = serve_bench(ctx, fa, systems)
         log("serving leg done")
     if not args.no_large:
-        log("config 5 leg: building and relaxing the 1M-atom system")
+        log("1M-atom leg: building and relaxing the system")
         out["large"] = large_bench(ctx, fa, systems)
-        log("config 5 leg done")
+        log("1M-atom leg done")
     if not args.no_cpu:
         log("gradient leg: CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(s, path)

@@ -153,7 +153,7 @@ class System:
         return d - L * np.round(d / L)
 
 
-def config1(seed=1):
+def fe_vacancy_16(seed=1):
     """bcc Fe a=2.855 A, 16^3 cells (8192 sites) minus one random site, Gaussian sigma=0.05 A noise."""
     rng = np.random.default_rng(seed)
     pos = bcc_sites(A_FE, 16)
@@ -165,7 +165,7 @@ def config1(seed=1):
     return s
 
 
-def config2(seed=2, cells=30):
+def fe_x_alloy(seed=2, cells=30):
     """bcc Fe 30^3 cells (54000 sites), 1% vacancies, species X at 2 per 100 Fe on random tetrahedral
     sites, thermal Gaussian sigma=0.08 A noise."""
     rng = np.random.default_rng(seed)
@@ -216,12 +216,12 @@ def defect_system(cells, n_vac, n_int, seed, clusters=False):
     return s
 
 
-def config3(seed=3):
+def fe_defects_20(seed=3):
     """bcc Fe 20^3 cells (16000 sites) with 20 vacancies and 40 interstitials."""
     return defect_system(20, 20, 40, seed)
 
 
-def config5(seed=5):
+def fe_clusters_80(seed=5):
     """bcc Fe 80^3 cells (1,024,000 sites), 0.1% vacancies in clusters of 1-6, 0.5% interstitials."""
     n_sites = 2 * 80**3
     return defect_system(80, int(round(0.001 * n_sites)), int(round(0.005 * n_sites)), seed, clusters=True)

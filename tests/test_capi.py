@@ -55,12 +55,12 @@ def test_setfl_writer_roundtrip():
 
 def test_config_generators():
     import numpy as np
-    c1 = systems.config1(1)
+    c1 = systems.fe_vacancy_16(1)
     assert c1.n == 8191 and abs(c1.basis[0, 0] - 16 * 2.855) < 1e-12
-    c2 = systems.config2(2, cells=10)
+    c2 = systems.fe_x_alloy(2, cells=10)
     assert c2.n == 2000 - 20 + int(round(0.02 * 1980))
     assert set(np.unique(c2.types)) == {0, 1}
-    c3 = systems.config3(3)
+    c3 = systems.fe_defects_20(3)
     assert c3.n == 16000 - 20 + 40
     assert len(systems.defect_adjacent_atoms(c3)) > 100
     tet = systems.tetra_sites(1.0, 1)

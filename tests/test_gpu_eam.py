@@ -113,12 +113,12 @@ def test_hvp_vs_oracle_dense_hessian(ctx, wen_gpu, wen_orc):
     assert np.max(np.abs(hv - ho)) <= 1e-10 * np.max(np.abs(ho))
 
 
-def test_synthetic_fs_config1_vs_oracle(ctx):
+def test_synthetic_fs_vacancy16_vs_oracle(ctx):
     path = systems.potential_path(("Fe",))
     pot = fa.EAM(ctx, path)
     data = po.DataEAM(path)
-    s = systems.config1(seed=1)
-    print(f"case config1 n={s.n}", flush=True)
+    s = systems.fe_vacancy_16(seed=1)
+    print(f"case fe_vacancy_16 n={s.n}", flush=True)
     rc = 5.3 + 2e-3 + (1.1 ** (1 / 3) - 1) * (5.3 + 2e-3)
     _, E, g = gpu_eval(ctx, pot, s.pos, s.basis, s.types, s.frozen, rc)
     _, Eo, go = orc_eval(data, s.pos, s.basis, s.types, s.frozen, rc)
@@ -136,12 +136,12 @@ def test_synthetic_fs_config1_vs_oracle(ctx):
     pot.set_precision(fa.FP64)
 
 
-def test_config2_two_species_scale(ctx):
+def test_fe_x_alloy_two_species_scale(ctx):
     path = systems.potential_path(("Fe", "X"))
     pot = fa.EAM(ctx, path)
     data = po.DataEAM(path)
-    s = systems.config2(seed=2, cells=16)
-    print(f"case config2(16^3) n={s.n}", flush=True)
+    s = systems.fe_x_alloy(seed=2, cells=16)
+    print(f"case fe_x_alloy(16^3) n={s.n}", flush=True)
     _, E, g = gpu_eval(ctx, pot, s.pos, s.basis, s.types, s.frozen)
     _, Eo, go = orc_eval(data, s.pos, s.basis, s.types, s.frozen)
     assert abs(E - Eo) <= 1e-10 * abs(Eo)
@@ -151,7 +151,7 @@ def test_config2_two_species_scale(ctx):
 
 def test_gradient_deterministic(ctx):
     pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
-    s = systems.config1(seed=4)
+    s = systems.fe_vacancy_16(seed=4)
     nl = fa.List(ctx, fa.Box(s.basis), 5.3)
     nl.rebuild(s.pos)
     g1 = pot.gradient(s.types, s.frozen, nl)
@@ -164,7 +164,7 @@ def test_gradient_timing_counts_calls(ctx):
     # flyg_eam_set_timing / flyg_eam_timing: one event pair per gradient call while on; the timed
     # gradient equals an untimed one bit for bit.
     pot = fa.EAM(ctx, systems.potential_path(("Fe",)))
-    s = systems.config1(seed=5)
+    s = systems.fe_vacancy_16(seed=5)
     nl = fa.List(ctx, fa.Box(s.basis), 5.3)
     nl.rebuild(s.pos)
     g0 = pot.gradient(s.types, s.frozen, nl)
@@ -209,7 +209,7 @@ def test_gradient_independent_of_lanes(ctx, precision):
     # Row sums use a fixed 16-slot tree, so 4, 8 and 16 lanes per atom give bitwise identical gradients.
     pot = fa.EAM(ctx, systems.potential_path(("Fe", "X")))
     pot.set_precision(precision)
-    s = systems.config2(seed=5, cells=10)
+    s = systems.fe_x_alloy(seed=5, cells=10)
     nl = fa.List(ctx, fa.Box(s.basis), 5.3)
     nl.rebuild(s.pos)
     out = []

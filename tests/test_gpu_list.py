@@ -39,9 +39,9 @@ def test_random_boxes_equal_pair_sets(ctx, seed):
 
 
 def test_bcc_config_pair_sets_and_counts(ctx):
-    s = systems.config1(seed=11)
+    s = systems.fe_vacancy_16(seed=11)
     rc = 5.3 + 2e-3 + (1.1 ** (1 / 3) - 1) * (5.3 + 2e-3)   # dimer list cut-off
-    print(f"case config1 n={s.n}", flush=True)
+    print(f"case fe_vacancy_16 n={s.n}", flush=True)
     gnl, (go, gj, gs, gdr) = gpu_pairs(ctx, s.basis, s.periodic, s.pos, rc)
     onl, (oo, oj, os_, odr) = oracle_pairs(s.basis, s.periodic, s.pos, rc)
     assert np.array_equal(go, oo)
@@ -86,7 +86,7 @@ def test_update_matches_brute_force(ctx):
 
 
 def test_positions_are_canonical(ctx):
-    s = systems.config1(seed=2)
+    s = systems.fe_vacancy_16(seed=2)
     shifted = s.pos + np.array([100.0, -45.7, 3.0])
     nl = fa.List(ctx, fa.Box(s.basis), 5.5)
     nl.rebuild(shifted)
