@@ -121,7 +121,7 @@ namespace flyg {
 
     template <int T>
     struct Team {
-      double* red;   // LDS, 4 * T
+      double* red;   // LDS, RED_LDS doubles: two 64-double reduction buffers, then top-2 partials
       int tid;
 
       __device__ __forceinline__ double wave(double v) const {
@@ -329,13 +329,15 @@ namespace flyg {
       }
     }
 
+    constexpr int RED_LDS = 256;   // the workgroup reductions use 160 doubles of it at most
+
     // Region rows staged in LDS per workgroup size (96 KB at 1024 threads: one workgroup per CU anyway).
     template <int T>
-    __host__ __device__ constexpr int lds_rows() { return T >= 1024 ? 3072 : (T >= 512 ? 1536 : 1024); }
+    __host__ __device__ constexpr int lds_rows() { return T >= 1024 ? 3072 : (T >= 512 ? 1792 : 1024); }
     // LDS doubles for the rotor's workspace vectors (the rest of the 160 KB after the reduction buffer
     // and the region rows).
     template <int T>
-    __host__ __device__ constexpr int lds_ws() { return T >= 1024 ? 3000 : (T >= 512 ? 11000 : 14000); }
+    __host__ __device__ constexpr int lds_ws() { return T >= 1024 ? 6800 : (T >= 512 ? 11750 : 14700); }
 
     // Rebuilds the active lists from the candidate rows at the current region positions (stable:
     // an active row is the ordered subsequence of its candidate row, whatever the workgroup shape).
@@ -1123,7 +1125,7 @@ namespace flyg {
 
     template <int T, int L, bool MIX, bool MULTI>
     __global__ __launch_bounds__(T) void k_search(BatchArgs A) {
-      __shared__ double red[4 * T];
+      __shared__ double red[RED_LDS];
       __shared__ double lds_core[128];
       __shared__ int s_id;
       __shared__ double3 lds_S[27];
