@@ -178,20 +178,23 @@ namespace flyg {
     }
 
     // make_ghosts with a fixed capacity; the counter keeps counting past it so the host can resize.
+    // Slots are taken with one atomic per wavefront (a prefix sum of the lanes' image counts): one per
+    // atom on the single counter serialises the kernel.  (Slot order is free: the rank placement sorts
+    // every cell by packed id afterwards.)  Launched with whole wavefronts.
     __global__ void k_ghosts_cap(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
                                  unsigned* __restrict__ eid, unsigned* __restrict__ st, long gcap) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (i >= n) return;
+      bool const inr = i < n;
       double cx[27][3];
       int cs[27];
       int m = 1;
-      double4 p = pos[i];
+      double4 p = inr ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
       cx[0][0] = p.x;
       cx[0][1] = p.y;
       cx[0][2] = p.z;
       cs[0] = 13;
       int const pow3[3] = {1, 3, 9};
-      for (int ax = 0; ax < 3; ++ax) {
+      for (int ax = 0; ax < 3 && inr; ++ax) {
         if (!b.per[ax]) continue;
         int const end = m;
         for (int sgn = 1; sgn >= -1; sgn -= 2) {
@@ -207,9 +210,20 @@ namespace flyg {
           }
         }
       }
-      unsigned g = static_cast<unsigned>(m - 1);
+      unsigned const g = static_cast<unsigned>(m - 1);
+      int const wl = static_cast<int>(threadIdx.x & 63);
+      unsigned incl = g;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        unsigned t = __shfl_up(incl, o, 64);
+        if (wl >= o) incl += t;
+      }
+      unsigned const total = __shfl(incl, 63, 64);
+      unsigned wbase = 0;
+      if (wl == 63 && total > 0) wbase = atomicAdd(&st[0], total);
+      wbase = __shfl(wbase, 63, 64);
       if (g == 0) return;
-      unsigned base = atomicAdd(&st[0], g);
+      unsigned const base = wbase + (incl - g);
       for (int k = 1; k < m; ++k) {
         long slot = static_cast<long>(base) + (k - 1);
         if (slot >= gcap) {
@@ -325,7 +339,9 @@ namespace flyg {
         }
       if (lane == 0) {
         ncnt[i] = cnt;
-        atomicMax(&st[1], static_cast<unsigned>(cnt));
+        // Only overflowing rows report their length (the host widens the rows to the longest one): an
+        // atomic per atom on the one counter serialised the kernel (half of its time).
+        if (cnt > K) atomicMax(&st[1], static_cast<unsigned>(cnt));
       }
     }
 
