@@ -5,7 +5,8 @@
 // packed entries), each lane striding the row by L; partial sums are combined with cross-lane
 // shuffles in a fixed order, so results are bitwise reproducible run to run.  No atomics touch
 // floating-point data.  Positions are double4 (x, y, z, w): a neighbour gather is one 32-byte load,
-// and pass 1 writes (x, y, z, F'(rho)) so pass 2 gets the neighbour's position and F' in that load.
+// and pass 1 writes (x, y, z, F'(rho)) so pass 2 gets the neighbour's position and F' in that load (in
+// the mixed form w holds F' as FP32 and the species, see pack_fp_type).
 //
 //   pass 1 (density):  rho_b = sum_a phi_{t_a->t_b}(r_ab);  F'_b = F_{t_b}'(rho_b)     (eam.cpp:62-86)
 //   pass 2 (force):    dE/dr_g = -sum_a [V'(r) + F'_g phi'_{a->g}(r) + F'_a phi'_{g->a}(r)] (r_a - r_g)/r
