@@ -133,8 +133,11 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
         log(f"search leg: {name} (autotune run + timed run of {n_search} searches)")
         b = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
         ids = b.submit(centres, radii, starts, axes)
-        b.run()   # first run includes the shape autotuner
-        b2 = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
+        t0 = time.perf_counter()
+        b.run()   # untimed: the first run of a batch includes the shape autotuner
+        first = time.perf_counter() - t0
+        T, L = b.shape()
+        b2 = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec, block_threads=T, lanes_per_atom=L)
         ids2 = b2.submit(centres, radii, starts, axes)
         t0 = time.perf_counter()
         b2.run()
@@ -143,7 +146,8 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
         res[name] = r
         ok = sum(1 for x in r if x["exit"] == 0)
         ms = np.array([x["device_ms"] for x in r])
-        out[name] = {"seconds": el, "converged": ok, "searches_per_s": n_search / el,
+        out[name] = {"seconds": el, "first_run_seconds_with_autotune": first, "shape": [T, L],
+                     "converged": ok, "searches_per_s": n_search / el,
                      "converged_per_s": ok / el, "exits": np.bincount([x["exit"] for x in r], minlength=7).tolist(),
                      "mean_grads": float(np.mean([x["grads"] for x in r])),
                      "mean_mobile": float(np.mean([x["n_mobile"] for x in r])),
@@ -162,7 +166,8 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     out["batched_mixed"]["max_abs_dE_vs_fp64"] = float(max(agree)) if agree else None
     # The same mixed batch with every converged saddle turned into a checked min->sp->min mechanism.
     log("search leg: batched_mixed with mechanisms")
-    bm = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=fa.MIXED)
+    T, L = out["batched_mixed"]["shape"]
+    bm = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=fa.MIXED, block_threads=T, lanes_per_atom=L)
     bm.set_mechanisms(minimiser=dict(f2norm=MECH_F2NORM))
     idm = bm.submit(centres, radii, starts, axes)
     t0 = time.perf_counter()

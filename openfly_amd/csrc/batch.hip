@@ -2251,6 +2251,14 @@ int flyg_batch_mechanism(flyg_batch b, int64_t id, flyg_mechanism* mech, double*
   });
 }
 
+int flyg_batch_shape(flyg_batch b, int* block_threads, int* lanes_per_atom) {
+  return bguard(b->ctx, [&] {
+    require(block_threads && lanes_per_atom, "null argument");
+    *block_threads = b->tuned_T;
+    *lanes_per_atom = b->tuned_L;
+  });
+}
+
 int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region) {
   return bguard(b->ctx, [&] {
     require(id >= 0 && id < static_cast<int64_t>(b->results.size()), "unknown search id");

@@ -172,6 +172,7 @@ _SIGS = {
     "flyg_batch_run": [_vp],
     "flyg_batch_result": [_vp, C.c_int64, C.POINTER(SearchStats), _d, _d],
     "flyg_batch_region": [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "flyg_batch_shape": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "flyg_mech_default_options": [C.POINTER(MechOptions)],
     "flyg_batch_set_mechanisms": [_vp, C.POINTER(MechOptions)],
     "flyg_batch_mechanism": [_vp, C.c_int64, C.POINTER(Mechanism), _d, _d],
@@ -592,6 +593,12 @@ class Batch:
 
     def run(self):
         self.ctx.check(lib().flyg_batch_run(self.h))
+
+    def shape(self):
+        """(block_threads, lanes_per_atom) the batch ran with; (0, 0) before its first run."""
+        t, l = C.c_int(), C.c_int()
+        self.ctx.check(lib().flyg_batch_shape(self.h, C.byref(t), C.byref(l)))
+        return t.value, l.value
 
     def set_mechanisms(self, **kw):
         """Check every converged saddle as a min->sp->min mechanism (Master::do_adj_min / saddle_2_mech).

@@ -145,3 +145,27 @@ def test_results_independent_of_kernel_shape(ctx, setup):
         for a, m in zip(outs[0], other):
             assert a["exit"] == m["exit"] and a["steps"] == m["steps"] and a["barrier"] == m["barrier"]
             assert np.array_equal(a["pos"], m["pos"])
+
+
+def test_tuned_shape_is_reported_and_reusable(ctx, setup):
+    # flyg_batch_shape: (0, 0) before the first run, then the autotuned (block threads, lanes per atom);
+    # a batch given that shape skips the tuning and reproduces the results bit for bit.
+    S = setup
+    st = starts(S, 3, seed0=61)
+    args = ([c for c, _, _, _ in st], [RADIUS] * len(st), np.stack([x for _, _, x, _ in st]),
+            np.stack([a for _, _, _, a in st]))
+    b = fa.Batch(ctx, S["pot"], fa.Box(S["basis"]), S["mn"], S["t"], dimer=dict(f2norm=1e-3), precision=fa.MIXED)
+    assert b.shape() == (0, 0)
+    ids = b.submit(*args)
+    b.run()
+    T, L = b.shape()
+    print(f"tuned shape {T}x{L}", flush=True)
+    assert T in (256, 512, 1024) and L in (8, 16)
+    b2 = fa.Batch(ctx, S["pot"], fa.Box(S["basis"]), S["mn"], S["t"], dimer=dict(f2norm=1e-3), precision=fa.MIXED,
+                  block_threads=T, lanes_per_atom=L)
+    ids2 = b2.submit(*args)
+    b2.run()
+    assert b2.shape() == (T, L)
+    for i, j in zip(ids, ids2):
+        r1, r2 = b.result(i), b2.result(j)
+        assert r1["exit"] == r2["exit"] and r1["steps"] == r2["steps"] and r1["barrier"] == r2["barrier"]
