@@ -177,18 +177,82 @@ namespace flyg {
       eid[i] = (13u << NB_SHIFT_BITS) | static_cast<unsigned>(i);
     }
 
+    // Slots for g images per lane, one atomic per wavefront (a prefix sum of the lanes' counts): one per
+    // atom on the single counter serialises the kernel.  Slot order is free: the rank placement sorts
+    // every cell by packed id afterwards.  Every lane of the wavefront must call it.
+    __device__ inline unsigned ghost_slots(unsigned g, unsigned* st) {
+      int const wl = static_cast<int>(threadIdx.x & 63);
+      unsigned incl = g;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        unsigned t = __shfl_up(incl, o, 64);
+        if (wl >= o) incl += t;
+      }
+      unsigned const total = __shfl(incl, 63, 64);
+      unsigned wbase = 0;
+      if (wl == 63 && total > 0) wbase = atomicAdd(&st[0], total);
+      wbase = __shfl(wbase, 63, 64);
+      return wbase + (incl - g);
+    }
+
     // make_ghosts with a fixed capacity; the counter keeps counting past it so the host can resize.
-    // Slots are taken with one atomic per wavefront (a prefix sum of the lanes' image counts): one per
-    // atom on the single counter serialises the kernel.  (Slot order is free: the rank placement sorts
-    // every cell by packed id afterwards.)  Launched with whole wavefronts.
+    // Launched with whole wavefronts.  Orthorhombic boxes: whether an axis contributes a +1 / -1 image
+    // depends only on the atom's own coordinate on that axis, so the images are the product of the
+    // per-axis choices (the same set, and the same coordinates, as the reference's chain) and live in
+    // registers; other boxes follow the chain of gen_image calls.
     __global__ void k_ghosts_cap(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
                                  unsigned* __restrict__ eid, unsigned* __restrict__ st, long gcap) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       bool const inr = i < n;
+      double4 p = inr ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+      if (b.ortho) {
+        double const x[3] = {p.x, p.y, p.z};
+        bool up[3], dn[3];
+        unsigned g = 1;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          up[ax] = inr && b.per[ax] && x[ax] < rc;
+          dn[ax] = inr && b.per[ax] && x[ax] > b.ext[ax] - rc;
+          g *= 1u + (up[ax] ? 1u : 0u) + (dn[ax] ? 1u : 0u);
+        }
+        g -= 1u;
+        unsigned const base = ghost_slots(g, st);
+        if (g == 0) return;
+        unsigned k = 0;
+#pragma unroll
+        for (int s2 = -1; s2 <= 1; ++s2)
+#pragma unroll
+          for (int s1 = -1; s1 <= 1; ++s1)
+#pragma unroll
+            for (int s0 = -1; s0 <= 1; ++s0) {
+              if (s0 == 0 && s1 == 0 && s2 == 0) continue;
+              bool const use = (s0 == 0 || (s0 > 0 ? up[0] : dn[0])) && (s1 == 0 || (s1 > 0 ? up[1] : dn[1])) &&
+                               (s2 == 0 || (s2 > 0 ? up[2] : dn[2]));
+              if (!use) continue;
+              long const slot = static_cast<long>(base) + k;
+              ++k;
+              if (slot >= gcap) {
+                atomicOr(&st[3], static_cast<unsigned>(OV_GHOST));
+                return;
+              }
+              double im[3] = {x[0], x[1], x[2]};
+              {
+#pragma clang fp contract(off)
+                if (s0 > 0) im[0] += b.ext[0];
+                if (s0 < 0) im[0] -= b.ext[0];
+                if (s1 > 0) im[1] += b.ext[1];
+                if (s1 < 0) im[1] -= b.ext[1];
+                if (s2 > 0) im[2] += b.ext[2];
+                if (s2 < 0) im[2] -= b.ext[2];
+              }
+              ent[n + slot] = make_double4(im[0], im[1], im[2], 0.0);
+              eid[n + slot] = (static_cast<unsigned>(shift_code(s0, s1, s2)) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+            }
+        return;
+      }
       double cx[27][3];
       int cs[27];
       int m = 1;
-      double4 p = inr ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
       cx[0][0] = p.x;
       cx[0][1] = p.y;
       cx[0][2] = p.z;
@@ -211,19 +275,8 @@ namespace flyg {
         }
       }
       unsigned const g = static_cast<unsigned>(m - 1);
-      int const wl = static_cast<int>(threadIdx.x & 63);
-      unsigned incl = g;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        unsigned t = __shfl_up(incl, o, 64);
-        if (wl >= o) incl += t;
-      }
-      unsigned const total = __shfl(incl, 63, 64);
-      unsigned wbase = 0;
-      if (wl == 63 && total > 0) wbase = atomicAdd(&st[0], total);
-      wbase = __shfl(wbase, 63, 64);
+      unsigned const base = ghost_slots(g, st);
       if (g == 0) return;
-      unsigned const base = wbase + (incl - g);
       for (int k = 1; k < m; ++k) {
         long slot = static_cast<long>(base) + (k - 1);
         if (slot >= gcap) {
