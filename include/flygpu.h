@@ -252,9 +252,9 @@ int flyg_batch_run(flyg_batch b);
 int flyg_batch_result(flyg_batch b, int64_t id, flyg_search_stats* stats, double* pos, double* axis);
 /* Mobile / region atom counts of search `id`. */
 int flyg_batch_region(flyg_batch b, int64_t id, int* n_mobile, int* n_region);
-/* Workgroup shape the batch ran with (autotuned on its first run unless set in its options; 0 before
- * that): a later batch of the same kind can take it as block_threads / lanes_per_atom and skip the
- * tuning.  No reference counterpart. */
+/* Workgroup shape the batch ran with (0 before its first run): set in its options, else taken from an
+ * earlier batch over the same potential, system size and precision, else autotuned on the first run.
+ * No reference counterpart. */
 int flyg_batch_shape(flyg_batch b, int* block_threads, int* lanes_per_atom);
 
 /* ---------------------------------------------------------------------------------------------------

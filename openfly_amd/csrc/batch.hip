@@ -23,8 +23,10 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 
 #include "plain.hpp"
@@ -2128,7 +2130,24 @@ int flyg_batch_run(flyg_batch b) {
       b->tuned_T = b->bopt.block_threads;
       b->tuned_L = b->bopt.lanes_per_atom;
     } else if (!b->tuned_T) {
-      autotune_shape(*b->eam, b->flight, b->pending, d, mix, slots, b->tuned_T, b->tuned_L);
+      // Batches over the same potential, system size and precision reuse the first one's tuning (the
+      // shape changes no result, only the time).
+      static std::mutex mu;
+      static std::map<std::tuple<flyg_eam_s const*, long, bool>, std::pair<int, int>> shapes;
+      auto const key = std::make_tuple(static_cast<flyg_eam_s const*>(b->eam), static_cast<long>(b->sys->n), mix);
+      {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = shapes.find(key);
+        if (it != shapes.end()) {
+          b->tuned_T = it->second.first;
+          b->tuned_L = it->second.second;
+        }
+      }
+      if (!b->tuned_T) {
+        autotune_shape(*b->eam, b->flight, b->pending, d, mix, slots, b->tuned_T, b->tuned_L);
+        std::lock_guard<std::mutex> lock(mu);
+        shapes[key] = {b->tuned_T, b->tuned_L};
+      }
     }
     // All queued searches go out in one launch; a search that leaves its region is relaunched with
     // larger skins on another stream as soon as its flag shows it finished, overlapping the rest.
