@@ -365,8 +365,8 @@ def main():
         print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every box of this build has a "
-              "single MI355X and the path is not sharded across GPUs.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the saddle-search path runs on a "
+              "single MI355X and is not sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
 
     import torch
