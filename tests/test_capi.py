@@ -53,7 +53,7 @@ def test_setfl_writer_roundtrip():
     assert np.allclose(d.eval("F", 0, 0, rho), -np.sqrt(rho), rtol=1e-6)
 
 
-def test_config_generators():
+def test_workload_generators():
     import numpy as np
     c1 = systems.fe_vacancy_16(1)
     assert c1.n == 8191 and abs(c1.basis[0, 0] - 16 * 2.855) < 1e-12
@@ -65,3 +65,12 @@ def test_config_generators():
     assert len(systems.defect_adjacent_atoms(c3)) > 100
     tet = systems.tetra_sites(1.0, 1)
     assert len(np.unique(np.round(tet, 6), axis=0)) == 12
+
+
+def test_bench_refuses_several_gpus():
+    # bench.py measures one GPU: --gpus above 1 exits with a message before touching a device.
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
