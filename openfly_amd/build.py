@@ -1,6 +1,6 @@
 """Builds libflygpu.so (HIP for gfx950) in-tree, next to this file.
 
-    python -m openfly_amd.build            # or: python openfly_amd/build.py
+    python -m openfly_amd.build            # or: python openfly_amd/build.py [--incremental]
 """
 from __future__ import annotations
 
@@ -33,11 +33,21 @@ FLAGS = [
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    # Largest translation units first: the search-kernel variants dominate the build time.
+    srcs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp"))]
+    return sorted(srcs, key=lambda f: (not os.path.basename(f).startswith("search_kernel"), f))
 
 
-def _compile(src):
+def _headers_mtime():
+    return max(os.path.getmtime(os.path.join(CSRC, f)) for f in os.listdir(CSRC) if f.endswith(".hpp"))
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj) and \
+            os.path.getmtime(obj) > max(os.path.getmtime(src), _headers_mtime(),
+                                        os.path.getmtime(os.path.join(HERE, "..", "include", "flygpu.h"))):
+        return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -45,11 +55,12 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, incremental=False):
+    """Compiles every source (incremental=True: only those older than their object or a header)."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,5 +73,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)
