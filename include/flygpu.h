@@ -40,12 +40,19 @@ extern "C" {
 #define FLYG_FP64 0  /* plain form: everything in FP64 */
 #define FLYG_MIXED 1 /* FP32 per-pair spline/force arithmetic, FP64 accumulation and FP64 energies */
 
-/* Dimer exit codes (reference Dimer::Exit, dimer.hpp:123-129). */
+/* Dimer exit codes.  The first five are the reference's Dimer::Exit (dimer.hpp:123-129) with the sign
+ * dropped (reference value = -code: success 0, convex -1, iter_max -2, collision -3, uninit -4), so a code
+ * can index a histogram; the C++ facade's Dimer::Exit carries the reference's own values.  The last two
+ * occur only in batched searches. */
 #define FLYG_EXIT_SUCCESS 0
 #define FLYG_EXIT_CONVEX 1
 #define FLYG_EXIT_ITER_MAX 2
 #define FLYG_EXIT_COLLISION 3
 #define FLYG_EXIT_UNINIT 4
+#define FLYG_EXIT_REGION 5 /* left the rows of its active region; the library re-runs it with larger skins, so
+                              a finished search reports it only if the region outgrew half the box */
+#define FLYG_EXIT_OOB 6    /* a spline was evaluated outside its table (Spline::OutOfBounds, caught by
+                              Master::find_batch, find.cpp:718-723) after the allowed re-perturbations */
 
 typedef struct flyg_ctx_s* flyg_ctx;
 typedef struct flyg_eam_s* flyg_eam;

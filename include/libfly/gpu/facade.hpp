@@ -604,7 +604,8 @@ namespace fly {
         bool debug = false;
       };
 
-      enum Exit : int { success = 0, convex = 1, iter_max = 2, collision = 3, uninit = 4 };
+      // The reference's values (dimer.hpp:123-129); the C-ABI reports them without the sign.
+      enum Exit : int { success = 0, convex = -1, iter_max = -2, collision = -3, uninit = -4 };
 
       Dimer(Options const& opt, Rotor::Options const& ropt, system::Box const& box) : m_ctx(gpu::context()) {
         flyg_dimer_options o{opt.n, opt.max_steps, opt.hist_check_freq, opt.f2norm, opt.skin_frac, opt.proj_tol,
@@ -641,7 +642,7 @@ namespace fly {
                                       static_cast<int>(hist_sp.size()), theta_tol, count_frozen, FLYG_HOST, &st),
                    m_ctx);
         m_last = st;
-        return static_cast<Exit>(st.exit_code);
+        return static_cast<Exit>(-st.exit_code);
       }
 
       flyg_search_stats const& last_stats() const { return m_last; }

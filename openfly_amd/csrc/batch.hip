@@ -192,6 +192,7 @@ namespace flyg {
         eam_density(*eam, *nl, dt.get(), S->d_rho.get());
         unsigned fl = ctx->take_flags();   // synchronises the stream before the scratch goes away
         if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+        if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
       }
       return S;
     }

@@ -65,6 +65,7 @@ namespace {
   void check_errors(flyg_ctx_s& ctx) {
     unsigned fl = ctx.take_flags();
     if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+    if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
     if (fl) throw Error(FLYG_DEVICE_ERROR, "device error flag set");
   }
 }  // namespace

@@ -33,7 +33,15 @@ namespace flyg {
   }
 
   // Device-side error flags (bit set, read back by the host after a call).
-  enum : unsigned { ERR_SPLINE_OOB = 1u, ERR_LIST_OVERFLOW = 2u, ERR_REGION = 4u, ERR_NAN = 8u };
+  enum : unsigned { ERR_SPLINE_OOB = 1u, ERR_LIST_OVERFLOW = 2u, ERR_REGION = 4u, ERR_NAN = 8u, ERR_TYPE = 16u };
+
+  // A device-resident TypeID the host could not check: out of range sets ERR_TYPE (reported as
+  // FLYG_INVALID_ARG at the next synchronisation) and species 0 is used so no table index leaves its array.
+  __device__ inline int checked_type(uint32_t t, int ntypes, unsigned& err) {
+    if (t < static_cast<uint32_t>(ntypes)) return static_cast<int>(t);
+    err |= ERR_TYPE;
+    return 0;
+  }
 
   // ----------------------------------------------------------------------------------------------
   // Growable device buffer (hipMalloc'd, never shrinks).

@@ -138,12 +138,12 @@ namespace flyg {
       int ti = 0;
       if (act) {
         pi = A.pos[i];
-        ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
+        ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, A.pos, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[p.j]) : A.t0;
+            int tj = MULTI ? checked_type(A.types[p.j], T.ntypes, e) : A.t0;
             if (MIX && !ENERGY) {
               float r2f = static_cast<float>(p.r2);
               float rf = r2f * __frsqrt_rn(r2f);
@@ -203,12 +203,12 @@ namespace flyg {
       unsigned e = 0;
       if (act) {
         double4 pi = posf[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
+        int ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, posf, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? (MIX ? packed_type(p.w) : static_cast<int>(A.types[p.j])) : A.t0;
+            int tj = MULTI ? (MIX ? packed_type(p.w) : checked_type(A.types[p.j], T.ntypes, e)) : A.t0;
             double cf;
             if (MIX) {
               float r2f = static_cast<float>(p.r2);
@@ -258,7 +258,7 @@ namespace flyg {
       int ti = 0;
       if (act) {
         double4 pi = A.pos[i];
-        ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
+        ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
         bool fi = A.frozen[i];
         double vix = fi ? 0 : v[3 * i], viy = fi ? 0 : v[3 * i + 1], viz = fi ? 0 : v[3 * i + 2];
         int const c = min(A.cnt[i], A.K);
@@ -272,7 +272,7 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
             double r = sqrt(r2);
-            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
+            int tj = MULTI ? checked_type(A.types[j], T.ntypes, e) : A.t0;
             double f, fp;
             spl64_ffp(T, T.phi[tj + T.ntypes * ti], r, f, fp, e);
             rho += f;
@@ -307,7 +307,7 @@ namespace flyg {
       if (act) {
         double4 pi = A.pos[i];
         double2 di = dfp[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
+        int ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
         double vix = v[3 * i], viy = v[3 * i + 1], viz = v[3 * i + 2];
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
@@ -320,7 +320,7 @@ namespace flyg {
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
             double r = sqrt(r2);
-            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
+            int tj = MULTI ? checked_type(A.types[j], T.ntypes, e) : A.t0;
             double2 dj = dfp[j];
             bool fj = A.frozen[j];
             double vjx = fj ? 0 : v[3 * j], vjy = fj ? 0 : v[3 * j + 1], vjz = fj ? 0 : v[3 * j + 2];
@@ -368,7 +368,7 @@ namespace flyg {
       unsigned e = 0;
       if (act) {
         double4 pi = A.pos[i];
-        int ti = MULTI ? static_cast<int>(A.types[i]) : A.t0;
+        int ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
         int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         for (int k = lane; k < c; k += L) {
@@ -379,7 +379,7 @@ namespace flyg {
           double dx = (pj.x + sh.x) - pi.x, dy = (pj.y + sh.y) - pi.y, dz = (pj.z + sh.z) - pi.z;
           double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 < T.rc2) {
-            int tj = MULTI ? static_cast<int>(A.types[j]) : A.t0;
+            int tj = MULTI ? checked_type(A.types[j], T.ntypes, e) : A.t0;
             rho += spl64<0>(T, T.phi[tj + T.ntypes * ti], sqrt(r2), e);
           }
         }
@@ -519,6 +519,7 @@ namespace flyg {
     HIPCHK(hipMemcpyAsync(E.ctx->hscal, E.ctx->dscal.get(), sizeof(double), hipMemcpyDeviceToHost, E.ctx->stream));
     unsigned fl = E.ctx->take_flags();
     if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+    if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
     return E.ctx->hscal[0];
   }
 
@@ -641,6 +642,7 @@ namespace flyg {
     HIPCHK(hipMemcpyAsync(out, dout.get(), m * sizeof(double), hipMemcpyDeviceToHost, st));
     unsigned fl = E.ctx->take_flags();
     if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
+    if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
   }
 
   void stage_atoms(flyg_eam_s& E, uint32_t const* types, uint8_t const* frozen, long n, int where,
@@ -650,24 +652,37 @@ namespace flyg {
       *d_frozen = frozen;
       return;
     }
-    E.types.reserve(n);
-    E.frozen.reserve(n);
     bool same_t = static_cast<long>(E.last_types.size()) == n &&
                   std::memcmp(E.last_types.data(), types, n * sizeof(uint32_t)) == 0;
     bool same_f = static_cast<long>(E.last_frozen.size()) == n &&
                   std::memcmp(E.last_frozen.data(), frozen, n * sizeof(uint8_t)) == 0;
-    if (!same_t) {
-      E.staged_single = n > 0 ? static_cast<int>(types[0]) : -1;
+    int single = -1;
+    if (!same_t) {   // validate before anything staged changes: a rejected call leaves no stale state
+      single = n > 0 ? static_cast<int>(types[0]) : -1;
       for (long i = 0; i < n; ++i) {
         require(types[i] < static_cast<uint32_t>(E.tab.ntypes), "TypeID out of range");
-        if (types[i] != types[0]) E.staged_single = -1;
+        if (types[i] != types[0]) single = -1;
       }
-      E.last_types.assign(types, types + n);
+    }
+    // The device buffers may be reallocated below (contents lost): forget what they held first.
+    if (!same_t || E.types.cap < static_cast<std::size_t>(n)) {
+      E.last_types.clear();
+      same_t = false;
+    }
+    if (!same_f || E.frozen.cap < static_cast<std::size_t>(n)) {
+      E.last_frozen.clear();
+      same_f = false;
+    }
+    E.types.reserve(n);
+    E.frozen.reserve(n);
+    if (!same_t) {
       HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, E.ctx->stream));
+      E.staged_single = single;
+      E.last_types.assign(types, types + n);
     }
     if (!same_f) {
-      E.last_frozen.assign(frozen, frozen + n);
       HIPCHK(hipMemcpyAsync(E.frozen.get(), frozen, n * sizeof(uint8_t), hipMemcpyHostToDevice, E.ctx->stream));
+      E.last_frozen.assign(frozen, frozen + n);
     }
     if (!same_t || !same_f) HIPCHK(hipStreamSynchronize(E.ctx->stream));
     *d_types = E.types.get();

@@ -234,13 +234,33 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
-def _ptr(a, dtype):
-    """(pointer, where, keepalive) of a numpy array or a torch tensor."""
+_TORCH_DTYPES = {np.uint32: "uint32", np.uint8: "uint8", np.float64: "float64"}
+
+
+def _ptr(a, dtype, numel=None, device=None):
+    """(pointer, where, keepalive) of a numpy array or a torch tensor.
+
+    A device tensor is passed as is, so it must already have the element type, be contiguous, hold
+    `numel` elements and live on `device`; anything else raises FlyError (the library would read it
+    as raw memory).  Host arrays are converted (a copy when needed) and must hold `numel` elements.
+    """
     if _is_torch(a):
         if a.is_cuda:
+            want = _TORCH_DTYPES[dtype]
+            got = str(a.dtype).replace("torch.", "")
+            if got != want:
+                raise FlyError(f"device tensor has dtype {got}, expected {want}")
+            if not a.is_contiguous():
+                raise FlyError("device tensor must be contiguous")
+            if numel is not None and a.numel() != numel:
+                raise FlyError(f"device tensor has {a.numel()} elements, expected {numel}")
+            if device is not None and a.device.index != device:
+                raise FlyError(f"device tensor is on cuda:{a.device.index}, the context is on cuda:{device}")
             return C.c_void_p(a.data_ptr()), DEVICE, a
         a = a.numpy()
     arr = np.ascontiguousarray(a, dtype=dtype)
+    if numel is not None and arr.size != numel:
+        raise FlyError(f"array has {arr.size} elements, expected {numel}")
     return C.c_void_p(arr.ctypes.data), HOST, arr
 
 
@@ -340,8 +360,9 @@ class EAM:
         return out
 
     def energy(self, types, frozen, nl: "List"):
-        tp, w1, k1 = _ptr(types, np.uint32)
-        fp, w2, k2 = _ptr(frozen, np.uint8)
+        dev = self.ctx.device
+        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if w1 != w2:
             raise FlyError("types and frozen must both be host or both be device arrays")
         E = C.c_double()
@@ -349,23 +370,25 @@ class EAM:
         return E.value
 
     def gradient(self, types, frozen, nl: "List", out=None):
-        tp, w1, k1 = _ptr(types, np.uint32)
-        fp, w2, k2 = _ptr(frozen, np.uint8)
+        dev = self.ctx.device
+        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if out is None:
             out = np.zeros((nl.n, 3))
-        op, w3, k3 = _ptr(out, np.float64)
+        op, w3, k3 = _ptr(out, np.float64, 3 * nl.n, dev)
         if not (w1 == w2 == w3):
             raise FlyError("all arrays must be host or all device")
         self.ctx.check(lib().flyg_eam_gradient(self.h, nl.h, tp, fp, nl.n, w1, op))
         return out if w3 == DEVICE else k3.reshape(-1, 3)
 
     def hvp(self, types, frozen, nl: "List", v, out=None):
-        tp, w1, k1 = _ptr(types, np.uint32)
-        fp, w2, k2 = _ptr(frozen, np.uint8)
-        vp, w3, k3 = _ptr(v, np.float64)
+        dev = self.ctx.device
+        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
+        vp, w3, k3 = _ptr(v, np.float64, 3 * nl.n, dev)
         if out is None:
             out = np.zeros((nl.n, 3))
-        op, w4, k4 = _ptr(out, np.float64)
+        op, w4, k4 = _ptr(out, np.float64, 3 * nl.n, dev)
         if not (w1 == w2 == w3 == w4):
             raise FlyError("all arrays must be host or all device")
         self.ctx.check(lib().flyg_eam_hvp(self.h, nl.h, tp, fp, nl.n, vp, op, w1))
@@ -394,13 +417,16 @@ class List:
             self.h = None
 
     def rebuild(self, positions, num_threads=1):
-        p, w, keep = _ptr(positions, np.float64)
-        n = (positions.numel() if _is_torch(positions) else np.size(positions)) // 3
+        size = positions.numel() if _is_torch(positions) else np.size(positions)
+        if size % 3:
+            raise FlyError("positions must hold 3 values per atom")
+        p, w, keep = _ptr(positions, np.float64, size, self.ctx.device)
+        n = size // 3
         self.ctx.check(lib().flyg_list_rebuild(self.h, p, n, w))
         self.n = n
 
     def update(self, delta):
-        p, w, keep = _ptr(delta, np.float64)
+        p, w, keep = _ptr(delta, np.float64, 3 * self.n, self.ctx.device)
         self.ctx.check(lib().flyg_list_update(self.h, p, self.n, w))
 
     def size(self):

@@ -29,6 +29,53 @@ namespace {
     Box box;
     List nl;
   };
+
+  // Options as flat double arrays (the order of the fields of DimerOptions / RotorOptions).
+  DimerOptions dimer_opts(double const* d) {
+    DimerOptions opt;
+    if (d) {
+      opt.n = static_cast<int>(d[0]);
+      opt.max_steps = static_cast<int>(d[1]);
+      opt.hist_check_freq = static_cast<int>(d[2]);
+      opt.f2norm = d[3];
+      opt.skin_frac = d[4];
+      opt.proj_tol = d[5];
+      opt.max_trust = d[6];
+      opt.min_trust = d[7];
+      opt.grow_trust = d[8];
+      opt.shrink_trust = d[9];
+      opt.convex_max = static_cast<int>(d[10]);
+      opt.use_history = d[11] != 0;
+    }
+    return opt;
+  }
+
+  RotorOptions rotor_opts(double const* r) {
+    RotorOptions ropt;
+    if (r) {
+      ropt.n = static_cast<int>(r[0]);
+      ropt.iter_max_rot = static_cast<int>(r[1]);
+      ropt.delta_r = r[2];
+      ropt.theta_tol = r[3];
+      ropt.relax_in_convex = r[4] != 0;
+    }
+    return ropt;
+  }
+
+  std::vector<std::vector<double>> hist_of(double const* hist, int nhist, long len) {
+    std::vector<std::vector<double>> hs;
+    for (int k = 0; k < nhist; ++k) hs.emplace_back(hist + len * k, hist + len * (k + 1));
+    return hs;
+  }
+
+  void put_stats(DimerStats const& st, long* stats, double* curv) {
+    if (stats) {
+      stats[0] = st.steps;
+      stats[1] = st.grads;
+      stats[2] = st.rebuilds;
+    }
+    if (curv) *curv = st.curv;
+  }
 }  // namespace
 
 extern "C" {
@@ -249,45 +296,17 @@ int orc_find_sp(void* b, void* d, double* r, double* ax, std::uint32_t const* id
                 double const* in_min, double const* hist, int nhist, double theta_tol, int count_frozen, int threads,
                 double const* dopts, double const* ropts, int* exit_code, long* stats, double* curv) {
   return guard([&] {
-    DimerOptions opt;
-    RotorOptions ropt;
-    if (dopts) {
-      opt.n = static_cast<int>(dopts[0]);
-      opt.max_steps = static_cast<int>(dopts[1]);
-      opt.hist_check_freq = static_cast<int>(dopts[2]);
-      opt.f2norm = dopts[3];
-      opt.skin_frac = dopts[4];
-      opt.proj_tol = dopts[5];
-      opt.max_trust = dopts[6];
-      opt.min_trust = dopts[7];
-      opt.grow_trust = dopts[8];
-      opt.shrink_trust = dopts[9];
-      opt.convex_max = static_cast<int>(dopts[10]);
-      opt.use_history = dopts[11] != 0;
-    }
-    if (ropts) {
-      ropt.n = static_cast<int>(ropts[0]);
-      ropt.iter_max_rot = static_cast<int>(ropts[1]);
-      ropt.delta_r = ropts[2];
-      ropt.theta_tol = ropts[3];
-      ropt.relax_in_convex = ropts[4] != 0;
-    }
+    DimerOptions opt = dimer_opts(dopts);
+    RotorOptions ropt = rotor_opts(ropts);
     EAM pot(static_cast<DataEAM*>(d));
     std::vector<double> rr(r, r + 3 * n), aa(ax, ax + 3 * n);
-    std::vector<std::vector<double>> hs;
-    for (int k = 0; k < nhist; ++k) hs.emplace_back(hist + 3 * n * k, hist + 3 * n * (k + 1));
     DimerStats st;
-    Exit e = dimer_find_sp(opt, ropt, *static_cast<Box*>(b), pot, rr, aa, id, fzn, n, in_min, hs, theta_tol,
-                           count_frozen, threads, &st);
+    Exit e = dimer_find_sp(opt, ropt, *static_cast<Box*>(b), pot, rr, aa, id, fzn, n, in_min, hist_of(hist, nhist, 3 * n),
+                           theta_tol, count_frozen, threads, &st);
     std::memcpy(r, rr.data(), rr.size() * sizeof(double));
     std::memcpy(ax, aa.data(), aa.size() * sizeof(double));
     *exit_code = static_cast<int>(e);
-    if (stats) {
-      stats[0] = st.steps;
-      stats[1] = st.grads;
-      stats[2] = st.rebuilds;
-    }
-    if (curv) *curv = st.curv;
+    put_stats(st, stats, curv);
   });
 }
 
@@ -331,6 +350,66 @@ int orc_normals(std::uint64_t const* seed, long m, double mean, double sd, doubl
     Normal nm{mean, sd};
     for (long i = 0; i < m; ++i) out[i] = nm(g);
   });
+}
+
+// Rotor::eff_gradient on an EAM system (rotor.cpp:49-164): ax rotated in place, out = effective gradient.
+int orc_rotor_eff_gradient(void* b, void* d, double const* r, double* ax, std::uint32_t const* id,
+                           std::uint8_t const* fzn, long n, int count_frozen, int threads, double const* ropts,
+                           double* out, double* curv, long* grads) {
+  return guard([&] {
+    EAM pot(static_cast<DataEAM*>(d));
+    std::vector<double> aa(ax, ax + 3 * n), o;
+    *curv = rotor_eff_gradient(rotor_opts(ropts), *static_cast<Box*>(b), pot, r, aa, id, fzn, n, count_frozen, threads,
+                               o, grads);
+    std::memcpy(ax, aa.data(), aa.size() * sizeof(double));
+    std::memcpy(out, o.data(), o.size() * sizeof(double));
+  });
+}
+
+// The rotor on E = 1/2 (x - c)^T A (x - c) (A: n3 x n3 row-major).
+int orc_rotor_quadratic(long n3, double const* A, double const* c, double const* x, double* ax, int count_frozen,
+                        double const* ropts, double* out, double* curv, long* grads) {
+  return guard([&] {
+    std::vector<double> aa(ax, ax + n3), o;
+    *curv = rotor_quadratic(rotor_opts(ropts), n3, A, c, x, aa, count_frozen, o, grads);
+    std::memcpy(ax, aa.data(), aa.size() * sizeof(double));
+    std::memcpy(out, o.data(), o.size() * sizeof(double));
+  });
+}
+
+int orc_dimer_quadratic(long n3, double const* A, double const* c, double* r, double* ax, double const* in_min,
+                        double const* hist, int nhist, double theta_tol, int count_frozen, double const* dopts,
+                        double const* ropts, int* exit_code, long* stats, double* curv) {
+  return guard([&] {
+    std::vector<double> rr(r, r + n3), aa(ax, ax + n3);
+    DimerStats st;
+    Exit e = dimer_quadratic(dimer_opts(dopts), rotor_opts(ropts), n3, A, c, rr, aa, in_min, hist_of(hist, nhist, n3),
+                             theta_tol, count_frozen, &st);
+    std::memcpy(r, rr.data(), rr.size() * sizeof(double));
+    std::memcpy(ax, aa.data(), aa.size() * sizeof(double));
+    *exit_code = static_cast<int>(e);
+    put_stats(st, stats, curv);
+  });
+}
+
+int orc_dimer_mueller_brown(double kz, double* r, double* ax, double const* in_min, double const* hist, int nhist,
+                            double theta_tol, double const* dopts, double const* ropts, int* exit_code, long* stats,
+                            double* curv) {
+  return guard([&] {
+    std::vector<double> rr(r, r + 3), aa(ax, ax + 3);
+    DimerStats st;
+    Exit e = dimer_mueller_brown(dimer_opts(dopts), rotor_opts(ropts), kz, rr, aa, in_min, hist_of(hist, nhist, 3),
+                                 theta_tol, &st);
+    std::memcpy(r, rr.data(), rr.size() * sizeof(double));
+    std::memcpy(ax, aa.data(), aa.size() * sizeof(double));
+    *exit_code = static_cast<int>(e);
+    put_stats(st, stats, curv);
+  });
+}
+
+int orc_mueller_brown(double const* x, double kz, double* e, double* g) {
+  mueller_brown(x, kz, e, g);
+  return 0;
 }
 
 }  // extern "C"

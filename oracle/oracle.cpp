@@ -634,8 +634,9 @@ namespace orc {
 
       explicit Rotor(RotorOptions const& o) : opt(o), core(o.n) {}
 
-      double eff_gradient(std::vector<double>& out, std::vector<double>& ax, std::uint32_t const* id,
-                          std::uint8_t const* fzn, long n, EAM& pot, List& nl, int count_frozen, int threads) {
+      // M: gradient(g) at the model's current positions; update(d): positions -= d (List::update).
+      template <typename M>
+      double eff_gradient(std::vector<double>& out, std::vector<double>& ax, long n, M& model, int count_frozen) {
         std::size_t N = static_cast<std::size_t>(3 * n);
         delta.resize(N);
         delta_prev.resize(N);
@@ -648,12 +649,12 @@ namespace orc {
         core.clear();
         double const dr = opt.delta_r;
 
-        pot.gradient(g0.data(), id, fzn, nl, threads);
+        model.gradient(g0.data());
         ++grads;
         for (std::size_t k = 0; k < N; ++k) delta[k] = -dr * ax[k];
-        nl.update(delta.data(), n);
+        model.update(delta.data());
         std::swap(delta, delta_prev);
-        pot.gradient(g1.data(), id, fzn, nl, threads);
+        model.gradient(g1.data());
         ++grads;
 
         double curv = [&] {
@@ -682,9 +683,9 @@ namespace orc {
             for (std::size_t k = 0; k < N; ++k) delta[k] = -dr * axisp[k];
             std::swap(delta, delta_prev);
             for (std::size_t k = 0; k < N; ++k) delta[k] = delta_prev[k] - delta[k];
-            nl.update(delta.data(), n);
+            model.update(delta.data());
 
-            pot.gradient(g1p.data(), id, fzn, nl, threads);
+            model.gradient(g1p.data());
             ++grads;
 
             for (std::size_t k = 0; k < N; ++k) dg[k] = g1p[k] - g0[k];
@@ -709,7 +710,7 @@ namespace orc {
         }();
 
         for (std::size_t k = 0; k < N; ++k) delta[k] = -delta_prev[k];
-        nl.update(delta.data(), n);
+        model.update(delta.data());
 
         double p = vdot(g0, ax);
         out.resize(N);
@@ -722,27 +723,40 @@ namespace orc {
         return curv;
       }
     };
+
+    // EAM model of a caller's system over a neighbour list (the reference's pot + nl pair).
+    struct EAMModel {
+      EAM& pot;
+      List& nl;
+      std::uint32_t const* id;
+      std::uint8_t const* fzn;
+      long n;
+      int threads;
+      void gradient(double* g) { pot.gradient(g, id, fzn, nl, threads); }
+      void update(double const* d) { nl.update(d, n); }
+      void rebuild(double const* r) { nl.rebuild(r, n, threads); }
+    };
   }  // namespace
 
   // ================================================================================================
   // Dimer::find_sp: dimer.cpp:28-171
   // ================================================================================================
 
-  Exit dimer_find_sp(DimerOptions const& opt, RotorOptions const& ropt, Box const& box, EAM& pot, std::vector<double>& r,
-                     std::vector<double>& ax, std::uint32_t const* id, std::uint8_t const* fzn, long n,
-                     double const* in_min, std::vector<std::vector<double>> const& hist_sp, double theta_tol,
-                     int count_frozen, int threads, DimerStats* stats) {
+  namespace {
+    // Dimer::find_sp's loop over a model (M as for Rotor::eff_gradient, plus rebuild(r): new list at r).
+    template <typename M>
+    Exit dimer_core(DimerOptions const& opt, RotorOptions const& ropt, M& model, double skin, std::vector<double>& r,
+                    std::vector<double>& ax, long n, double const* in_min,
+                    std::vector<std::vector<double>> const& hist_sp, double theta_tol, int count_frozen,
+                    DimerStats* stats) {
     Rotor rotor(ropt);
     StepLBFGS core(opt.n);
-    double rot_rcut = pot.r_cut() + ropt.delta_r * 2;   // rotor.hpp:123
-    double skin = std::max(std::pow(opt.skin_frac, 1. / 3.) - 1, 0.0) * rot_rcut;
-    List nl(box, rot_rcut + skin);
     std::size_t N = static_cast<std::size_t>(3 * n);
     std::vector<double> eff;
     DimerStats st;
 
-    nl.rebuild(r.data(), n, threads);
-    double curv = rotor.eff_gradient(eff, ax, id, fzn, n, pot, nl, count_frozen, threads);
+    model.rebuild(r.data());
+    double curv = rotor.eff_gradient(eff, ax, n, model, count_frozen);
     double trust = opt.min_trust, acc = 0, convex_count = 0;
     Exit ret = iter_max;
 
@@ -784,13 +798,13 @@ namespace orc {
       acc += std::sqrt(vdot(Hg, Hg));   // flat-array colwise().sum(): the L2 norm of the step
       for (std::size_t k = 0; k < N; ++k) r[k] -= Hg[k];
       if (acc > 0.5 * skin) {
-        nl.rebuild(r.data(), n, threads);
+        model.rebuild(r.data());
         acc = 0;
         ++st.rebuilds;
       } else {
-        nl.update(Hg.data(), n);
+        model.update(Hg.data());
       }
-      curv = rotor.eff_gradient(eff, ax, id, fzn, n, pot, nl, count_frozen, threads);
+      curv = rotor.eff_gradient(eff, ax, n, model, count_frozen);
       convex_count = curv > 0 ? convex_count + 1 : 0;
       double proj = vdot(eff, Hg);
       if (proj < -opt.proj_tol) {
@@ -804,6 +818,111 @@ namespace orc {
     st.curv = curv;
     if (stats) *stats = st;
     return ret;
+    }
+  }  // namespace
+
+  Exit dimer_find_sp(DimerOptions const& opt, RotorOptions const& ropt, Box const& box, EAM& pot, std::vector<double>& r,
+                     std::vector<double>& ax, std::uint32_t const* id, std::uint8_t const* fzn, long n,
+                     double const* in_min, std::vector<std::vector<double>> const& hist_sp, double theta_tol,
+                     int count_frozen, int threads, DimerStats* stats) {
+    double rot_rcut = pot.r_cut() + ropt.delta_r * 2;   // rotor.hpp:123
+    double skin = std::max(std::pow(opt.skin_frac, 1. / 3.) - 1, 0.0) * rot_rcut;
+    List nl(box, rot_rcut + skin);
+    EAMModel model{pot, nl, id, fzn, n, threads};
+    return dimer_core(opt, ropt, model, skin, r, ax, n, in_min, hist_sp, theta_tol, count_frozen, stats);
+  }
+
+  double rotor_eff_gradient(RotorOptions const& ropt, Box const& box, EAM& pot, double const* r, std::vector<double>& ax,
+                            std::uint32_t const* id, std::uint8_t const* fzn, long n, int count_frozen, int threads,
+                            std::vector<double>& out, long* grads) {
+    List nl(box, pot.r_cut() + ropt.delta_r * 2);   // the rotor's list cutoff, rotor.hpp:123
+    nl.rebuild(r, n, threads);
+    EAMModel model{pot, nl, id, fzn, n, threads};
+    Rotor rotor(ropt);
+    double c = rotor.eff_gradient(out, ax, n, model, count_frozen);
+    if (grads) *grads = rotor.grads;
+    return c;
+  }
+
+  // ------------------------------------------------------------------------------------------------
+  // Analytic surfaces for pinning the rotor and dimer (no reference counterpart).
+  // ------------------------------------------------------------------------------------------------
+
+  namespace {
+    // E = 1/2 (x - c)^T A (x - c), g = A (x - c); A dense 3n x 3n row-major.
+    struct QuadModel {
+      long n3;
+      double const* A;
+      double const* c;
+      std::vector<double> x;
+      void gradient(double* g) {
+        for (long i = 0; i < n3; ++i) {
+          double v = 0;
+          for (long j = 0; j < n3; ++j) v += A[i * n3 + j] * (x[j] - c[j]);
+          g[i] = v;
+        }
+      }
+      void update(double const* d) {
+        for (long i = 0; i < n3; ++i) x[i] -= d[i];
+      }
+      void rebuild(double const* r) { x.assign(r, r + n3); }
+    };
+
+    // One atom on the Mueller-Brown surface in (x, y) plus kz z^2 / 2 (K. Mueller and L. D. Brown,
+    // Theor. Chim. Acta 53 (1979) 75).
+    struct MBModel {
+      double kz;
+      std::vector<double> x = std::vector<double>(3);
+      void gradient(double* g) { mueller_brown(x.data(), kz, nullptr, g); }
+      void update(double const* d) {
+        for (int i = 0; i < 3; ++i) x[static_cast<std::size_t>(i)] -= d[i];
+      }
+      void rebuild(double const* r) { x.assign(r, r + 3); }
+    };
+  }  // namespace
+
+  void mueller_brown(double const* x, double kz, double* e, double* g) {
+    static double const A[4] = {-200, -100, -170, 15}, a[4] = {-1, -1, -6.5, 0.7}, b[4] = {0, 0, 11, 0.6},
+                        c[4] = {-10, -10, -6.5, 0.7}, x0[4] = {1, 0, -0.5, -1}, y0[4] = {0, 0.5, 1.5, 1};
+    double E = 0, gx = 0, gy = 0;
+    for (int k = 0; k < 4; ++k) {
+      double dx = x[0] - x0[k], dy = x[1] - y0[k];
+      double t = A[k] * std::exp(a[k] * dx * dx + b[k] * dx * dy + c[k] * dy * dy);
+      E += t;
+      gx += t * (2 * a[k] * dx + b[k] * dy);
+      gy += t * (b[k] * dx + 2 * c[k] * dy);
+    }
+    if (e) *e = E + 0.5 * kz * x[2] * x[2];
+    if (g) {
+      g[0] = gx;
+      g[1] = gy;
+      g[2] = kz * x[2];
+    }
+  }
+
+  double rotor_quadratic(RotorOptions const& ropt, long n3, double const* A, double const* c, double const* x,
+                         std::vector<double>& ax, int count_frozen, std::vector<double>& out, long* grads) {
+    QuadModel model{n3, A, c, std::vector<double>(x, x + n3)};
+    Rotor rotor(ropt);
+    double cv = rotor.eff_gradient(out, ax, n3 / 3, model, count_frozen);
+    if (grads) *grads = rotor.grads;
+    return cv;
+  }
+
+  Exit dimer_quadratic(DimerOptions const& opt, RotorOptions const& ropt, long n3, double const* A, double const* c,
+                       std::vector<double>& r, std::vector<double>& ax, double const* in_min,
+                       std::vector<std::vector<double>> const& hist_sp, double theta_tol, int count_frozen,
+                       DimerStats* stats) {
+    QuadModel model{n3, A, c, {}};
+    return dimer_core(opt, ropt, model, 1e300, r, ax, n3 / 3, in_min, hist_sp, theta_tol, count_frozen, stats);
+  }
+
+  Exit dimer_mueller_brown(DimerOptions const& opt, RotorOptions const& ropt, double kz, std::vector<double>& r,
+                           std::vector<double>& ax, double const* in_min, std::vector<std::vector<double>> const& hist_sp,
+                           double theta_tol, DimerStats* stats) {
+    MBModel model{kz};
+    // One atom, counted as "frozen present" so the rotor does not project out its (only) translation.
+    return dimer_core(opt, ropt, model, 1e300, r, ax, 1, in_min, hist_sp, theta_tol, 1, stats);
   }
 
   // ================================================================================================

@@ -307,6 +307,25 @@ namespace orc {
                      double const* in_min, std::vector<std::vector<double>> const& hist_sp, double theta_tol,
                      int count_frozen, int threads, DimerStats* stats);
 
+  // Rotor::eff_gradient (rotor.cpp:49-164) on a caller's system: a list at the rotor's cutoff is built at
+  // r; ax is rotated in place; out receives the effective gradient; returns the curvature.
+  double rotor_eff_gradient(RotorOptions const& ropt, Box const& box, EAM& pot, double const* r, std::vector<double>& ax,
+                            std::uint32_t const* id, std::uint8_t const* fzn, long n, int count_frozen, int threads,
+                            std::vector<double>& out, long* grads);
+
+  // The same rotor and dimer on analytic surfaces (test infrastructure for pinning them):
+  // E = 1/2 (x - c)^T A (x - c) over n3 coordinates, and one atom on the Mueller-Brown surface.
+  void mueller_brown(double const* x, double kz, double* e, double* g);
+  double rotor_quadratic(RotorOptions const& ropt, long n3, double const* A, double const* c, double const* x,
+                         std::vector<double>& ax, int count_frozen, std::vector<double>& out, long* grads);
+  Exit dimer_quadratic(DimerOptions const& opt, RotorOptions const& ropt, long n3, double const* A, double const* c,
+                       std::vector<double>& r, std::vector<double>& ax, double const* in_min,
+                       std::vector<std::vector<double>> const& hist_sp, double theta_tol, int count_frozen,
+                       DimerStats* stats);
+  Exit dimer_mueller_brown(DimerOptions const& opt, RotorOptions const& ropt, double kz, std::vector<double>& r,
+                           std::vector<double>& ax, double const* in_min, std::vector<std::vector<double>> const& hist_sp,
+                           double theta_tol, DimerStats* stats);
+
   // ------------------------------------------------------------------------------------------------
   // Xoshiro256** (random.hpp:42-162) and the Gaussian sampler used for seeded perturbations
   // ------------------------------------------------------------------------------------------------

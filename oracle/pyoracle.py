@@ -74,6 +74,14 @@ def lib():
             "orc_perturb": (C.c_int, [_u64, C.c_double, C.c_double, _vp, C.c_long, _d, _u8, C.c_long, _d, _d, _d]),
             "orc_xoshiro": (C.c_int, [_u64, C.c_long, C.c_int, _u64]),
             "orc_normals": (C.c_int, [_u64, C.c_long, C.c_double, C.c_double, _d]),
+            "orc_rotor_eff_gradient": (C.c_int, [_vp, _vp, _d, _d, _u32, _u8, C.c_long, C.c_int, C.c_int, _d, _d,
+                                                 _d, _l]),
+            "orc_rotor_quadratic": (C.c_int, [C.c_long, _d, _d, _d, _d, C.c_int, _d, _d, _d, _l]),
+            "orc_dimer_quadratic": (C.c_int, [C.c_long, _d, _d, _d, _d, _d, _d, C.c_int, C.c_double, C.c_int, _d,
+                                              _d, _i, _l, _d]),
+            "orc_dimer_mueller_brown": (C.c_int, [C.c_double, _d, _d, _d, _d, C.c_int, C.c_double, _d, _d, _i, _l,
+                                                  _d]),
+            "orc_mueller_brown": (C.c_int, [_d, C.c_double, _d, _d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -378,3 +386,80 @@ def normals(seed, m, mean=0.0, sd=1.0):
     out = np.zeros(m)
     _check(lib().orc_normals(_p(s, _u64), m, mean, sd, _p(out, _d)))
     return out
+
+
+def rotor_eff_gradient(box, data, pos, axis, types, frozen, count_frozen=None, threads=1, rotor=None):
+    """Rotor::eff_gradient at pos: (curvature, rotated axis, effective gradient, gradient evaluations)."""
+    n = pos.size // 3
+    r = _f64(pos).ravel()
+    ax = _f64(axis).copy().ravel()
+    t, f = _sys(types, frozen, n)
+    if count_frozen is None:
+        count_frozen = int(f.sum())
+    out = np.zeros(3 * n)
+    curv = C.c_double()
+    grads = C.c_long()
+    _check(lib().orc_rotor_eff_gradient(box.h, data.h, _p(r, _d), _p(ax, _d), _p(t, _u32), _p(f, _u8), n,
+                                        count_frozen, threads, _p(_opts(ROTOR_DEFAULTS, rotor), _d), _p(out, _d),
+                                        C.byref(curv), C.byref(grads)))
+    return curv.value, ax.reshape(-1, 3), out.reshape(-1, 3), grads.value
+
+
+def rotor_quadratic(A, c, x, axis, count_frozen=1, rotor=None):
+    """The same rotor on E = 1/2 (x - c)^T A (x - c): (curvature, axis, effective gradient, evaluations)."""
+    A = _f64(A)
+    n3 = A.shape[0]
+    c = _f64(c).ravel()
+    x = _f64(x).ravel()
+    ax = _f64(axis).copy().ravel()
+    out = np.zeros(n3)
+    curv = C.c_double()
+    grads = C.c_long()
+    _check(lib().orc_rotor_quadratic(n3, _p(A, _d), _p(c, _d), _p(x, _d), _p(ax, _d), count_frozen,
+                                     _p(_opts(ROTOR_DEFAULTS, rotor), _d), _p(out, _d), C.byref(curv),
+                                     C.byref(grads)))
+    return curv.value, ax, out, grads.value
+
+
+def _dimer_out(r, ax, ex, st, curv):
+    return dict(pos=r, axis=ax, exit=ex.value, steps=int(st[0]), grads=int(st[1]), rebuilds=int(st[2]),
+                curv=curv.value)
+
+
+def dimer_quadratic(A, c, x, axis, in_min, hist=None, theta_tol=0.0, count_frozen=1, dimer=None, rotor=None):
+    """Dimer::find_sp's loop on E = 1/2 (x - c)^T A (x - c)."""
+    A = _f64(A)
+    n3 = A.shape[0]
+    c = _f64(c).ravel()
+    r = _f64(x).copy().ravel()
+    ax = _f64(axis).copy().ravel()
+    mn = _f64(in_min).ravel()
+    hs = np.zeros(1) if not hist else _f64(np.stack([np.ravel(h) for h in hist]))
+    ex, st, curv = C.c_int(), np.zeros(3, np.int64), C.c_double()
+    _check(lib().orc_dimer_quadratic(n3, _p(A, _d), _p(c, _d), _p(r, _d), _p(ax, _d), _p(mn, _d), _p(hs, _d),
+                                     len(hist or []), theta_tol, count_frozen, _p(_opts(DIMER_DEFAULTS, dimer), _d),
+                                     _p(_opts(ROTOR_DEFAULTS, rotor), _d), C.byref(ex), _p(st, _l), C.byref(curv)))
+    return _dimer_out(r, ax, ex, st, curv)
+
+
+def dimer_mueller_brown(x, axis, in_min, kz=100.0, hist=None, theta_tol=0.0, dimer=None, rotor=None):
+    """Dimer::find_sp's loop for one atom on the Mueller-Brown surface (x, y) + kz z^2 / 2."""
+    r = _f64(x).copy().ravel()
+    ax = _f64(axis).copy().ravel()
+    mn = _f64(in_min).ravel()
+    hs = np.zeros(1) if not hist else _f64(np.stack([np.ravel(h) for h in hist]))
+    ex, st, curv = C.c_int(), np.zeros(3, np.int64), C.c_double()
+    _check(lib().orc_dimer_mueller_brown(kz, _p(r, _d), _p(ax, _d), _p(mn, _d), _p(hs, _d), len(hist or []),
+                                         theta_tol, _p(_opts(DIMER_DEFAULTS, dimer), _d),
+                                         _p(_opts(ROTOR_DEFAULTS, rotor), _d), C.byref(ex), _p(st, _l),
+                                         C.byref(curv)))
+    return _dimer_out(r, ax, ex, st, curv)
+
+
+def mueller_brown(x, kz=100.0):
+    """(energy, gradient) of the Mueller-Brown surface as the oracle evaluates it."""
+    x = _f64(x).ravel()
+    e = C.c_double()
+    g = np.zeros(3)
+    lib().orc_mueller_brown(_p(x, _d), kz, C.byref(e), _p(g, _d))
+    return e.value, g

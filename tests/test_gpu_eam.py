@@ -217,3 +217,113 @@ def test_gradient_independent_of_lanes(ctx, precision):
         pot.set_lanes(L)
         out.append(pot.gradient(s.types, s.frozen, nl))
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[1], out[2])
+
+
+@pytest.fixture(scope="module")
+def alloy30():
+    """The 30^3 Fe-X alloy of the gradient benchmark (54,529 atoms) with its FP64 oracle gradient at the
+    benchmark's list cutoff (r_c + 0.6 A skin)."""
+    path = systems.potential_path(("Fe", "X"))
+    s = systems.fe_x_alloy(seed=2, cells=30)
+    data = po.DataEAM(path)
+    rc = 5.3 + 0.6
+    _, Eo, go = orc_eval(data, s.pos, s.basis, s.types, s.frozen, rc)
+    return path, s, rc, Eo, go
+
+
+# Mixed form (FP32 pair distance, splines and force magnitude; FP64 sums): the largest force error
+# against the FP64 oracle, relative to the largest force component.  Measured 3e-7 on the 30^3 alloy.
+MIXED_FORCE_RTOL = 2e-6
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_fe_x_alloy_mixed_vs_oracle_30(ctx, alloy30, lanes):
+    # The kernels the gradient benchmark times: two species (MULTI) in the mixed form.
+    path, s, rc, Eo, go = alloy30
+    print(f"case fe_x_alloy(30^3) n={s.n} mixed lanes={lanes}", flush=True)
+    pot = fa.EAM(ctx, path)
+    pot.set_precision(fa.MIXED)
+    pot.set_lanes(lanes)
+    nl = fa.List(ctx, fa.Box(s.basis), rc)
+    nl.rebuild(s.pos)
+    g = pot.gradient(s.types, s.frozen, nl)
+    err = np.max(np.abs(g - go)) / np.max(np.abs(go))
+    rms = np.sqrt(np.mean((g - go) ** 2)) / np.sqrt(np.mean(go ** 2))
+    print(f"mixed two-species max rel force error {err:.3e}, rms rel {rms:.3e}")
+    assert err < MIXED_FORCE_RTOL
+    E = pot.energy(s.types, s.frozen, nl)   # energies stay FP64
+    assert abs(E - Eo) <= 1e-10 * abs(Eo)
+
+
+def test_fe_x_alloy_fp64_vs_oracle_30(ctx, alloy30):
+    path, s, rc, Eo, go = alloy30
+    pot = fa.EAM(ctx, path)
+    nl = fa.List(ctx, fa.Box(s.basis), rc)
+    nl.rebuild(s.pos)
+    g = pot.gradient(s.types, s.frozen, nl)
+    assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+
+
+def test_rejected_types_leave_no_stale_staging(ctx, wen_gpu, wen_orc):
+    # A call rejected for an out-of-range TypeID must not leave the staged types (or the one-species
+    # shortcut) of that call behind: the next call with the earlier types matches the oracle.
+    rng = np.random.default_rng(31)
+    pos, basis = H.kat_lattice()
+    pos = pos + rng.normal(0, 0.05, pos.shape)
+    n = len(pos)
+    t1 = (rng.random(n) < 0.2).astype(np.uint32)
+    f = np.zeros(n, np.uint8)
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut())
+    nl.rebuild(pos)
+    g_first = wen_gpu.gradient(t1, f, nl)
+    # A larger system (reallocates the staging buffers) whose last type is out of range.
+    big = np.concatenate([pos, pos + np.array([basis[0, 0], 0, 0])])
+    bigbox = basis.copy()
+    bigbox[0, 0] *= 2
+    nl2 = fa.List(ctx, fa.Box(bigbox), wen_gpu.r_cut())
+    nl2.rebuild(big)
+    bad = np.zeros(2 * n, np.uint32)
+    bad[-1] = 7
+    with pytest.raises(fa.FlyError):
+        wen_gpu.gradient(bad, np.zeros(2 * n, np.uint8), nl2)
+    # One species but one bad entry (the one-species shortcut must not stick either).
+    bad1 = np.zeros(n, np.uint32)
+    bad1[-1] = 9
+    with pytest.raises(fa.FlyError):
+        wen_gpu.gradient(bad1, f, nl)
+    g = wen_gpu.gradient(t1, f, nl)
+    onl = po.List(po.Box(basis), wen_orc.r_cut)
+    onl.rebuild(pos, 4)
+    go = po.gradient(wen_orc, onl, t1, f, 4)
+    assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+    assert np.array_equal(g, g_first)
+
+
+def test_device_arrays_checked(ctx, wen_gpu):
+    torch = pytest.importorskip("torch")
+    pos, basis = H.kat_lattice()
+    n = len(pos)
+    dev = torch.device(f"cuda:{ctx.device}")
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut())
+    nl.rebuild(pos)
+    t = torch.zeros(n, dtype=torch.uint32, device=dev)
+    f = torch.zeros(n, dtype=torch.uint8, device=dev)
+    g = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    wen_gpu.gradient(t, f, nl, g)
+    torch.cuda.synchronize()
+    assert np.linalg.norm(g[0].cpu().numpy() - H.KAT_G0) < 1e-10
+    with pytest.raises(fa.FlyError):   # FP32 output: would receive 3n doubles
+        wen_gpu.gradient(t, f, nl, torch.zeros((n, 3), dtype=torch.float32, device=dev))
+    with pytest.raises(fa.FlyError):   # non-contiguous view
+        wen_gpu.gradient(t, f, nl, torch.zeros((3, n), dtype=torch.float64, device=dev).t())
+    with pytest.raises(fa.FlyError):   # too few elements
+        wen_gpu.gradient(t, f, nl, torch.zeros((n - 1, 3), dtype=torch.float64, device=dev))
+    with pytest.raises(fa.FlyError):   # int64 types
+        wen_gpu.gradient(t.to(torch.int64), f, nl, g)
+    # Out-of-range device TypeIDs are caught by the kernels and reported at the next synchronisation.
+    tb = t.clone()
+    tb[5] = 11
+    with pytest.raises(fa.FlyError):
+        wen_gpu.energy(tb, f, nl)
+    E = wen_gpu.energy(t, f, nl)
+    assert abs(E - H.KAT_E0) <= 1e-10 * abs(H.KAT_E0)
