@@ -73,28 +73,20 @@ def cpu_baseline(sys_, path, target_s=12.0):
                       f"one rebuild, {el:.1f} s, OpenMP static schedule over {threads} threads"}
 
 
-def perturbed_starts(sys_, centres, radii, seed, stddev=0.6, r_pert=4.0):
-    """Master::perturb-style starts (find.cpp:1147-1214): every mobile atom within r_pert of the centre gets
-    a Gaussian kick scaled by (1 - r/r_pert) and a standard-normal axis component, the centre a full kick;
-    the axis is normalised.  Atoms farther than the search radius are frozen."""
-    rng = np.random.default_rng(seed)
-    L = np.diag(sys_.basis)
-    starts = np.repeat(sys_.pos[None], len(centres), axis=0)
-    axes = np.zeros_like(starts)
-    for k, (c, R) in enumerate(zip(centres, radii)):
-        d = sys_.pos - sys_.pos[c]
+def explicit_starts(fa, s, centres, radii, prng):
+    """The starts flyg_batch_submit_perturbed draws (flyg_perturb with each search's generator state and
+    the atoms beyond its radius frozen), as full arrays for the plain form and the CPU oracle."""
+    L = np.diag(s.basis)
+    starts, axes, frozen = [], [], []
+    for c, R, st in zip(centres, radii, prng):
+        d = s.pos - s.pos[c]
         d -= L * np.round(d / L)
-        r = np.sqrt(np.einsum("ij,ij->i", d, d))
-        sd = -1.0
-        while sd < 0:
-            sd = rng.normal(stddev, stddev / 3.0)
-        near = np.nonzero((r < r_pert) & (r <= R))[0]
-        for j in near:
-            scale = 1.0 if j == c else (1.0 - r[j] / r_pert)
-            starts[k, j] += rng.normal(0.0, sd, 3) * scale
-            axes[k, j] += rng.normal(0.0, 1.0, 3)
-        axes[k] /= np.linalg.norm(axes[k])
-    return starts, axes
+        f = (np.einsum("ij,ij->i", d, d) > R * R).astype(np.uint8)
+        x, ax, _ = fa.perturb(fa.Box(s.basis), s.pos, f, int(c), st.copy())
+        starts.append(x)
+        axes.append(ax)
+        frozen.append(f)
+    return starts, axes, frozen
 
 
 WEN = os.path.join(ROOT, "tests", "golden", "wen.eam.fs")   # the reference's Fe potential (test fixture)
@@ -122,7 +114,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     cand = systems.defect_adjacent_atoms(s)
     centres = rng.choice(cand, n_search, replace=True)
     radii = rng.uniform(4.0, 9.0, n_search)
-    starts, axes = perturbed_starts(s, centres, radii, seed=34)
+    prng = np.random.default_rng(34).integers(1, 2 ** 63, size=(n_search, 4), dtype=np.uint64)
     box = fa.Box(s.basis)
     dimer = dict(f2norm=1e-3)
     out = {"workload": f"bcc Fe 20^3 + 20 vacancies + 40 interstitials ({s.n} atoms) relaxed to min0, "
@@ -132,13 +124,13 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     for name, prec in (("batched_mixed", fa.MIXED), ("batched_fp64", fa.FP64)):
         log(f"search leg: {name} (autotune run + timed run of {n_search} searches)")
         b = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
-        ids = b.submit(centres, radii, starts, axes)
+        ids = b.submit_perturbed(centres, radii, prng.copy())
         t0 = time.perf_counter()
         b.run()   # untimed: the first run of a batch includes the shape autotuner
         first = time.perf_counter() - t0
         T, L = b.shape()
         b2 = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec, block_threads=T, lanes_per_atom=L)
-        ids2 = b2.submit(centres, radii, starts, axes)
+        ids2 = b2.submit_perturbed(centres, radii, prng.copy())
         t0 = time.perf_counter()
         b2.run()
         el = time.perf_counter() - t0
@@ -169,7 +161,7 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     T, L = out["batched_mixed"]["shape"]
     bm = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=fa.MIXED, block_threads=T, lanes_per_atom=L)
     bm.set_mechanisms(minimiser=dict(f2norm=MECH_F2NORM))
-    idm = bm.submit(centres, radii, starts, axes)
+    idm = bm.submit_perturbed(centres, radii, prng.copy())
     t0 = time.perf_counter()
     bm.run()
     el = time.perf_counter() - t0
@@ -183,19 +175,16 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
     log(f"search leg: plain FP64 sample of {plain_sample}")
     dim = fa.Dimer(ctx, box, dimer=dimer)
-    L = np.diag(s.basis)
+    starts, axes, frozen = explicit_starts(fa, s, centres[:plain_sample], radii[:plain_sample], prng[:plain_sample])
     t0 = time.perf_counter()
     plain = []
     for k in range(plain_sample):
-        d = s.pos - s.pos[centres[k]]
-        d -= L * np.round(d / L)
-        fz = (np.einsum("ij,ij->i", d, d) > radii[k] ** 2).astype(np.uint8)
-        plain.append(dim.find_sp(starts[k], axes[k], s.types, fz, s.pos, pot))
+        plain.append(dim.find_sp(starts[k], axes[k], s.types, frozen[k], s.pos, pot))
     el = time.perf_counter() - t0
     out["plain_fp64_gpu"] = {"sample": plain_sample, "seconds": el, "searches_per_s": plain_sample / el,
                              "exits_match_batched_fp64": int(sum(1 for k in range(plain_sample)
                                                                  if plain[k]["exit"] == f[k]["exit"]))}
-    return out, (s, centres, radii, starts, axes, dimer, m)
+    return out, (s, centres, radii, prng, dimer, m)
 
 
 def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
@@ -247,7 +236,7 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
     rng = np.random.default_rng(55)
     centres = serve.sample_centres(s, n_search, rng)
     radii = rng.uniform(4.0, 9.0, n_search)
-    kicks = [serve.sparse_kick(s, int(c), float(R), rng) for c, R in zip(centres, radii)]
+    prng = rng.integers(1, 2 ** 63, size=(n_search, 4), dtype=np.uint64)
     srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3), max_batch=n_search, streams=1)
     box = fa.Box(s.basis)
     t0 = time.perf_counter()
@@ -262,10 +251,10 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
         return [got[int(i)] for i in ids]
 
     log("1M-atom leg: autotune on 64 searches")
-    drain(srv.submit(sysid, centres[:64], radii[:64], kicks[:64]))
+    drain(srv.submit_perturbed(sysid, centres[:64], radii[:64], prng[:64].copy()))
     log(f"1M-atom leg: {n_search} searches")
     t0 = time.perf_counter()
-    res = drain(srv.submit(sysid, centres, radii, kicks))
+    res = drain(srv.submit_perturbed(sysid, centres, radii, prng.copy()))
     el = time.perf_counter() - t0
     st = srv.status()
     conv = [r for r in res if r["exit"] == 0]
@@ -302,18 +291,17 @@ def cpu_search_baseline(data_args, cpu_sample=16):
     barriers with the oracle's for the same starts."""
     from oracle import pyoracle as po
     import concurrent.futures as cf
-    s, centres, radii, starts, axes, dimer, mixed = data_args
+    import openfly_amd as fa
+    s, centres, radii, prng, dimer, mixed = data_args
     threads = os.cpu_count() or 1
     k = min(cpu_sample, threads)
     data = po.DataEAM(WEN)
     box = po.Box(s.basis)
-    L = np.diag(s.basis)
+    starts, axes, frozen = explicit_starts(fa, s, centres[:k], radii[:k], prng[:k])
 
     def one(i):
-        d = s.pos - s.pos[centres[i]]
-        d -= L * np.round(d / L)
-        fz = (np.einsum("ij,ij->i", d, d) > radii[i] ** 2).astype(np.uint8)
-        return po.find_sp(po.Box(s.basis), data, starts[i], axes[i], s.types, fz, s.pos, threads=1, dimer=dimer)
+        return po.find_sp(po.Box(s.basis), data, starts[i], axes[i], s.types, frozen[i], s.pos, threads=1,
+                          dimer=dimer)
 
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=k) as ex:

@@ -131,6 +131,29 @@ int main(int argc, char** argv) {
     double Emin = pot.energy(mn, nl, 1);
     std::printf("vacancy hop barrier %.6f eV\n", Esp - Emin);
     REQUIRE(Esp - Emin > 0.5 && Esp - Emin < 0.8);
+
+    // Rotor::eff_gradient at the saddle (call shape of dimer.cpp:64): negative curvature, |g_eff| = |g|
+    // (with count_frozen > 0 there is no translation projection, so g_eff is an exact reflection), and the
+    // list is back where it was (same gradient after the call).
+    saddle::Rotor rotor({});
+    neigh::List rnl(cell.box, rotor.r_cut(pot));
+    rnl.rebuild(sp, 1);
+    system::SoA<PotentialGradient> g_before(cell.atoms.size()), g_after(cell.atoms.size()), eff(cell.atoms.size());
+    pot.gradient(g_before, sp, rnl, 1);
+    system::SoA<Axis> axis(cell.atoms.size());
+    for (std::int64_t i = 0; i < cell.atoms.size(); ++i) axis(ax_, i) = Vec(out(ax_, i));
+    double curv = rotor.eff_gradient(eff, axis, sp, pot, rnl, 1, 1);
+    pot.gradient(g_after, sp, rnl, 1);
+    double ge = 0, gb = 0, dmax = 0;
+    for (std::size_t k = 0; k < eff[g_].size(); ++k) {
+      ge += eff[g_][k] * eff[g_][k];
+      gb += g_before[g_][k] * g_before[g_][k];
+      dmax = std::max(dmax, std::abs(g_after[g_][k] - g_before[g_][k]));
+    }
+    std::printf("rotor curvature %.6f, |g_eff| %.3e, |g| %.3e\n", curv, std::sqrt(ge), std::sqrt(gb));
+    REQUIRE(curv < 0);
+    REQUIRE(std::abs(std::sqrt(ge) - std::sqrt(gb)) < 1e-9 + 1e-6 * std::sqrt(gb));
+    REQUIRE(dmax < 1e-10);   // positions return to x up to the rounding of x + d - d
   }
 
   // ---- Spline::OutOfBounds surfaces as the reference's exception type ----

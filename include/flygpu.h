@@ -60,6 +60,7 @@ typedef struct flyg_list_s* flyg_list;
 typedef struct flyg_steplbfgs_s* flyg_steplbfgs;
 typedef struct flyg_lbfgs_s* flyg_lbfgs;
 typedef struct flyg_dimer_s* flyg_dimer;
+typedef struct flyg_rotor_s* flyg_rotor;
 typedef struct flyg_batch_s* flyg_batch;
 typedef struct flyg_server_s* flyg_server;
 
@@ -213,10 +214,24 @@ typedef struct {
   int rot_iters;   /* batched searches: rotor iterations */
   double device_ms; /* batched searches: device time its workgroup spent on the search (last launch) */
   double device_grad_ms; /* ... of which in gradient evaluations */
+  int retries;     /* batched searches started by Master::perturb: re-perturbations after a spline error */
 } flyg_search_stats;
 
 void flyg_dimer_default_options(flyg_dimer_options* opt);
 void flyg_rotor_default_options(flyg_rotor_options* opt);
+/* Rotor(Options) (rotor.hpp:67). */
+int flyg_rotor_create(flyg_ctx ctx, const flyg_rotor_options* opt, flyg_rotor* out);
+int flyg_rotor_destroy(flyg_rotor r);
+/* Rotor::r_cut(pot) (rotor.hpp:123): pot.r_cut() + 2 delta_r, the least cut-off of the list eff_gradient uses. */
+int flyg_rotor_r_cut(flyg_rotor r, flyg_eam eam, double* r_cut);
+/* Rotor::eff_gradient(out, inout, in, pot, nl, count_frozen, threads) -> curvature (rotor.hpp:105-111,
+ * rotor.cpp:49-164): the list must be ready (rebuilt or updated) at the positions, with a cut-off of at
+ * least flyg_rotor_r_cut; it is moved by +-delta_r along the axis and returned to the same state, never
+ * rebuilt.  axis (3n) is rotated towards the lowest curvature mode in place; eff (3n) receives
+ * g - 2 (g.n) n (or -(g.n) n at positive curvature without relax_in_convex), with its atom-mean removed
+ * when count_frozen == 0.  *curv: the curvature along the returned axis. */
+int flyg_rotor_eff_gradient(flyg_rotor r, flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen,
+                            int64_t n, double* axis, double* eff, int count_frozen, int where, double* curv);
 /* Dimer(Options, Rotor::Options, Box) */
 int flyg_dimer_create(flyg_ctx ctx, const flyg_dimer_options* opt, const flyg_rotor_options* ropt,
                       const double basis[9], const int periodic[3], flyg_dimer* out);
@@ -252,6 +267,20 @@ int flyg_batch_destroy(flyg_batch b);
  * (a perturbed copy of min_pos and its dimer axis, as Master::perturb writes them). */
 int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const double* radius, const double* start_pos,
                       const double* start_axis, int64_t* ids);
+/* flyg_batch_submit with Dimer::find_sp's saddle-history test (dimer.cpp:102-131): every hist_check_freq
+ * steps a search whose direction from the minimum (x - min_pos) is within theta_tol of the direction to
+ * one of the n_hist known saddles (hist_pos: n_hist * 3n full positions, shared by these searches) stops
+ * with FLYG_EXIT_COLLISION.  theta_tol = 0 or n_hist = 0: no test. */
+int flyg_batch_submit_ex(flyg_batch b, int count, const int64_t* centre, const double* radius, const double* start_pos,
+                         const double* start_axis, double theta_tol, const double* hist_pos, int n_hist, int64_t* ids);
+/* Master::find_batch's searches (find.cpp:684-732): search k starts from flyg_perturb of min_pos around
+ * centre[k] (atoms beyond radius[k] frozen) drawn from its own Xoshiro256** state prng[4k .. 4k+3], which
+ * advances and is written back; a search that evaluates a spline outside its table is re-perturbed from
+ * the same stream and run again, at most three times (find.cpp:706-723; stats.retries counts them, and
+ * FLYG_EXIT_OOB reports the fourth failure, which the reference throws).  History as flyg_batch_submit_ex. */
+int flyg_batch_submit_perturbed(flyg_batch b, int count, const int64_t* centre, const double* radius, uint64_t* prng,
+                                double stddev, double r_pert, double theta_tol, const double* hist_pos, int n_hist,
+                                int64_t* ids);
 /* Run every queued search to completion on the device. */
 int flyg_batch_run(flyg_batch b);
 /* Results of search `id`: exit code / stats (energy = FP64 E(sp) - E(min0), the barrier), and the full
@@ -343,6 +372,11 @@ int flyg_server_remove_system(flyg_server s, int64_t sys_id);
 int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* centre, const double* radius,
                        const int64_t* kick_offsets, const int64_t* kick_atoms, const double* kick_pos,
                        const double* kick_axis, int64_t* ids);
+/* Queue `count` searches on a system started by Master::perturb, as flyg_batch_submit_perturbed: search k
+ * draws from its own Xoshiro256** state prng[4k .. 4k+3] (advanced and written back) and is re-perturbed
+ * after a spline error, at most three times. */
+int flyg_server_submit_perturbed(flyg_server s, int64_t sys_id, int count, const int64_t* centre, const double* radius,
+                                 uint64_t* prng, double stddev, double r_pert, int64_t* ids);
 /* Harvest finished launches and start queued work on free streams; waits up to wait_ms for at least one
  * search to finish.  Writes up to `cap` finished search ids. */
 int flyg_server_progress(flyg_server s, double wait_ms, int64_t* done_ids, int cap, int* n_done);
@@ -356,6 +390,23 @@ int flyg_server_set_mechanisms(flyg_server s, const flyg_mech_options* opt);
 /* Mechanism of a finished search; rev / fwd: 3 per mobile atom (the order of flyg_server_result's atoms). */
 int flyg_server_mechanism(flyg_server s, int64_t id, flyg_mechanism* mech, double* rev, double* fwd);
 int flyg_server_status(flyg_server s, flyg_server_state* state);
+
+/* ---------------------------------------------------------------------------------------------------
+ * Search starts: Master::perturb (find.cpp:1147-1214) and find_batch's history tolerance (find.cpp:692).
+ * ------------------------------------------------------------------------------------------------- */
+/* Master::perturb of `pos` around atom `centre` (host arrays, orthorhombic box): sd ~ N(stddev, stddev/3)
+ * redrawn until positive; each non-frozen atom at r < r_pert from the centre (every periodic image, in
+ * ascending (index, r) order) moves by N(0, sd) (1 - r/r_pert) per component and gets N(0, 1) axis
+ * components; the centre moves by N(0, sd) and gets N(0, 1) axis components; the axis is normalised.
+ * Gaussians come from the Marsaglia polar method on Xoshiro256** (random.hpp:96-161) with the state
+ * prng[4], which advances (a later call continues the stream, as Master's per-thread generator does).
+ * out_pos / out_axis: 3n each; *sd (may be NULL) receives the sampled standard deviation. */
+int flyg_perturb(const double basis[9], const int periodic[3], const double* pos, const uint8_t* frozen, int64_t n,
+                 int64_t centre, double stddev, double r_pert, uint64_t prng[4], double* out_pos, double* out_axis,
+                 double* sd);
+/* Master::find_batch's collision angle after `tot` searches at one site: ((30 - 2.5) e^(-0.02 tot) + 2.5)
+ * degrees, in radians (find.cpp:692). */
+int flyg_theta_tol(int tot, double* theta_tol);
 
 #ifdef __cplusplus
 }

@@ -578,12 +578,35 @@ namespace fly {
         bool relax_in_convex = true;
         bool debug = false;
       };
-      explicit Rotor(Options const& opt) : m_opt(opt) {}
+      explicit Rotor(Options const& opt) : m_opt(opt), m_ctx(gpu::context()) {
+        flyg_rotor_options r{opt.n, opt.iter_max_rot, opt.delta_r, opt.theta_tol, opt.relax_in_convex ? 1 : 0};
+        gpu::check(flyg_rotor_create(m_ctx, &r, &m_h), m_ctx);
+      }
+      Rotor(Rotor const&) = delete;
+      ~Rotor() {
+        if (m_h) flyg_rotor_destroy(m_h);
+      }
+
+      // Rotor::eff_gradient (rotor.hpp:105-111): nl ready at the positions, cut-off >= r_cut(pot); the
+      // list is moved by +-delta_r and returned to the same state.  Returns the curvature along the axis.
+      auto eff_gradient(system::SoA<PotentialGradient&> out, system::SoA<Axis&> inout,
+                        system::SoA<TypeID const&, Frozen const&> in, potential::Generic& pot, neigh::List& nl,
+                        int count_frozen, int = 1) -> double {
+        if (in.size() != out.size() || in.size() != inout.size()) throw RuntimeError("Rotor size mismatch");
+        double curv = 0;
+        gpu::check(flyg_rotor_eff_gradient(m_h, pot.handle(), nl.handle(), in[id_].data(), in[fzn_].data(), in.size(),
+                                           inout[ax_].data(), out[g_].data(), count_frozen, FLYG_HOST, &curv),
+                   m_ctx);
+        return curv;
+      }
+
       auto r_cut(potential::Generic const& pot) const noexcept { return pot.r_cut() + m_opt.delta_r * 2; }
       Options const& options() const { return m_opt; }
 
     private:
       Options m_opt;
+      flyg_ctx m_ctx;
+      flyg_rotor m_h = nullptr;
     };
 
     class Dimer {

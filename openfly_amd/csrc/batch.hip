@@ -14,6 +14,7 @@
 #include <tuple>
 #include <type_traits>
 
+#include "perturb.hpp"
 #include "search_kernel.hpp"
 
 
@@ -114,6 +115,7 @@ namespace flyg {
       int periodic[3];
       BoxDesc box{};
       std::vector<double> base;       // canonical, strictly inside [0, L)
+      std::vector<double> raw;        // the caller's minimum as given (perturbations start from it)
       std::vector<uint32_t> types;
       HostGrid grid;
       DBuf<double4> d_base;
@@ -144,6 +146,7 @@ namespace flyg {
               "batched searches need a fully periodic orthorhombic box");
       require(n < MAX_ATOMS, "too many atoms");
       S->base.resize(3 * n);
+      S->raw.assign(min_pos, min_pos + 3 * n);
       S->types.assign(types, types + n);
       S->single_type = static_cast<int>(types[0]);
       for (long i = 1; i < n; ++i)
@@ -204,9 +207,27 @@ namespace flyg {
       std::vector<unsigned> ent;    // packed (shift, local)
     };
 
+    // Known saddles shared by the searches of one submission (Master's per-site cache, find.cpp:692-732):
+    // sp_k - min over all atoms (the caller's frame) and the norms of those vectors.
+    struct HistGroup {
+      int nh = 0;
+      long n3 = 0;
+      std::vector<double> d;
+      std::vector<double> norm;
+    };
+
     struct Pending {
       int64_t id;
       int launches = 0;
+      double cos_tol = 2.0;               // cos(theta_tol) of the history test (no collision above 1)
+      std::shared_ptr<HistGroup> hist;    // may be null
+      // Master::perturb recipe of the start (re-perturbed after a spline-out-of-bounds failure).
+      bool perturbed = false;
+      Xoshiro256 prng{{0, 0, 0, 0}};
+      double stddev = 0, r_pert = 0;
+      int retries = 0;
+      double skin0 = 0;                   // the skin and mechanism flag the start was queued with
+      bool mech = false;
       SysState* sys = nullptr;
       int64_t centre;
       double radius;
@@ -291,6 +312,53 @@ namespace flyg {
       P.skin_f = std::min(cap, std::max(std::max(skin, mech_f), d1 + 1.0));
       P.skin_m = std::min(cap, std::max(std::max(std::max(2.0 * skin, mech_m), P.skin_f), d1 + d2 + 1.5));
       return P;
+    }
+
+    // A search started from Master::perturb of the caller's minimum (find.cpp:700-711): the mobile set
+    // decides which atoms are frozen, the cell grid supplies the candidates near the centre, and prng is
+    // the search's own generator (advanced; a re-perturbation continues it).
+    Pending make_perturbed(SysState& S, double rc, int64_t id, int64_t centre, double radius, double skin, bool mech,
+                           Xoshiro256 prng, double stddev, double r_pert) {
+      require(centre >= 0 && centre < S.n, "centre out of range");
+      std::vector<int> mob;
+      mobile_set(S, centre, radius, mob);
+      std::vector<long> cand;
+      S.grid.query(&S.base[3 * centre], r_pert + 0.5, [&](int j, int, double) { cand.push_back(j); });
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      Kick k;
+      perturb(S.box, S.raw.data(), S.n, centre, &cand,
+              [&](long j) { return !std::binary_search(mob.begin(), mob.end(), static_cast<int>(j)); }, stddev, r_pert,
+              prng, k);
+      Pending P = make_pending(S, rc, id, centre, radius, skin, k.pos.data(), k.axis.data(), k.atoms.data(),
+                               static_cast<long>(k.atoms.size()), mech);
+      P.perturbed = true;
+      P.prng = prng;
+      P.stddev = stddev;
+      P.r_pert = r_pert;
+      P.skin0 = skin;
+      P.mech = mech;
+      return P;
+    }
+
+    std::shared_ptr<HistGroup> make_hist(SysState const& S, double const* hist_pos, int n_hist) {
+      if (n_hist <= 0) return nullptr;
+      require(hist_pos != nullptr, "null history");
+      auto H = std::make_shared<HistGroup>();
+      H->nh = n_hist;
+      H->n3 = 3 * S.n;
+      H->d.resize(static_cast<std::size_t>(n_hist) * H->n3);
+      H->norm.resize(n_hist);
+      for (int q = 0; q < n_hist; ++q) {
+        double nn = 0;
+        for (long i = 0; i < H->n3; ++i) {
+          double v = hist_pos[q * H->n3 + i] - S.raw[i];
+          H->d[q * H->n3 + i] = v;
+          nn += v * v;
+        }
+        H->norm[q] = std::sqrt(nn);
+      }
+      return H;
     }
 
     // Region of a search: mobile atoms, then S1 (frozen atoms within rF = r_c + skin_f of a mobile atom),
@@ -412,6 +480,8 @@ namespace flyg {
       d.shrink_trust = o.shrink_trust;
       d.delta_r = r.delta_r;
       d.theta_tol = r.theta_tol;
+      d.hist_check_freq = std::max(1, o.hist_check_freq);
+      d.use_history = o.use_history;
       return d;
     }
 
@@ -502,8 +572,10 @@ namespace flyg {
       std::size_t b = 4096;
       for (auto const& P : work) {
         Region const& g = *P.region;
+        int const nh = P.hist ? P.hist->nh : 0;
         b += sizeof(SearchHdr) + static_cast<std::size_t>(g.nr) * (4 + 4 + 4 + 32 + 32 + 1 + 8 + 8) + g.ent.size() * 8 +
-             static_cast<std::size_t>(3 * g.nm) * 8 * (4 + nvec(dopt.n, dopt.rn)) + sizeof(SearchOut) + 3 * 256;
+             static_cast<std::size_t>(3 * g.nm) * 8 * (4 + nh + nvec(dopt.n, dopt.rn)) + 8 * nh + sizeof(SearchOut) +
+             3 * 256;
       }
       return b;
     }
@@ -514,7 +586,7 @@ namespace flyg {
       require(!F.busy, "internal: flight already in use");
       F.work = std::move(work);
       long const ns = static_cast<long>(F.work.size());
-      long nreg = 0, nrow = 0, nent = 0, n3 = 0, nws = 0;
+      long nreg = 0, nrow = 0, nent = 0, n3 = 0, nws = 0, nhv = 0, nhn = 0;
       std::vector<SearchHdr> hdr(ns);
       for (long k = 0; k < ns; ++k) {
         Pending const& P = F.work[k];
@@ -537,6 +609,12 @@ namespace flyg {
         h.natoms = P.sys->n;
         h.centre_local = static_cast<int>(std::lower_bound(P.mobile.begin(), P.mobile.end(), static_cast<int>(P.centre)) -
                                           P.mobile.begin());
+        h.n_hist = P.hist ? P.hist->nh : 0;
+        h.hist_off = nhv;
+        h.hn_off = nhn;
+        h.cos_tol = P.cos_tol;
+        nhv += 3L * g.nm * h.n_hist;
+        nhn += h.n_hist;
         nreg += g.nr;
         nrow += g.nr + 1;
         nent += static_cast<long>(g.ent.size());
@@ -547,7 +625,8 @@ namespace flyg {
       F.n3 = n3;
       Layout in;
       std::size_t o_hdr = in.add<SearchHdr>(ns), o_gidx = in.add<int>(nreg), o_row = in.add<int>(nrow),
-                  o_ent = in.add<unsigned>(nent), o_sx = in.add<double>(n3), o_sax = in.add<double>(n3);
+                  o_ent = in.add<unsigned>(nent), o_sx = in.add<double>(n3), o_sax = in.add<double>(n3),
+                  o_hv = in.add<double>(nhv), o_hn = in.add<double>(nhn);
       Layout sc;
       std::size_t o_cur = sc.add<double4>(nreg), o_base = sc.add<double4>(nreg), o_type = sc.add<uint8_t>(nreg),
                   o_rhof = sc.add<double>(nreg), o_fp = sc.add<double>(nreg), o_act = sc.add<unsigned>(nent),
@@ -590,6 +669,14 @@ namespace flyg {
         e0 = e1;
         std::copy(P.x.begin(), P.x.end(), hx + hdr[k].v3_off);
         std::copy(P.ax.begin(), P.ax.end(), ha + hdr[k].v3_off);
+        if (hdr[k].n_hist > 0) {   // the known saddles' components on this search's mobile atoms
+          HistGroup const& H = *P.hist;
+          double* hv = reinterpret_cast<double*>(hb + o_hv) + hdr[k].hist_off;
+          for (int q = 0; q < H.nh; ++q)
+            for (int i = 0; i < g.nm; ++i)
+              for (int c = 0; c < 3; ++c) hv[(static_cast<long>(q) * g.nm + i) * 3 + c] = H.d[q * H.n3 + 3L * P.mobile[i] + c];
+          std::copy(H.norm.begin(), H.norm.end(), reinterpret_cast<double*>(hb + o_hn) + hdr[k].hn_off);
+        }
       }
       F.d_in.reserve(grow(in.size, F.d_in.cap));
       F.d_scr.reserve(grow(sc.size, F.d_scr.cap));
@@ -615,6 +702,8 @@ namespace flyg {
       A.ws = reinterpret_cast<double*>(ds + o_ws);
       A.start_x = reinterpret_cast<double const*>(di + o_sx);
       A.start_ax = reinterpret_cast<double const*>(di + o_sax);
+      A.hist = reinterpret_cast<double const*>(di + o_hv);
+      A.hnorm = reinterpret_cast<double const*>(di + o_hn);
       char* hb_out = F.h_out.p;
       std::memset(hb_out + o_flags, 0, ns * sizeof(int));
       A.done = reinterpret_cast<int*>(hb_out + o_flags);
@@ -681,6 +770,15 @@ namespace flyg {
         long off = F.v3off[k];
         ++P.launches;
         double cap = P.sys->skin_cap(rc);
+        // Spline out of bounds: re-perturb and run again, at most three times (find.cpp:706-723).
+        if (o.exit_code == EXIT_OOB && P.perturbed && P.retries < 3) {
+          Pending q = make_perturbed(*P.sys, rc, P.id, P.centre, P.radius, P.skin0, P.mech, P.prng, P.stddev, P.r_pert);
+          q.retries = P.retries + 1;
+          q.cos_tol = P.cos_tol;
+          q.hist = P.hist;
+          requeue.push_back(std::move(q));
+          continue;
+        }
         if (o.exit_code == EXIT_REGION && (P.skin_f < cap || P.skin_m < cap)) {
           Pending q = P;
           q.skin_f = std::min(cap, std::max(q.skin_f * 1.5, o.max_disp + 1.0));
@@ -699,6 +797,7 @@ namespace flyg {
         R.st.energy = o.barrier;
         R.st.max_disp = o.max_disp;
         R.st.launches = P.launches;
+        R.st.retries = P.retries;
         R.st.rot_iters = o.rot_iters;
         R.st.device_ms = static_cast<double>(o.ticks) / wall_clock_khz();
         R.st.device_grad_ms = static_cast<double>(o.grad_ticks) / wall_clock_khz();
@@ -923,7 +1022,7 @@ namespace {
     } else {
       flyg_rotor_default_options(&r);
     }
-    require(o.n > 0 && o.n <= 32 && r.n > 0 && r.n <= 32, "batched searches support L-BFGS histories up to 12");
+    require(o.n > 0 && o.n <= 32 && r.n > 0 && r.n <= 32, "batched searches support L-BFGS histories up to 32");
   }
 }  // namespace
 
@@ -967,16 +1066,53 @@ int flyg_batch_destroy(flyg_batch b) {
 
 int flyg_batch_submit(flyg_batch b, int count, const int64_t* centre, const double* radius, const double* start_pos,
                       const double* start_axis, int64_t* ids) {
+  return flyg_batch_submit_ex(b, count, centre, radius, start_pos, start_axis, 0.0, nullptr, 0, ids);
+}
+
+int flyg_batch_submit_ex(flyg_batch b, int count, const int64_t* centre, const double* radius, const double* start_pos,
+                         const double* start_axis, double theta_tol, const double* hist_pos, int n_hist, int64_t* ids) {
   return bguard(b->ctx, [&] {
     require(count >= 0 && centre && radius && start_pos && start_axis, "null argument");
+    require(n_hist >= 0, "negative history length");
     long const n = b->sys->n;
+    auto H = make_hist(*b->sys, hist_pos, n_hist);
+    std::vector<Pending> add;
     for (int k = 0; k < count; ++k) {
-      int64_t id = static_cast<int64_t>(b->results.size());
-      b->pending.push_back(make_pending(*b->sys, b->eam->tab.rc, id, centre[k], radius[k], b->bopt.list_skin,
-                                        start_pos + 3 * n * static_cast<long>(k),
-                                        start_axis + 3 * n * static_cast<long>(k), nullptr, 0, b->mopt.enabled != 0));
+      add.push_back(make_pending(*b->sys, b->eam->tab.rc, static_cast<int64_t>(b->results.size()) + k, centre[k],
+                                 radius[k], b->bopt.list_skin, start_pos + 3 * n * static_cast<long>(k),
+                                 start_axis + 3 * n * static_cast<long>(k), nullptr, 0, b->mopt.enabled != 0));
+      add.back().cos_tol = std::cos(theta_tol);
+      add.back().hist = H;
+    }
+    for (int k = 0; k < count; ++k) {
+      if (ids) ids[k] = add[k].id;
+      b->pending.push_back(std::move(add[k]));
       b->results.emplace_back();
-      if (ids) ids[k] = id;
+    }
+  });
+}
+
+int flyg_batch_submit_perturbed(flyg_batch b, int count, const int64_t* centre, const double* radius, uint64_t* prng,
+                                double stddev, double r_pert, double theta_tol, const double* hist_pos, int n_hist,
+                                int64_t* ids) {
+  return bguard(b->ctx, [&] {
+    require(count >= 0 && centre && radius && prng, "null argument");
+    require(n_hist >= 0, "negative history length");
+    auto H = make_hist(*b->sys, hist_pos, n_hist);
+    std::vector<Pending> add;
+    std::vector<Xoshiro256> gens;
+    for (int k = 0; k < count; ++k) {
+      Xoshiro256 g{{prng[4 * k], prng[4 * k + 1], prng[4 * k + 2], prng[4 * k + 3]}};
+      add.push_back(make_perturbed(*b->sys, b->eam->tab.rc, static_cast<int64_t>(b->results.size()) + k, centre[k],
+                                   radius[k], b->bopt.list_skin, b->mopt.enabled != 0, g, stddev, r_pert));
+      add.back().cos_tol = std::cos(theta_tol);
+      add.back().hist = H;
+    }
+    for (int k = 0; k < count; ++k) {
+      for (int c = 0; c < 4; ++c) prng[4 * k + c] = add[k].prng.s[c];
+      if (ids) ids[k] = add[k].id;
+      b->pending.push_back(std::move(add[k]));
+      b->results.emplace_back();
     }
   });
 }
@@ -1247,6 +1383,30 @@ int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* 
                                  kick_axis + 3 * a, kick_atoms + a, b - a, s->mopt.enabled != 0));
     }
     for (int k = 0; k < count; ++k) {
+      int64_t id = s->next_id++;
+      s->owner[id] = sys_id;
+      if (ids) ids[k] = id;
+      s->queue.push_back(std::move(add[k]));
+      ++S.active;
+    }
+  });
+}
+
+int flyg_server_submit_perturbed(flyg_server s, int64_t sys_id, int count, const int64_t* centre, const double* radius,
+                                 uint64_t* prng, double stddev, double r_pert, int64_t* ids) {
+  return bguard(s->ctx, [&] {
+    auto it = s->systems.find(sys_id);
+    require(it != s->systems.end() && !it->second->retired, "unknown system id");
+    require(count >= 0 && centre && radius && prng, "null argument");
+    SysState& S = *it->second;
+    std::vector<Pending> add;
+    for (int k = 0; k < count; ++k) {
+      Xoshiro256 g{{prng[4 * k], prng[4 * k + 1], prng[4 * k + 2], prng[4 * k + 3]}};
+      add.push_back(make_perturbed(S, s->eam->tab.rc, s->next_id + k, centre[k], radius[k], s->sopt.list_skin,
+                                   s->mopt.enabled != 0, g, stddev, r_pert));
+    }
+    for (int k = 0; k < count; ++k) {
+      for (int c = 0; c < 4; ++c) prng[4 * k + c] = add[k].prng.s[c];
       int64_t id = s->next_id++;
       s->owner[id] = sys_id;
       if (ids) ids[k] = id;

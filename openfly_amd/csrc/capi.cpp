@@ -1,9 +1,12 @@
 // extern "C" boundary of libflygpu (declared in include/flygpu.h).  Every entry point catches C++
 // exceptions and returns a status code; the message is kept on the context.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <sstream>
 
+#include "perturb.hpp"
 #include "plain.hpp"
 
 namespace flyg {
@@ -506,6 +509,69 @@ int flyg_dimer_destroy(flyg_dimer d) {
   return FLYG_OK;
 }
 
+int flyg_rotor_create(flyg_ctx ctx, const flyg_rotor_options* opt, flyg_rotor* out) {
+  return guard(ctx, [&] {
+    require(out != nullptr, "null output handle");
+    auto r = std::make_unique<flyg_rotor_s>();
+    r->ctx = ctx;
+    r->d.ctx = ctx;
+    if (opt) {
+      r->d.ropt = *opt;
+    } else {
+      flyg_rotor_default_options(&r->d.ropt);
+    }
+    require(r->d.ropt.n > 0 && r->d.ropt.n <= 32, "L-BFGS history length must be in [1, 32]");
+    require(r->d.ropt.delta_r > 0, "delta_r must be positive");
+    ctx->retain();
+    *out = r.release();
+  });
+}
+
+int flyg_rotor_destroy(flyg_rotor r) {
+  if (!r) return FLYG_OK;
+  flyg_ctx_s* ctx = r->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete r;
+  ctx->release();
+  return FLYG_OK;
+}
+
+int flyg_rotor_r_cut(flyg_rotor r, flyg_eam eam, double* r_cut) {
+  return guard(r->ctx, [&] {
+    require(r_cut != nullptr, "null argument");
+    *r_cut = eam->tab.rc + r->d.ropt.delta_r * 2;
+  });
+}
+
+int flyg_rotor_eff_gradient(flyg_rotor r, flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen,
+                            int64_t n, double* axis, double* eff, int count_frozen, int where, double* curv) {
+  return guard(r->ctx, [&] {
+    require(eam->ctx == r->ctx && nl->ctx == r->ctx, "rotor, potential and list belong to different contexts");
+    require(n > 0 && axis && eff && curv, "null or empty argument");
+    flyg_ctx_s& ctx = *r->ctx;
+    uint32_t const* t;
+    uint8_t const* f;
+    stage_atoms(*eam, types, frozen, n, where, &t, &f);
+    long N = 3 * n;
+    double* ax = axis;
+    double* ef = eff;
+    flyg::DVec hax, hef;
+    if (where == FLYG_HOST) {
+      ax = hax.p(N);
+      ef = hef.p(N);
+      HIPCHK(hipMemcpyAsync(ax, axis, N * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    }
+    double c = rotor_eff_gradient(r->d, *nl, *eam, t, f, n, ax, ef, count_frozen);
+    if (where == FLYG_HOST) {
+      HIPCHK(hipMemcpyAsync(axis, ax, N * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+      HIPCHK(hipMemcpyAsync(eff, ef, N * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    }
+    check_errors(ctx);
+    *curv = c;
+  });
+}
+
 int flyg_dimer_find_sp(flyg_dimer d, flyg_eam eam, double* pos, double* axis, const uint32_t* types,
                        const uint8_t* frozen, int64_t n, const double* in_min, const double* hist_sp, int n_hist,
                        double theta_tol, int count_frozen, int where, flyg_search_stats* stats) {
@@ -545,6 +611,38 @@ int flyg_dimer_find_sp(flyg_dimer d, flyg_eam eam, double* pos, double* axis, co
     }
     check_errors(ctx);
     if (stats) *stats = st;
+  });
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Master::perturb
+// ---------------------------------------------------------------------------------------------------
+
+int flyg_perturb(const double basis[9], const int periodic[3], const double* pos, const uint8_t* frozen, int64_t n,
+                 int64_t centre, double stddev, double r_pert, uint64_t prng[4], double* out_pos, double* out_axis,
+                 double* sd) {
+  return guard(nullptr, [&] {
+    require(basis && periodic && pos && frozen && prng && out_pos && out_axis && n > 0, "null or empty argument");
+    BoxDesc box = make_box(basis, periodic);
+    Xoshiro256 g{{prng[0], prng[1], prng[2], prng[3]}};
+    Kick k;
+    perturb(box, pos, n, centre, nullptr, [&](long j) { return frozen[j] != 0; }, stddev, r_pert, g, k);
+    std::copy(pos, pos + 3 * n, out_pos);
+    std::fill(out_axis, out_axis + 3 * n, 0.0);
+    for (std::size_t q = 0; q < k.atoms.size(); ++q)
+      for (int c = 0; c < 3; ++c) {
+        out_pos[3 * k.atoms[q] + c] = k.pos[3 * q + c];
+        out_axis[3 * k.atoms[q] + c] = k.axis[3 * q + c];
+      }
+    for (int c = 0; c < 4; ++c) prng[c] = g.s[c];
+    if (sd) *sd = k.sd;
+  });
+}
+
+int flyg_theta_tol(int tot, double* theta_tol) {
+  return guard(nullptr, [&] {
+    require(theta_tol != nullptr, "null argument");
+    *theta_tol = ((30 - 2.5) * std::exp(-0.02 * tot) + 2.5) / 360. * 2. * M_PI;
   });
 }
 

@@ -463,13 +463,12 @@ namespace flyg {
     // energy = true: FP64 densities and pair energies (energies are always FP64); otherwise the density
     // pass of a gradient in the given precision.
     template <int L>
-    void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy, int precision = FLYG_FP64) {
+    void launch_density(flyg_eam_s& E, Shifts const& S, PairArgs const& A, bool energy, int precision, double4* pf) {
       hipStream_t st = E.ctx->stream;
       unsigned* err = E.ctx->dflags.get();
       bool multi = A.t0 < 0;
       bool mix = precision == FLYG_MIXED;
       unsigned g = grid_for(A.n, L);
-      double4* pf = E.posf.get();
       if (multi) {
         if (energy)
           k_density<L, true, true, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, E.eatom.get(), err);
@@ -488,7 +487,8 @@ namespace flyg {
     }
 
     template <int L>
-    void launch_force(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* grad, int precision) {
+    void launch_force(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* grad, int precision,
+                      double4 const* pf) {
       hipStream_t st = E.ctx->stream;
       unsigned* err = E.ctx->dflags.get();
       bool multi = A.t0 < 0;
@@ -496,14 +496,14 @@ namespace flyg {
       unsigned g = grid_for(A.n, L);
       if (multi) {
         if (mix)
-          k_force<L, true, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+          k_force<L, true, true><<<g, NT, 0, st>>>(E.tab, S, A, pf, grad, err);
         else
-          k_force<L, true, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+          k_force<L, true, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, grad, err);
       } else {
         if (mix)
-          k_force<L, false, true><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+          k_force<L, false, true><<<g, NT, 0, st>>>(E.tab, S, A, pf, grad, err);
         else
-          k_force<L, false, false><<<g, NT, 0, st>>>(E.tab, S, A, E.posf.get(), grad, err);
+          k_force<L, false, false><<<g, NT, 0, st>>>(E.tab, S, A, pf, grad, err);
       }
     }
   }  // namespace
@@ -513,7 +513,7 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.posf.reserve(nl.n);
     E.eatom.reserve(nl.n);
-    launch_density<LANES>(E, shifts_of(nl.box), pair_args(E, nl, types, frozen), true);
+    launch_density<LANES>(E, shifts_of(nl.box), pair_args(E, nl, types, frozen), true, FLYG_FP64, E.posf.get());
     HIPCHK(hipGetLastError());
     dev_sum(*E.ctx, E.eatom.get(), nl.n, E.ctx->dscal.get());
     HIPCHK(hipMemcpyAsync(E.ctx->hscal, E.ctx->dscal.get(), sizeof(double), hipMemcpyDeviceToHost, E.ctx->stream));
@@ -524,19 +524,21 @@ namespace flyg {
   }
 
   namespace {
-    void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L) {
+    // Density then force pass; pf holds the pass-1 output (positions + F') of every atom.
+    void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L,
+                       double4* pf) {
       switch (L) {
         case 4:
-          launch_density<4>(E, S, A, false, precision);
-          launch_force<4>(E, S, A, d_grad, precision);
+          launch_density<4>(E, S, A, false, precision, pf);
+          launch_force<4>(E, S, A, d_grad, precision, pf);
           break;
         case 16:
-          launch_density<16>(E, S, A, false, precision);
-          launch_force<16>(E, S, A, d_grad, precision);
+          launch_density<16>(E, S, A, false, precision, pf);
+          launch_force<16>(E, S, A, d_grad, precision, pf);
           break;
         default:
-          launch_density<8>(E, S, A, false, precision);
-          launch_force<8>(E, S, A, d_grad, precision);
+          launch_density<8>(E, S, A, false, precision, pf);
+          launch_force<8>(E, S, A, d_grad, precision, pf);
       }
     }
 
@@ -553,9 +555,9 @@ namespace flyg {
       int best = 8;
       float best_ms = 1e30f;
       for (int L : {4, 8, 16}) {
-        gradient_with(E, S, A, d_grad, precision, L);   // warm
+        gradient_with(E, S, A, d_grad, precision, L, E.posf.get());   // warm
         HIPCHK(hipEventRecord(a, st));
-        for (int r = 0; r < 3; ++r) gradient_with(E, S, A, d_grad, precision, L);
+        for (int r = 0; r < 3; ++r) gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
         HIPCHK(hipEventRecord(b, st));
         HIPCHK(hipEventSynchronize(b));
         float ms = 0;
@@ -589,7 +591,7 @@ namespace flyg {
       }
       HIPCHK(hipEventRecord(E.events[E.events_used].first, E.ctx->stream));
     }
-    gradient_with(E, S, A, d_grad, precision, L);
+    gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
     if (E.timing) HIPCHK(hipEventRecord(E.events[E.events_used++].second, E.ctx->stream));
     HIPCHK(hipGetLastError());
   }

@@ -226,13 +226,17 @@ namespace flyg {
 
   namespace {
 
-    void gradient_at(flyg_dimer_s& d, flyg_eam_s& e, uint32_t const* t, uint8_t const* f, double* out) {
-      eam_gradient(e, *d.nl, t, f, out, FLYG_FP64);
+    void gradient_at(flyg_dimer_s& d, flyg_list_s& nl, flyg_eam_s& e, uint32_t const* t, uint8_t const* f, double* out) {
+      eam_gradient(e, nl, t, f, out, FLYG_FP64);
       ++d.grads;
     }
 
-    // Returns the curvature; leaves |g_eff|^2 in d.slots[0].
-    double eff_gradient(flyg_dimer_s& d, flyg_eam_s& e, uint32_t const* t, uint8_t const* f, long n, int count_frozen) {
+  }  // namespace
+
+  // Rotor::eff_gradient on the list nl (ready at the positions; moved by +-delta_r and back, never rebuilt).
+  // Axis in/out in d.ax, effective gradient to d.eff.  Returns the curvature; leaves |g_eff|^2 in d.slots[0].
+  double eff_gradient(flyg_dimer_s& d, flyg_list_s& nl, flyg_eam_s& e, uint32_t const* t, uint8_t const* f, long n,
+                      int count_frozen) {
       flyg_ctx_s& ctx = *d.ctx;
       flyg_rotor_options const& o = d.ropt;
       long N = 3 * n;
@@ -250,10 +254,10 @@ namespace flyg {
       double const inv_n = 1.0 / static_cast<double>(n);
 
       d.rcore->clear();
-      gradient_at(d, e, t, f, g0);
+      gradient_at(d, nl, e, t, f, g0);
       vec(ctx, lin(N, dprev, ax, cst(-dr)));
-      list_update(*d.nl, dprev, n);
-      gradient_at(d, e, t, f, g1);
+      list_update(nl, dprev, n);
+      gradient_at(d, nl, e, t, f, g1);
 
       double curv = 0;
       for (int i = 0;; ++i) {
@@ -288,8 +292,8 @@ namespace flyg {
         vec(ctx, lin(N, axp, ax, cst(std::cos(theta_1)), theta, cst(std::sin(theta_1))));
         vec(ctx, lin(N, u, axp, cst(-dr), dprev, cst(-1)));
         vec(ctx, lin(N, dprev, axp, cst(-dr)));
-        list_update(*d.nl, u, n);
-        gradient_at(d, e, t, f, g1p);
+        list_update(nl, u, n);
+        gradient_at(d, nl, e, t, f, g1p);
         vec(ctx, with_dots(lin(N, u, g1p, cst(1), g0, cst(-1)), sl + 0, {{nullptr, axp}}));
         double c_x1 = read_scalar(ctx, sl + 0) / dr;
         double a_1 = (c_x0 - c_x1 + b_1 * std::sin(2 * theta_1)) / (1 - std::cos(2 * theta_1));
@@ -308,7 +312,7 @@ namespace flyg {
       }
       // Move the list back to the centre.
       vec(ctx, lin(N, u, dprev, cst(-1)));
-      list_update(*d.nl, u, n);
+      list_update(nl, u, n);
 
       vec(ctx, dots(N, sl + 0, {{g0, ax}}));
       double k = (!o.relax_in_convex && curv > 0) ? -1.0 : -2.0;
@@ -339,7 +343,23 @@ namespace flyg {
       return curv;
     }
 
-  }  // namespace
+  double rotor_eff_gradient(flyg_dimer_s& d, flyg_list_s& nl, flyg_eam_s& e, uint32_t const* t, uint8_t const* f,
+                            long n, double* d_axis, double* d_eff, int count_frozen) {
+    flyg_ctx_s& ctx = *d.ctx;
+    require(nl.built, "List used before rebuild");
+    require(nl.n == n, "rotor size mismatch");
+    require(nl.rc >= e.tab.rc + 2 * d.ropt.delta_r, "the list's cut-off is below the rotor's r_cut()");
+    long N = 3 * n;
+    if (!d.rcore) d.rcore = std::make_unique<DevLBFGS>(&ctx, d.ropt.n);
+    d.slots.reserve(16);
+    for (DVec* v : {&d.ax, &d.eff, &d.g0, &d.g1, &d.g1p, &d.dg, &d.delta_g, &d.axisp, &d.delta_prev, &d.u}) v->p(N);
+    copy(ctx, d.ax.get(), d_axis, N);
+    double curv = eff_gradient(d, nl, e, t, f, n, count_frozen);
+    copy(ctx, d_axis, d.ax.get(), N);
+    copy(ctx, d_eff, d.eff.get(), N);
+    check_flags(ctx);
+    return curv;
+  }
 
   int dimer_find_sp(flyg_dimer_s& d, flyg_eam_s& e, double* d_pos, double* d_axis, uint32_t const* t,
                     uint8_t const* f, long n, double const* d_in_min, double const* d_hist, int n_hist,
@@ -375,7 +395,7 @@ namespace flyg {
 
     int rebuilds = 0;
     list_rebuild(*d.nl, x, n);
-    double curv = eff_gradient(d, e, t, f, n, count_frozen);
+    double curv = eff_gradient(d, *d.nl, e, t, f, n, count_frozen);
     double mag_g = read_scalar(ctx, sl);
     check_flags(ctx);
     double trust = o.min_trust, acc = 0, convex_count = 0;
@@ -422,7 +442,7 @@ namespace flyg {
       } else {
         list_update(*d.nl, q, n);
       }
-      curv = eff_gradient(d, e, t, f, n, count_frozen);
+      curv = eff_gradient(d, *d.nl, e, t, f, n, count_frozen);
       convex_count = curv > 0 ? convex_count + 1 : 0;
       vec(ctx, dots(N, sl + 8, {{d.eff.get(), q}}));
       double two[2];

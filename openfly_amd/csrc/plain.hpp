@@ -74,7 +74,15 @@ struct flyg_dimer_s {
   long grads = 0;
 };
 
+// Rotor (rotor.hpp:42-139): the plain form's rotor state (the dimer's, without its own list).
+struct flyg_rotor_s {
+  flyg_ctx_s* ctx;
+  flyg_dimer_s d;
+};
+
 namespace flyg {
+  double rotor_eff_gradient(flyg_dimer_s& d, flyg_list_s& nl, flyg_eam_s& e, uint32_t const* t, uint8_t const* f,
+                            long n, double* d_axis, double* d_eff, int count_frozen);
   bool lbfgs_minimise(flyg_lbfgs_s& m, flyg_eam_s& e, double* d_pos, double* d_grad, uint32_t const* d_types,
                       uint8_t const* d_frozen, long n, int* iterations);
   int dimer_find_sp(flyg_dimer_s& d, flyg_eam_s& e, double* d_pos, double* d_axis, uint32_t const* d_types,

@@ -59,6 +59,11 @@ namespace flyg {
       double3 const* sS;      // the 27 image shifts of the box, by shift code
       long natoms;            // atoms in the system (centroids run over all of them)
       int centre_local;       // mobile index of the search's centre atom
+      // Known saddles of the history test (Dimer::find_sp's hist_sp, dimer.cpp:102-131): n_hist vectors
+      // sp_k - min over the mobile atoms (3*nm each, from hist_off) and their full-system norms (hn_off).
+      int n_hist;
+      long hist_off, hn_off;
+      double cos_tol;         // cos(theta_tol): a known saddle closer in angle than theta_tol is a collision
     };
 
     struct SearchOut {
@@ -74,7 +79,7 @@ namespace flyg {
     };
 
     struct DimOpt {
-      int n, max_steps, convex_max, rn, iter_max_rot, relax_in_convex;
+      int n, max_steps, convex_max, rn, iter_max_rot, relax_in_convex, hist_check_freq, use_history;
       double f2norm, proj_tol, max_trust, min_trust, grow_trust, shrink_trust, delta_r, theta_tol;
       // Mechanisms (Master::do_adj_min / saddle_2_mech): 0 = off.
       int mech, min_iter_max;
@@ -101,6 +106,8 @@ namespace flyg {
       double const* start_ax;
       double* out_x;
       double* out_ax;
+      double const* hist;        // known-saddle vectors of every search (SearchHdr::hist_off)
+      double const* hnorm;       // their norms (SearchHdr::hn_off)
       double* out_rev;           // mechanism minima (when mechanisms are on)
       double* out_fwd;
       SearchOut* out;
@@ -853,7 +860,7 @@ namespace flyg {
         return true;
       };
 
-      // Dimer::find_sp (dimer.cpp:28-171) without a history of known saddles.
+      // Dimer::find_sp (dimer.cpp:28-171).
       double mag_g = 0;
       if (!displacement_ok()) {
         ret = EXIT_REGION;
@@ -873,6 +880,23 @@ namespace flyg {
             ret = 1;
             steps = i;
             break;
+          }
+          // History test every hist_check_freq steps (dimer.cpp:102-131): the angle between x - min and
+          // sp_k - min below theta_tol for a known saddle k ends the search as a collision.  Frozen atoms
+          // sit at the minimum, so x - min is zero there and the dot product runs over the mobile atoms.
+          if (o.use_history && h.n_hist > 0 && i % o.hist_check_freq == 0) {
+            tm.lin(dg, n3, x, 1.0, xb, -1.0);
+            double const on = sqrt(tm.dot(dg, dg, n3));
+            bool hit = false;
+            for (int k = 0; k < h.n_hist && !hit; ++k) {
+              double const dt = tm.dot(dg, A.hist + h.hist_off + static_cast<long>(k) * n3, n3);
+              hit = dt / (A.hnorm[h.hn_off + k] * on) > h.cos_tol;
+            }
+            if (hit) {
+              ret = 3;
+              steps = i;
+              break;
+            }
           }
           double* q = tc.newton_step(tm, x, eff);
           double nrm = sqrt(tm.dot(q, q, n3));

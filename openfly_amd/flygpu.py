@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libflygpu.so")
 FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR, FLYG_BUSY = 0, 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 FP64, MIXED = 0, 1
-EXIT_NAMES = {0: "success", 1: "convex", 2: "iter_max", 3: "collision", 4: "uninit"}
+EXIT_NAMES = {0: "success", 1: "convex", 2: "iter_max", 3: "collision", 4: "uninit", 5: "region", 6: "spline OOB"}
 
 
 class FlyError(RuntimeError):
@@ -75,7 +75,8 @@ class RotorOptions(C.Structure):
 class SearchStats(C.Structure):
     _fields_ = [("exit_code", C.c_int), ("steps", C.c_int), ("grads", C.c_int64), ("rebuilds", C.c_int),
                 ("curv", C.c_double), ("energy", C.c_double), ("max_disp", C.c_double), ("launches", C.c_int),
-                ("rot_iters", C.c_int), ("device_ms", C.c_double), ("device_grad_ms", C.c_double)]
+                ("rot_iters", C.c_int), ("device_ms", C.c_double), ("device_grad_ms", C.c_double),
+                ("retries", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -162,6 +163,10 @@ _SIGS = {
     "flyg_dimer_create": [_vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), _d, C.POINTER(C.c_int),
                           C.POINTER(_vp)],
     "flyg_dimer_destroy": [_vp],
+    "flyg_rotor_create": [_vp, C.POINTER(RotorOptions), C.POINTER(_vp)],
+    "flyg_rotor_destroy": [_vp],
+    "flyg_rotor_r_cut": [_vp, _vp, _d],
+    "flyg_rotor_eff_gradient": [_vp, _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int, C.c_int, _d],
     "flyg_dimer_find_sp": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int, C.c_double, C.c_int,
                            C.c_int, C.POINTER(SearchStats)],
     "flyg_batch_default_options": [C.POINTER(BatchOptions)],
@@ -169,6 +174,12 @@ _SIGS = {
                           _d, C.POINTER(C.c_int), _d, _u32, C.c_int64, C.POINTER(_vp)],
     "flyg_batch_destroy": [_vp],
     "flyg_batch_submit": [_vp, C.c_int, _i64, _d, _d, _d, _i64],
+    "flyg_batch_submit_ex": [_vp, C.c_int, _i64, _d, _d, _d, C.c_double, _d, C.c_int, _i64],
+    "flyg_batch_submit_perturbed": [_vp, C.c_int, _i64, _d, C.POINTER(C.c_uint64), C.c_double, C.c_double,
+                                    C.c_double, _d, C.c_int, _i64],
+    "flyg_perturb": [_d, C.POINTER(C.c_int), _d, _u8, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                     C.POINTER(C.c_uint64), _d, _d, _d],
+    "flyg_theta_tol": [C.c_int, _d],
     "flyg_batch_run": [_vp],
     "flyg_batch_result": [_vp, C.c_int64, C.POINTER(SearchStats), _d, _d],
     "flyg_batch_region": [_vp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)],
@@ -185,6 +196,8 @@ _SIGS = {
     "flyg_server_add_system": [_vp, _d, C.POINTER(C.c_int), _d, _u32, C.c_int64, _i64],
     "flyg_server_remove_system": [_vp, C.c_int64],
     "flyg_server_submit": [_vp, C.c_int64, C.c_int, _i64, _d, _i64, _i64, _d, _d, _i64],
+    "flyg_server_submit_perturbed": [_vp, C.c_int64, C.c_int, _i64, _d, C.POINTER(C.c_uint64), C.c_double,
+                                     C.c_double, _i64],
     "flyg_server_progress": [_vp, C.c_double, _i64, C.c_int, C.POINTER(C.c_int)],
     "flyg_server_result": [_vp, C.c_int64, C.POINTER(SearchStats), C.POINTER(C.c_int), _i64, _d, _d],
     "flyg_server_release": [_vp, C.c_int64],
@@ -544,6 +557,43 @@ class LBFGS:
         return pos, grad, bool(failed.value), it.value
 
 
+class Rotor:
+    """saddle::Rotor(options): eff_gradient on a ready neighbour list (rotor.hpp:105-123)."""
+
+    def __init__(self, ctx: Context, **rotor):
+        self.ctx = ctx
+        self.ropt = rotor_options(**rotor)
+        h = _vp()
+        ctx.check(lib().flyg_rotor_create(ctx.h, C.byref(self.ropt), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.flyg_rotor_destroy(self.h)
+            self.h = None
+
+    def r_cut(self, pot: EAM):
+        r = C.c_double()
+        self.ctx.check(lib().flyg_rotor_r_cut(self.h, pot.h, C.byref(r)))
+        return r.value
+
+    def eff_gradient(self, axis, types, frozen, pot: EAM, nl: "List", count_frozen=None):
+        """(curvature, rotated axis, effective gradient); host arrays."""
+        n = nl.n
+        ax = np.ascontiguousarray(axis, dtype=np.float64).reshape(n, 3).copy()
+        t = np.ascontiguousarray(types, dtype=np.uint32).reshape(n)
+        f = np.ascontiguousarray(frozen, dtype=np.uint8).reshape(n)
+        if count_frozen is None:
+            count_frozen = int(f.sum())
+        eff = np.zeros((n, 3))
+        c = C.c_double()
+        self.ctx.check(lib().flyg_rotor_eff_gradient(self.h, pot.h, nl.h, C.c_void_p(t.ctypes.data),
+                                                     C.c_void_p(f.ctypes.data), n, C.c_void_p(ax.ctypes.data),
+                                                     C.c_void_p(eff.ctypes.data), int(count_frozen), HOST,
+                                                     C.byref(c)))
+        return c.value, ax, eff
+
+
 class Dimer:
     """saddle::Dimer(Options, Rotor::Options, Box)."""
 
@@ -606,15 +656,42 @@ class Batch:
             _lib.flyg_batch_destroy(self.h)
             self.h = None
 
-    def submit(self, centres, radii, start_pos, start_axis):
+    def _hist(self, hist):
+        if hist is None or len(hist) == 0:
+            return np.zeros(1), 0
+        h = np.ascontiguousarray(np.stack([np.asarray(x, dtype=np.float64).reshape(self.n, 3) for x in hist]))
+        return h, len(hist)
+
+    def submit(self, centres, radii, start_pos, start_axis, theta_tol=0.0, hist=None):
+        """Queue searches from explicit starts; `hist` (known saddles, full positions) with theta_tol adds
+        Dimer::find_sp's collision test."""
         c = np.ascontiguousarray(np.atleast_1d(centres), dtype=np.int64)
         r = np.ascontiguousarray(np.atleast_1d(radii), dtype=np.float64)
         k = len(c)
         sp = np.ascontiguousarray(start_pos, dtype=np.float64).reshape(k, self.n, 3)
         sa = np.ascontiguousarray(start_axis, dtype=np.float64).reshape(k, self.n, 3)
         ids = np.zeros(k, np.int64)
-        self.ctx.check(lib().flyg_batch_submit(self.h, k, c.ctypes.data_as(_i64), r.ctypes.data_as(_d),
-                                               sp.ctypes.data_as(_d), sa.ctypes.data_as(_d), ids.ctypes.data_as(_i64)))
+        h, nh = self._hist(hist)
+        self.ctx.check(lib().flyg_batch_submit_ex(self.h, k, c.ctypes.data_as(_i64), r.ctypes.data_as(_d),
+                                                  sp.ctypes.data_as(_d), sa.ctypes.data_as(_d), float(theta_tol),
+                                                  h.ctypes.data_as(_d), nh, ids.ctypes.data_as(_i64)))
+        return ids
+
+    def submit_perturbed(self, centres, radii, prng, stddev=0.6, r_pert=4.0, theta_tol=0.0, hist=None):
+        """Master::find_batch's searches: starts from Master::perturb with per-search Xoshiro256** states
+        prng (k x 4 uint64, advanced in place), re-perturbed up to three times after a spline error."""
+        c = np.ascontiguousarray(np.atleast_1d(centres), dtype=np.int64)
+        r = np.ascontiguousarray(np.atleast_1d(radii), dtype=np.float64)
+        k = len(c)
+        if not (isinstance(prng, np.ndarray) and prng.dtype == np.uint64 and prng.flags.c_contiguous and
+                prng.shape == (k, 4)):
+            raise FlyError("prng must be a C-contiguous (k, 4) uint64 array")
+        ids = np.zeros(k, np.int64)
+        h, nh = self._hist(hist)
+        self.ctx.check(lib().flyg_batch_submit_perturbed(self.h, k, c.ctypes.data_as(_i64), r.ctypes.data_as(_d),
+                                                         prng.ctypes.data_as(C.POINTER(C.c_uint64)), float(stddev),
+                                                         float(r_pert), float(theta_tol), h.ctypes.data_as(_d), nh,
+                                                         ids.ctypes.data_as(_i64)))
         return ids
 
     def run(self):
@@ -734,6 +811,21 @@ class Server:
                                                 kp.ctypes.data_as(_d), ka.ctypes.data_as(_d), ids.ctypes.data_as(_i64)))
         return ids
 
+    def submit_perturbed(self, sys_id, centres, radii, prng, stddev=0.6, r_pert=4.0):
+        """Searches started by Master::perturb; prng: C-contiguous (k, 4) uint64 states, advanced in place."""
+        c = np.ascontiguousarray(np.atleast_1d(centres), dtype=np.int64)
+        r = np.ascontiguousarray(np.atleast_1d(radii), dtype=np.float64)
+        k = len(c)
+        if not (isinstance(prng, np.ndarray) and prng.dtype == np.uint64 and prng.flags.c_contiguous and
+                prng.shape == (k, 4)):
+            raise FlyError("prng must be a C-contiguous (k, 4) uint64 array")
+        ids = np.zeros(k, np.int64)
+        self.ctx.check(lib().flyg_server_submit_perturbed(self.h, int(sys_id), k, c.ctypes.data_as(_i64),
+                                                          r.ctypes.data_as(_d),
+                                                          prng.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                          float(stddev), float(r_pert), ids.ctypes.data_as(_i64)))
+        return ids
+
     def progress(self, wait_ms=0.0, cap=4096):
         """Ids of searches that finished since the last call (waits up to wait_ms for one)."""
         buf = np.zeros(cap, np.int64)
@@ -783,3 +875,31 @@ class Server:
         st = ServerState()
         self.ctx.check(lib().flyg_server_status(self.h, C.byref(st)))
         return st.as_dict()
+
+
+def perturb(box: Box, pos, frozen, centre, prng, stddev=0.6, r_pert=4.0):
+    """Master::perturb (flyg_perturb): (positions, axis, sd); prng (4 uint64) advances in place."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    f = np.ascontiguousarray(frozen, dtype=np.uint8).reshape(n)
+    if not (isinstance(prng, np.ndarray) and prng.dtype == np.uint64 and prng.shape == (4,)):
+        raise FlyError("prng must be a (4,) uint64 array")
+    out = np.zeros((n, 3))
+    ax = np.zeros((n, 3))
+    sd = C.c_double()
+    b, p = box.args()
+    rc = lib().flyg_perturb(b, p, pos.ctypes.data_as(_d), f.ctypes.data_as(_u8), n, int(centre), float(stddev),
+                            float(r_pert), prng.ctypes.data_as(C.POINTER(C.c_uint64)), out.ctypes.data_as(_d),
+                            ax.ctypes.data_as(_d), C.byref(sd))
+    if rc:
+        _raise(None, rc)
+    return out, ax, sd.value
+
+
+def theta_tol(tot):
+    """Master::find_batch's collision angle (radians) after `tot` searches at one site."""
+    t = C.c_double()
+    rc = lib().flyg_theta_tol(int(tot), C.byref(t))
+    if rc:
+        _raise(None, rc)
+    return t.value

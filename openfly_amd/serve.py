@@ -27,7 +27,7 @@ class Request:
     system: systems.System   # the kMC system it came from (shared by that loop's requests)
     centres: np.ndarray
     radii: np.ndarray
-    kicks: list              # per search: (atoms, positions (m, 3), axis (m, 3))
+    prng: np.ndarray         # per search: Xoshiro256** state of its Master::perturb start (k x 4 uint64)
     sys_id: int = -1
     remaining: int = 0
     admitted: float = -1.0
@@ -78,32 +78,6 @@ def sample_centres(s: systems.System, k, rng, r=3.0):
     return out
 
 
-def sparse_kick(s: systems.System, centre, radius, rng, stddev=0.6, r_pert=4.0):
-    """Master::perturb-style start (find.cpp:1147-1214) in sparse form: every non-frozen atom within r_pert
-    of the centre gets a Gaussian kick of standard deviation sd ~ N(stddev, stddev/3) (redrawn while
-    negative) scaled by (1 - r / r_pert) (the centre: 1), and a standard-normal axis component; the axis
-    is normalised.  Returns (atoms, positions, axis)."""
-    L = np.diag(s.basis)
-    cand = _near_of(s).within(s.pos[centre], max(r_pert, 0.0) + 1e-9)
-    d = s.pos[cand] - s.pos[centre]
-    d -= L * np.round(d / L)
-    rr = np.sqrt(np.einsum("ij,ij->i", d, d))
-    sd = -1.0
-    while sd < 0:
-        sd = rng.normal(stddev, stddev / 3.0)
-    keep = (rr < r_pert) & (rr <= radius)
-    near = cand[keep]
-    r = dict(zip(cand.tolist(), rr.tolist()))
-    pos = s.pos[near].copy()
-    ax = np.zeros((len(near), 3))
-    for q, j in enumerate(near):
-        scale = 1.0 if j == centre else (1.0 - r[j] / r_pert)
-        pos[q] += rng.normal(0.0, sd, 3) * scale
-        ax[q] = rng.normal(0.0, 1.0, 3)
-    ax /= np.linalg.norm(ax)
-    return near.astype(np.int64), pos, ax
-
-
 def make_pool(count, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), relax=None):
     """`count` kMC systems: bcc Fe with N log-uniform in [n_min, n_max] atoms and a defect fraction uniform
     in `defects` (half vacancies, half tetrahedral interstitials).  relax(system) -> positions, if given,
@@ -125,7 +99,8 @@ def make_pool(count, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), re
 
 def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0)):
     """Poisson arrivals at `rate`/s, each from a uniformly chosen system of `pool`, asking for a uniform
-    number of searches in `searches` around defect-adjacent atoms with radii uniform in `radii`."""
+    number of searches in `searches` around defect-adjacent atoms with radii uniform in `radii`; each search
+    starts from Master::perturb (in the library) with its own generator state."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     reqs = []
@@ -134,8 +109,8 @@ def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0,
         k = int(rng.integers(searches[0], searches[1] + 1))
         centres = sample_centres(s, k, rng)
         rad = rng.uniform(radii[0], radii[1], k)
-        kicks = [sparse_kick(s, int(c), float(R), rng) for c, R in zip(centres, rad)]
-        reqs.append(Request(float(t[q]), s, centres, rad, kicks))
+        prng = rng.integers(1, 2 ** 63, size=(k, 4), dtype=np.uint64)
+        reqs.append(Request(float(t[q]), s, centres, rad, prng))
     return reqs
 
 
@@ -192,7 +167,7 @@ def serve(server: fg.Server, reqs, poll_ms=2.0, keep_positions=False, max_second
                 break
             slot[1] += 1
             r.sys_id = slot[0]
-            ids = server.submit(r.sys_id, r.centres, r.radii, r.kicks)
+            ids = server.submit_perturbed(r.sys_id, r.centres, r.radii, r.prng.copy())
             r.remaining = len(ids)
             r.admitted = time.perf_counter() - t0
             for sid in ids:

@@ -14,5 +14,13 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch ships its own HIP/HSA runtime; tests that hand torch device tensors to the library need it
+    # initialised before the library's context opens the device.
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     import openfly_amd as fa
     return fa.Context(0)

@@ -93,3 +93,17 @@ def pair_set(off, j, s):
     i = np.repeat(np.arange(n), np.diff(off))
     keys = np.stack([i, j.astype(np.int64), s.astype(np.int64)], axis=1)
     return keys[np.lexsort(keys.T[::-1])]
+
+
+def library_kick(s, centre, radius, state):
+    """Master::perturb start of system s (library flyg_perturb) around `centre` with atoms beyond `radius`
+    frozen, in the sparse form the serving API takes: (atoms, positions, axis) of the displaced atoms.
+    `state` (4 uint64) advances."""
+    import openfly_amd as fa
+    L = np.diag(s.basis)
+    d = s.pos - s.pos[centre]
+    d -= L * np.round(d / L)
+    frozen = (np.einsum("ij,ij->i", d, d) > radius * radius).astype(np.uint8)
+    x, ax, _ = fa.perturb(fa.Box(s.basis), s.pos, frozen, int(centre), state)
+    idx = np.nonzero(np.any(ax != 0, axis=1))[0].astype(np.int64)
+    return idx, x[idx], ax[idx]

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import openfly_amd as fa
-from openfly_amd import serve, systems
+from openfly_amd import systems
 from oracle import pyoracle as po
 from tests import helpers as H
 
@@ -56,7 +56,7 @@ def starts(S, k, seed):
     out = []
     for q in range(k):
         c = int(S["near"][q % len(S["near"])])
-        idx, p, a = serve.sparse_kick(S["s"], c, RADIUS, rng)
+        idx, p, a = H.library_kick(S["s"], c, RADIUS, rng.integers(1, 2 ** 63, size=4, dtype=np.uint64))
         x = S["s"].pos.copy()
         ax = np.zeros_like(x)
         x[idx] = p

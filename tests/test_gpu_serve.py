@@ -44,7 +44,8 @@ def requests_for(s, k, seed):
     rng = np.random.default_rng(seed)
     centres = serve.sample_centres(s, k, rng)
     radii = rng.uniform(5.0, 7.0, k)
-    kicks = [serve.sparse_kick(s, int(c), float(R), rng) for c, R in zip(centres, radii)]
+    states = rng.integers(1, 2 ** 63, size=(k, 4), dtype=np.uint64)
+    kicks = [H.library_kick(s, int(c), float(R), st) for c, R, st in zip(centres, radii, states)]
     return centres, radii, kicks
 
 
@@ -138,7 +139,7 @@ def test_memory_budget_gates_admission(ctx, wen):
 
 def test_served_barriers_match_fp64_energy(ctx, wen):
     s = wen["sys"]
-    centres, radii, kicks = requests_for(s, 8, 13)
+    centres, radii, kicks = requests_for(s, 24, 13)
     srv = fa.Server(ctx, wen["pot"], dimer=dict(f2norm=1e-3))
     sid = srv.add_system(fa.Box(s.basis), s.pos, s.types)
     res = drain(srv, srv.submit(sid, centres, radii, kicks))
