@@ -136,6 +136,16 @@ int flyg_eam_energy(flyg_eam eam, flyg_list nl, const uint32_t* types, const uin
 /* EAM::gradient (eam.cpp:53-120): two passes (density then force); frozen atoms get exactly 0. */
 int flyg_eam_gradient(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
                       double* grad);
+/* No counterpart in the reference: EAM::gradient for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same kernels and results as
+ * flyg_eam_gradient with FLYG_DEVICE pointers (types, frozen, grad on the context's device), but the
+ * density pass's output goes to a library-held device array of at most 131072 atoms (32 B per atom, one
+ * copy per GPU, held until the process ends) instead of a per-handle buffer, and the lanes per atom are
+ * not tuned (set_lanes, an earlier tuning for this size, else 8).  Larger systems are refused
+ * (FLYG_INVALID_ARG).  Calls that overlap on one GPU share that array and race.  Asynchronous on the
+ * context's stream; spline and type errors surface at the next synchronising call. */
+int flyg_eam_gradient_noalloc(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
+                              double* grad);
 /* H v with H the analytic Hessian of EAM::hessian (eam.cpp:122-233) restricted to non-frozen atoms,
  * computed matrix-free in two passes (no 3N x 3N matrix). */
 int flyg_eam_hvp(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,

@@ -92,6 +92,7 @@ int flyg_ctx_create(int device, flyg_ctx* out) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
+    noalloc_scratch_init(device);   // the allocation-free gradient's scratch address on this device
     c->dflags.reserve(16);
     HIPCHK(hipMemset(c->dflags.get(), 0, 16 * sizeof(unsigned)));
     c->dscal.reserve(64);
@@ -328,6 +329,16 @@ int flyg_eam_gradient(flyg_eam eam, flyg_list nl, const uint32_t* types, const u
     eam_gradient(*eam, *nl, t, f, eam->vec_out.get(), eam->precision);
     HIPCHK(hipMemcpyAsync(grad, eam->vec_out.get(), 3 * n * sizeof(double), hipMemcpyDeviceToHost, eam->ctx->stream));
     check_errors(*eam->ctx);
+  });
+}
+
+int flyg_eam_gradient_noalloc(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
+                              double* grad) {
+  return guard(eam->ctx, [&] {
+    require(nl->ctx == eam->ctx, "list and potential belong to different contexts");
+    require(n == nl->n, "EAM gradient size mismatch");
+    require(types && frozen && grad, "null argument");
+    eam_gradient_noalloc(*eam, *nl, types, frozen, grad, eam->precision);   // asynchronous; errors at the next sync
   });
 }
 

@@ -19,6 +19,7 @@
 //      which is H v for the analytic Hessian of eam.cpp:122-233 (pinned against it in the tests).
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include "eam_host.hpp"
 
@@ -593,6 +594,54 @@ namespace flyg {
     }
     gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
     if (E.timing) HIPCHK(hipEventRecord(E.events[E.events_used++].second, E.ctx->stream));
+    HIPCHK(hipGetLastError());
+  }
+
+  // ---------------------------------------------------------------------------------------------------
+  // Allocation-free gradient.  The pass-1 output goes to a file-scope device array instead of the
+  // handle's buffer; each GPU holds its own copy of the array (at its own address, looked up once per
+  // device when a context is created there) until the process ends, and every call on that GPU shares it,
+  // so calls that overlap on one GPU race.  The density pass writes every entry the force pass reads, so
+  // nothing needs clearing between calls.
+  // ---------------------------------------------------------------------------------------------------
+
+  namespace {
+    static __device__ __align__(256) double4 scratch[NOALLOC_MAX_ATOMS];
+
+    constexpr int MAX_DEVICES = 64;
+    std::mutex g_scratch_mu;
+    double4* g_scratch_ptr[MAX_DEVICES] = {};
+  }  // namespace
+
+  void noalloc_scratch_init(int device) {
+    require(device >= 0 && device < MAX_DEVICES, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    // The array's name is a host-side stand-in; kernels get this device address (the device is current).
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double4*>(p);
+  }
+
+  void eam_gradient_noalloc(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen,
+                            double* d_grad, int precision) {
+    require(nl.built, "List used before rebuild");
+    require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
+    require(nl.n <= NOALLOC_MAX_ATOMS, "the allocation-free gradient takes at most 131072 atoms");
+    double4* pf = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(g_scratch_mu);
+      int const dev = E.ctx->device;
+      if (dev >= 0 && dev < MAX_DEVICES) pf = g_scratch_ptr[dev];
+    }
+    require(pf != nullptr, "no scratch address for this device (its context was not created)");
+    Shifts S = shifts_of(nl.box);
+    PairArgs A = pair_args(E, nl, d_types, d_frozen);
+    // The lanes per atom row change no result bit: a fixed choice, or an earlier tuning, without timing.
+    int L = E.lanes > 0 ? E.lanes : 8;
+    auto it = E.tuned.find(std::make_pair(A.n, precision));
+    if (E.lanes <= 0 && it != E.tuned.end()) L = it->second;
+    gradient_with(E, S, A, d_grad, precision, L, pf);
     HIPCHK(hipGetLastError());
   }
 

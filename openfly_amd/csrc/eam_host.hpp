@@ -134,6 +134,12 @@ namespace flyg {
   double eam_energy(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen);
   void eam_gradient(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen, double* d_grad,
                     int precision);
+  // Allocation-free gradient: the pass-1 output lives in a file-scope device array of NOALLOC_MAX_ATOMS
+  // entries; noalloc_scratch_init(device) looks its address up once per device (that device current).
+  constexpr long NOALLOC_MAX_ATOMS = 131072;
+  void noalloc_scratch_init(int device);
+  void eam_gradient_noalloc(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen,
+                            double* d_grad, int precision);
   void eam_hvp(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen, double const* d_v,
                double* d_hv);
   void eam_table_eval(flyg_eam_s& e, int kind, int a, int b, double const* x, long m, int deriv, double* out);

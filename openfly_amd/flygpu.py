@@ -148,6 +148,7 @@ _SIGS = {
     "flyg_list_positions": [_vp, _vp, C.c_int],
     "flyg_eam_energy": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _d],
     "flyg_eam_gradient": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _vp],
+    "flyg_eam_gradient_noalloc": [_vp, _vp, _vp, _vp, C.c_int64, _vp],
     "flyg_eam_hvp": [_vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int],
     "flyg_steplbfgs_create": [_vp, C.c_int, C.POINTER(_vp)],
     "flyg_steplbfgs_destroy": [_vp],
