@@ -234,6 +234,8 @@ namespace flyg {
         }
       }
 
+      __device__ __forceinline__ void sync() const { __syncthreads(); }
+
       __device__ __forceinline__ unsigned any_or(unsigned e) const {
         __shared__ unsigned flag;
         if (tid == 0) flag = 0;
@@ -243,6 +245,70 @@ namespace flyg {
         unsigned r = flag;
         __syncthreads();
         return r;
+      }
+    };
+
+    // ---------------------------------------------------------------------------------------------
+    // One wavefront's vector engine.  The searches' vector algebra (L-BFGS two-loops, rotor, trust
+    // steps: 3*nm elements, a few hundred to a few thousand) runs on wave 0 of the workgroup while the
+    // other waves wait at the next barrier: element i belongs to lane i % 64, each lane adds its elements
+    // in increasing i, and the 64 lane sums meet in a fixed tree (quad butterflies, two rotations within
+    // each row of 16, then the four row sums in order).  A dot product needs no barrier and no LDS round
+    // trip, and its bits do not depend on the workgroup shape.
+    // ---------------------------------------------------------------------------------------------
+
+    struct Wave {
+      int lane;
+
+      template <int CTRL>
+      __device__ __forceinline__ static double dpp(double v) {
+        return __builtin_amdgcn_update_dpp(0.0, v, CTRL, 0xF, 0xF, false);
+      }
+      __device__ __forceinline__ static double lane_value(double v, int l) {
+        long long b = __double_as_longlong(v);
+        int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+        int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+      }
+      // Sum of the 64 lanes' values; every lane gets the same bits.
+      __device__ __forceinline__ static double total(double v) {
+        v = __dadd_rn(v, dpp<0xB1>(v));    // quad_perm [1,0,3,2]
+        v = __dadd_rn(v, dpp<0x4E>(v));    // quad_perm [2,3,0,1]
+        v = __dadd_rn(v, dpp<0x124>(v));   // row_ror:4
+        v = __dadd_rn(v, dpp<0x128>(v));   // row_ror:8: lane 16 r holds row r's sum
+        return __dadd_rn(__dadd_rn(lane_value(v, 0), lane_value(v, 16)),
+                         __dadd_rn(lane_value(v, 32), lane_value(v, 48)));
+      }
+
+      __device__ __forceinline__ double dot(double const* a, double const* b, int n) const {
+        double s = 0;
+        for (int i = lane; i < n; i += 64) s = __dadd_rn(s, a[i] * b[i]);
+        return total(s);
+      }
+
+      __device__ __forceinline__ void dot2(double const* a, double const* b, double const* c, double const* d, int n,
+                                           double& x, double& y) const {
+        double s = 0, t = 0;
+        for (int i = lane; i < n; i += 64) {
+          s = __dadd_rn(s, a[i] * b[i]);
+          t = __dadd_rn(t, c[i] * d[i]);
+        }
+        x = total(s);
+        y = total(t);
+      }
+
+      // A wavefront executes in lockstep; only the compiler must keep LDS accesses in program order.
+      __device__ __forceinline__ void sync() const { __builtin_amdgcn_wave_barrier(); }
+
+      // o = ca a + cb b + cc c (b, c may be null); o may alias a.
+      __device__ __forceinline__ void lin(double* o, int n, double const* a, double ca, double const* b = nullptr,
+                                          double cb = 0, double const* c = nullptr, double cc = 0) const {
+        for (int i = lane; i < n; i += 64) {
+          double v = ca * a[i];
+          if (b) v += cb * b[i];
+          if (c) v += cc * c[i];
+          o[i] = v;
+        }
       }
     };
 
@@ -275,6 +341,7 @@ namespace flyg {
       unsigned* act;
       int* alen;
       double act_f, act_m;   // active-list skins (mobile-frozen, mobile-mobile)
+      unsigned* errw;        // LDS: spline-error bits of the search's region gradients
     };
 
     __device__ inline int vix(int nt, int a, int b) { return a > b ? a + nt * b : b + nt * a; }
@@ -521,8 +588,10 @@ namespace flyg {
           g[3 * r0 + 2] = tz;
         }
       }
+      // Spline errors gather in one LDS word (sticky for the search): every thread reads it after the barrier.
+      if (e) atomicOr(v.errw, e);
       __syncthreads();
-      return tm.any_or(e);
+      return *v.errw;
     }
 
     // FP64 local energy (all terms that depend on mobile positions).  With base = true the densities
@@ -599,8 +668,8 @@ namespace flyg {
       __syncthreads();
     }
 
-    // L-BFGS two-loop recursion on the workgroup (lbfgs_core.cpp:29-87).
-    template <int T>
+    // L-BFGS two-loop recursion (lbfgs_core.cpp:29-87) on a team (the workgroup or one wavefront).
+    template <class TM>
     struct Core {
       double* s;      // m vectors
       double* y;      // m vectors
@@ -611,7 +680,8 @@ namespace flyg {
       double* alpha;  // LDS, m
       int m, k, n3;
 
-      __device__ __forceinline__ double* newton_step(Team<T> const& tm, double const* x, double const* g) {
+      // The scalars rho and alpha live in LDS; every lane of the team stores the same value.
+      __device__ __forceinline__ double* newton_step(TM const& tm, double const* x, double const* g) {
         int prev = k > 0 ? (k - 1) % m : 0;
         if (k > 0) {
           double* sp = s + static_cast<long>(prev) * n3;
@@ -619,18 +689,18 @@ namespace flyg {
           tm.lin(sp, n3, x, 1.0, px, -1.0);
           tm.lin(yp, n3, g, 1.0, pg, -1.0);
           double sy = tm.dot(sp, yp, n3);
-          if (tm.tid == 0) rho[prev] = 1.0 / fabs(sy);
+          rho[prev] = 1.0 / fabs(sy);
         }
         tm.lin(px, n3, x, 1.0);
         tm.lin(pg, n3, g, 1.0);
         tm.lin(q, n3, g, 1.0);
         int incur = k <= m ? 0 : k - m;
         int bound = k <= m ? k : m;
-        __syncthreads();
+        tm.sync();
         for (int i = bound - 1; i >= 0; --i) {
           int j = (i + incur) % m;
           double a = rho[j] * tm.dot(s + static_cast<long>(j) * n3, q, n3);
-          if (tm.tid == 0) alpha[j] = a;
+          alpha[j] = a;
           tm.lin(q, n3, q, 1.0, y + static_cast<long>(j) * n3, -a);
         }
         if (k > 0) {
@@ -638,21 +708,22 @@ namespace flyg {
           double scale = 1.0 / rho[prev] / tm.dot(yp, yp, n3);
           tm.lin(q, n3, q, scale);
         }
-        __syncthreads();
+        tm.sync();
         for (int i = 0; i < bound; ++i) {
           int j = (i + incur) % m;
           double beta = rho[j] * tm.dot(y + static_cast<long>(j) * n3, q, n3);
           tm.lin(q, n3, q, 1.0, s + static_cast<long>(j) * n3, alpha[j] - beta);
         }
         ++k;
-        __syncthreads();
+        tm.sync();
         return q;
       }
     };
 
     template <int T, int L, bool MIX, bool MULTI>
     __device__ __forceinline__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
-                               double4* lds_cur, double* lds_ws_buf, SplineSet const* lds_spl) {
+                               double4* lds_cur, double* lds_ws_buf, SplineSet const* lds_spl, unsigned* lds_err,
+                               double* bc) {
       long const t_begin = static_cast<long>(wall_clock64());
       SearchHdr const h = A.hdr[sid];
       DimOpt const& o = A.o;
@@ -676,6 +747,10 @@ namespace flyg {
       v.alen = A.alen + h.reg_off;
       v.act_f = fmin(ACT_SKIN_F, h.skin_f);
       v.act_m = fmin(ACT_SKIN_M, h.skin_m);
+      v.errw = lds_err;
+      // The vector algebra runs on wave 0 (w0); bc[] carries its results to the other waves.
+      bool const w0 = tm.tid < 64;
+      Wave const wv{tm.tid & 63};
       int const n3 = v.n3;
       // Workspace: the vectors the rotor touches every iteration (10 named ones and the rotation core)
       // come first and live in LDS when they fit; the translation core and xr stay in global memory.
@@ -695,10 +770,12 @@ namespace flyg {
       double* xb = vecp(9);
       double* xr = vecp(nvec(o.n, o.rn) - 1);   // positions at the last active-list refresh
       int const r0v = 10, t0 = 13 + 2 * o.rn;
-      Core<T> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
-                 lds_core, lds_core + 32, o.n, 0, n3};
-      Core<T> rc{vecp(r0v), vecp(r0v + o.rn), vecp(r0v + 2 * o.rn), vecp(r0v + 2 * o.rn + 1), vecp(r0v + 2 * o.rn + 2),
-                 lds_core + 64, lds_core + 96, o.rn, 0, n3};
+      Core<Wave> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
+                    lds_core, lds_core + 32, o.n, 0, n3};
+      Core<Wave> rc{vecp(r0v), vecp(r0v + o.rn), vecp(r0v + 2 * o.rn), vecp(r0v + 2 * o.rn + 1),
+                    vecp(r0v + 2 * o.rn + 2), lds_core + 64, lds_core + 96, o.rn, 0, n3};
+      // The mechanism relaxations keep the workgroup-wide algebra (same memory as the translation core).
+      Core<Team<T>> tcm{tc.s, tc.y, tc.px, tc.pg, tc.q, tc.rho, tc.alpha, o.n, 0, n3};
       double const dr = o.delta_r;
       unsigned err = 0;
       long grads = 0;
@@ -707,6 +784,7 @@ namespace flyg {
       // Gather the region from the resident system, then the base state: region at base positions,
       // frozen density parts, base local energy.
       if (tm.tid < 27) lds_S[tm.tid] = h.sS[tm.tid];
+      if (tm.tid == 0) *lds_err = 0;
       for (int i = tm.tid; i < v.nr; i += T) {
         int gi = v.gidx[i];
         double4 b = h.sbase[gi];
@@ -799,9 +877,22 @@ namespace flyg {
         ++refreshes;
       };
 
-      // Rotor::eff_gradient (rotor.cpp:49-164); returns false on a spline error.
-      auto eff_gradient = [&](double& mag_g) -> bool {
-        rc.k = 0;
+      // Trial end of the dimer for the rotor, placed by wave 0: cur = x + dr * d over the mobile atoms (the
+      // arithmetic of place()).
+      auto place_w = [&](double const* d) {
+        for (int a = wv.lane; a < v.nm; a += 64) {
+          double px = x[3 * a], py = x[3 * a + 1], pz = x[3 * a + 2];
+          px += dr * d[3 * a];
+          py += dr * d[3 * a + 1];
+          pz += dr * d[3 * a + 2];
+          v.cur[a] = make_double4(px, py, pz, 0.0);
+        }
+      };
+
+      // Rotor::eff_gradient (rotor.cpp:49-164); returns false on a spline error.  Gradients run on the
+      // whole workgroup, the rotor's algebra on wave 0 between them (two barriers per rotor iteration).
+      // With qproj, bc[7] also receives g_eff . qproj for the trust update.
+      auto eff_gradient = [&](double& mag_g, double const* qproj) -> bool {
         place(v, tm, x, 0.0, nullptr);
         ensure_active();
         err |= timed_gradient(g0);
@@ -809,54 +900,89 @@ namespace flyg {
         err |= timed_gradient(g1);
         grads += 2;
         if (err) return false;
+        rc.k = 0;
+        double* theta = nullptr;
+        double b_1 = 0, c_x0 = 0, theta_1 = 0;
         for (int i = 0;; ++i) {
-          tm.lin(dg, n3, g1, 1.0, g0, -1.0);
-          double t = tm.dot(dg, ax, n3);
-          tm.lin(dgp, n3, dg, 1.0, ax, -t);
-          double* theta = rc.newton_step(tm, ax, dgp);
-          double ot = tm.dot(theta, ax, n3);
-          tm.lin(theta, n3, theta, 1.0, ax, -ot);
-          double nn = tm.dot(theta, theta, n3);
-          tm.lin(theta, n3, theta, 1.0 / sqrt(nn));
-          double b1r, c0r;
-          tm.dot2(dg, theta, dg, ax, n3, b1r, c0r);
-          double b_1 = b1r / dr, c_x0 = c0r / dr;
-          double theta_1 = -0.5 * atan(b_1 / fabs(c_x0));
+          if (w0) {
+            wv.lin(dg, n3, g1, 1.0, g0, -1.0);
+            double t = wv.dot(dg, ax, n3);
+            wv.lin(dgp, n3, dg, 1.0, ax, -t);
+            theta = rc.newton_step(wv, ax, dgp);
+            double ot = wv.dot(theta, ax, n3);
+            wv.lin(theta, n3, theta, 1.0, ax, -ot);
+            double nn = wv.dot(theta, theta, n3);
+            wv.lin(theta, n3, theta, 1.0 / sqrt(nn));
+            double b1r, c0r;
+            wv.dot2(dg, theta, dg, ax, n3, b1r, c0r);
+            b_1 = b1r / dr;
+            c_x0 = c0r / dr;
+            theta_1 = -0.5 * atan(b_1 / fabs(c_x0));
+            bool const stop =
+                fabs(theta_1) < o.theta_tol || (i >= o.iter_max_rot && c_x0 < 0) || i > 10 * o.iter_max_rot;
+            if (!stop) {
+              wv.lin(axp, n3, ax, cos(theta_1), theta, sin(theta_1));
+              place_w(axp);
+            }
+            if (wv.lane == 0) {
+              bc[0] = stop ? 1.0 : 0.0;
+              bc[1] = c_x0;
+            }
+          }
+          __syncthreads();
           ++rot_iters;
-          if (fabs(theta_1) < o.theta_tol || (i >= o.iter_max_rot && c_x0 < 0) || i > 10 * o.iter_max_rot) {
-            curv = c_x0;
+          if (bc[0] != 0.0) {
+            curv = bc[1];
             break;
           }
-          double ct = cos(theta_1), st = sin(theta_1);
-          tm.lin(axp, n3, ax, ct, theta, st);
-          place(v, tm, x, dr, axp);
           err |= timed_gradient(g1p);
           ++grads;
           if (err) return false;
-          tm.lin(dg, n3, g1p, 1.0, g0, -1.0);
-          double c_x1 = tm.dot(dg, axp, n3) / dr;
-          double a_1 = (c_x0 - c_x1 + b_1 * sin(2 * theta_1)) / (1 - cos(2 * theta_1));
-          double theta_min = 0.5 * atan(b_1 / a_1);
-          if (a_1 * cos(2 * theta_min) - a_1 + b_1 * sin(2 * theta_min) > 0) theta_min += M_PI / 2;
-          double cm = cos(theta_min), sm = sin(theta_min);
-          tm.lin(ax, n3, ax, cm, theta, sm);
-          double w1 = sin(theta_1 - theta_min) / sin(theta_1);
-          double w2 = sin(theta_min) / sin(theta_1);
-          double w3 = 1 - cos(theta_min) - sin(theta_min) * tan(0.5 * theta_1);
-          tm.lin(g1, n3, g1, w1, g1p, w2, g0, w3);
-          if (fabs(theta_min) < o.theta_tol) {
-            tm.lin(dg, n3, g1, 1.0, g0, -1.0);
-            curv = tm.dot(dg, ax, n3) / dr;
+          if (w0) {
+            wv.lin(dg, n3, g1p, 1.0, g0, -1.0);
+            double c_x1 = wv.dot(dg, axp, n3) / dr;
+            double a_1 = (c_x0 - c_x1 + b_1 * sin(2 * theta_1)) / (1 - cos(2 * theta_1));
+            double theta_min = 0.5 * atan(b_1 / a_1);
+            if (a_1 * cos(2 * theta_min) - a_1 + b_1 * sin(2 * theta_min) > 0) theta_min += M_PI / 2;
+            double cm = cos(theta_min), sm = sin(theta_min);
+            wv.lin(ax, n3, ax, cm, theta, sm);
+            double w1 = sin(theta_1 - theta_min) / sin(theta_1);
+            double w2 = sin(theta_min) / sin(theta_1);
+            double w3 = 1 - cos(theta_min) - sin(theta_min) * tan(0.5 * theta_1);
+            wv.lin(g1, n3, g1, w1, g1p, w2, g0, w3);
+            bool const stop = fabs(theta_min) < o.theta_tol;
+            double c = 0;
+            if (stop) {
+              wv.lin(dg, n3, g1, 1.0, g0, -1.0);
+              c = wv.dot(dg, ax, n3) / dr;
+            }
+            if (wv.lane == 0) {
+              bc[2] = stop ? 1.0 : 0.0;
+              bc[3] = c;
+            }
+          }
+          __syncthreads();
+          if (bc[2] != 0.0) {
+            curv = bc[3];
             break;
           }
         }
-        double p = tm.dot(g0, ax, n3);
-        if (!o.relax_in_convex && curv > 0) {
-          tm.lin(eff, n3, ax, -p);
-        } else {
-          tm.lin(eff, n3, g0, 1.0, ax, -2.0 * p);
+        if (w0) {
+          double p = wv.dot(g0, ax, n3);
+          if (!o.relax_in_convex && curv > 0) {
+            wv.lin(eff, n3, ax, -p);
+          } else {
+            wv.lin(eff, n3, g0, 1.0, ax, -2.0 * p);
+          }
+          double mg = wv.dot(eff, eff, n3);
+          double pj = qproj ? wv.dot(eff, qproj, n3) : 0.0;
+          if (wv.lane == 0) {
+            bc[4] = mg;
+            bc[7] = pj;
+          }
         }
-        mag_g = tm.dot(eff, eff, n3);
+        __syncthreads();
+        mag_g = bc[4];
         return true;
       };
 
@@ -865,7 +991,7 @@ namespace flyg {
       if (!displacement_ok()) {
         ret = EXIT_REGION;
         steps = 0;
-      } else if (!eff_gradient(mag_g)) {
+      } else if (!eff_gradient(mag_g, nullptr)) {
         ret = EXIT_OOB;
         steps = 0;
       } else {
@@ -881,41 +1007,47 @@ namespace flyg {
             steps = i;
             break;
           }
-          // History test every hist_check_freq steps (dimer.cpp:102-131): the angle between x - min and
-          // sp_k - min below theta_tol for a known saddle k ends the search as a collision.  Frozen atoms
-          // sit at the minimum, so x - min is zero there and the dot product runs over the mobile atoms.
-          if (o.use_history && h.n_hist > 0 && i % o.hist_check_freq == 0) {
-            tm.lin(dg, n3, x, 1.0, xb, -1.0);
-            double const on = sqrt(tm.dot(dg, dg, n3));
+          double* q = nullptr;
+          if (w0) {
+            // History test every hist_check_freq steps (dimer.cpp:102-131): the angle between x - min and
+            // sp_k - min below theta_tol for a known saddle k ends the search as a collision.  Frozen atoms
+            // sit at the minimum, so x - min is zero there and the dot product runs over the mobile atoms.
             bool hit = false;
-            for (int k = 0; k < h.n_hist && !hit; ++k) {
-              double const dt = tm.dot(dg, A.hist + h.hist_off + static_cast<long>(k) * n3, n3);
-              hit = dt / (A.hnorm[h.hn_off + k] * on) > h.cos_tol;
+            if (o.use_history && h.n_hist > 0 && i % o.hist_check_freq == 0) {
+              wv.lin(dg, n3, x, 1.0, xb, -1.0);
+              double const on = sqrt(wv.dot(dg, dg, n3));
+              for (int k = 0; k < h.n_hist && !hit; ++k) {
+                double const dt = wv.dot(dg, A.hist + h.hist_off + static_cast<long>(k) * n3, n3);
+                hit = dt / (A.hnorm[h.hn_off + k] * on) > h.cos_tol;
+              }
             }
-            if (hit) {
-              ret = 3;
-              steps = i;
-              break;
+            if (!hit) {
+              q = tc.newton_step(wv, x, eff);
+              double nrm = sqrt(wv.dot(q, q, n3));
+              double scale = fmin(1.0, trust / nrm);
+              wv.lin(q, n3, q, scale);
+              wv.lin(x, n3, x, 1.0, q, -1.0);
             }
+            if (wv.lane == 0) bc[5] = hit ? 1.0 : 0.0;
           }
-          double* q = tc.newton_step(tm, x, eff);
-          double nrm = sqrt(tm.dot(q, q, n3));
-          double scale = fmin(1.0, trust / nrm);
-          tm.lin(q, n3, q, scale);
-          tm.lin(x, n3, x, 1.0, q, -1.0);
           __syncthreads();
+          if (bc[5] != 0.0) {
+            ret = 3;
+            steps = i;
+            break;
+          }
           if (!displacement_ok()) {
             ret = EXIT_REGION;
             steps = i;
             break;
           }
-          if (!eff_gradient(mag_g)) {
+          if (!eff_gradient(mag_g, q)) {
             ret = EXIT_OOB;
             steps = i;
             break;
           }
           convex_count = curv > 0 ? convex_count + 1 : 0;
-          double proj = tm.dot(eff, q, n3);
+          double const proj = bc[7];
           if (proj < -o.proj_tol) {
             trust = fmax(o.min_trust, o.shrink_trust * trust);
           } else if (proj > o.proj_tol) {
@@ -942,7 +1074,7 @@ namespace flyg {
       double* xfw = g1p;
       if (o.mech && ret == 0) {
         auto minimise = [&](int& iters) -> int {   // 0 converged, 1 iter_max, EXIT_REGION, EXIT_OOB
-          tc.k = 0;
+          tcm.k = 0;
           place(v, tm, x, 0.0, nullptr);
           if (!displacement_ok()) return EXIT_REGION;
           ensure_active();
@@ -954,7 +1086,7 @@ namespace flyg {
           for (int i = 0; i < o.min_iter_max; ++i) {
             iters = i;
             if (mag < o.min_f2norm * o.min_f2norm) return 0;
-            double* q = tc.newton_step(tm, x, g0);
+            double* q = tcm.newton_step(tm, x, g0);
             double nrm = sqrt(tm.dot(q, q, n3));
             double scale = fmin(1.0, trust / nrm);
             tm.lin(q, n3, q, scale);
@@ -1154,6 +1286,8 @@ namespace flyg {
       __shared__ double4 lds_cur[lds_rows<T>()];
       __shared__ double lds_wsb[lds_ws<T>()];
       __shared__ SplineSet lds_spl;
+      __shared__ unsigned lds_err;
+      __shared__ double lds_bc[8];
       Team<T> tm{red, static_cast<int>(threadIdx.x)};
       if (threadIdx.x == 0) {   // constant indices only
 #pragma unroll
@@ -1171,7 +1305,7 @@ namespace flyg {
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb, &lds_spl);
+        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb, &lds_spl, &lds_err, lds_bc);
       }
     }
 
