@@ -151,6 +151,22 @@ int flyg_eam_gradient_noalloc(flyg_eam eam, flyg_list nl, const uint32_t* types,
 int flyg_eam_hvp(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
                  const double* v, double* hv, int where);
 
+/* EAM::hessian (eam.cpp:122-233) of the non-frozen atoms, mass-weighted with the potential's masses
+ * (Hessian::mass_weight, hessian.hpp:136-143) and diagonalised (Hessian::eigenvalues, hessian.hpp:86-93):
+ * the 3M eigenvalues in increasing order (M = non-frozen atoms; *n_dof = 3M, eigenvalues holds 3n).
+ * These are the reference's 3n eigenvalues without the 3 per frozen atom that its zero rows add.  The
+ * block is assembled on the device from 3M Hessian-vector products and diagonalised there by cyclic
+ * Jacobi rotations; at most 16384 degrees of freedom. */
+int flyg_eam_hessian_eigenvalues(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
+                                 int where, double* eigenvalues, int64_t* n_dof);
+/* Master's reading of a spectrum (find.cpp:849-866 at a saddle, 1113-1126 at a minimum): the sum of
+ * log(lambda) over eigenvalues above zero_tol (Master::Options::hessian_eigen_zero_tol, 1e-6), and the
+ * numbers of eigenvalues within zero_tol of zero and below -zero_tol. */
+int flyg_log_prod_eigen(const double* eigenvalues, int64_t n, double zero_tol, double* log_prod, int64_t* n_zero,
+                        int64_t* n_negative);
+/* Mechanism::kinetic_pre (find.cpp:287): sqrt(exp(log_prod_min - log_prod_sp) / (2 pi amu)), amu in kg. */
+int flyg_prefactor(double log_prod_min, double log_prod_sp, double* prefactor);
+
 /* ---------------------------------------------------------------------------------------------------
  * StepLBFGS: replaces minimise::StepLBFGS (lbfgs_core.hpp:37-119, lbfgs_core.cpp:29-87).
  * ------------------------------------------------------------------------------------------------- */

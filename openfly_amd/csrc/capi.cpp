@@ -342,6 +342,59 @@ int flyg_eam_gradient_noalloc(flyg_eam eam, flyg_list nl, const uint32_t* types,
   });
 }
 
+int flyg_eam_hessian_eigenvalues(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n,
+                                 int where, double* eigenvalues, int64_t* n_dof) {
+  return guard(eam->ctx, [&] {
+    require(nl->ctx == eam->ctx, "list and potential belong to different contexts");
+    require(n == nl->n, "EAM Hessian size mismatch");
+    require(types && frozen && eigenvalues && n_dof, "null argument");
+    std::vector<uint32_t> ht(types, types + (where == FLYG_HOST ? n : 0));
+    std::vector<uint8_t> hf(frozen, frozen + (where == FLYG_HOST ? n : 0));
+    if (where == FLYG_DEVICE) {
+      ht.resize(n);
+      hf.resize(n);
+      HIPCHK(hipMemcpyAsync(ht.data(), types, n * sizeof(uint32_t), hipMemcpyDeviceToHost, eam->ctx->stream));
+      HIPCHK(hipMemcpyAsync(hf.data(), frozen, n * sizeof(uint8_t), hipMemcpyDeviceToHost, eam->ctx->stream));
+      HIPCHK(hipStreamSynchronize(eam->ctx->stream));
+    }
+    uint32_t const* t;
+    uint8_t const* f;
+    stage_atoms(*eam, types, frozen, n, where, &t, &f);
+    std::vector<double> w = hessian_eigenvalues(*eam, *nl, t, f, hf, ht, nullptr);
+    std::copy(w.begin(), w.end(), eigenvalues);
+    *n_dof = static_cast<int64_t>(w.size());
+  });
+}
+
+int flyg_log_prod_eigen(const double* eigenvalues, int64_t n, double zero_tol, double* log_prod, int64_t* n_zero,
+                        int64_t* n_negative) {
+  return guard(nullptr, [&] {
+    require(eigenvalues && log_prod && n_zero && n_negative && n >= 0, "null argument");
+    double lp = 0;
+    int64_t z = 0, neg = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      double e = eigenvalues[i];
+      if (e < -zero_tol) {
+        ++neg;
+      } else if (e > zero_tol) {
+        lp += std::log(e);
+      } else {
+        ++z;
+      }
+    }
+    *log_prod = lp;
+    *n_zero = z;
+    *n_negative = neg;
+  });
+}
+
+int flyg_prefactor(double log_prod_min, double log_prod_sp, double* prefactor) {
+  return guard(nullptr, [&] {
+    require(prefactor != nullptr, "null argument");
+    *prefactor = std::sqrt(std::exp(log_prod_min - log_prod_sp) / (2 * M_PI * 1.6605390666050e-27));
+  });
+}
+
 int flyg_eam_hvp(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, const double* v,
                  double* hv, int where) {
   return guard(eam->ctx, [&] {

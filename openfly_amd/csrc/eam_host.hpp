@@ -140,6 +140,10 @@ namespace flyg {
   void noalloc_scratch_init(int device);
   void eam_gradient_noalloc(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen,
                             double* d_grad, int precision);
+  // Sorted eigenvalues of the mass-weighted Hessian block of the non-frozen atoms (hessian.hip).
+  std::vector<double> hessian_eigenvalues(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types,
+                                          uint8_t const* d_frozen, std::vector<uint8_t> const& h_frozen,
+                                          std::vector<uint32_t> const& h_types, int* sweeps);
   void eam_hvp(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen, double const* d_v,
                double* d_hv);
   void eam_table_eval(flyg_eam_s& e, int kind, int a, int b, double const* x, long m, int deriv, double* out);

@@ -150,6 +150,9 @@ _SIGS = {
     "flyg_eam_gradient": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _vp],
     "flyg_eam_gradient_noalloc": [_vp, _vp, _vp, _vp, C.c_int64, _vp],
     "flyg_eam_hvp": [_vp, _vp, _vp, _vp, C.c_int64, _vp, _vp, C.c_int],
+    "flyg_eam_hessian_eigenvalues": [_vp, _vp, _vp, _vp, C.c_int64, C.c_int, _d, _i64],
+    "flyg_log_prod_eigen": [_d, C.c_int64, C.c_double, _d, _i64, _i64],
+    "flyg_prefactor": [C.c_double, C.c_double, _d],
     "flyg_steplbfgs_create": [_vp, C.c_int, C.POINTER(_vp)],
     "flyg_steplbfgs_destroy": [_vp],
     "flyg_steplbfgs_clear": [_vp],
@@ -407,6 +410,39 @@ class EAM:
             raise FlyError("all arrays must be host or all device")
         self.ctx.check(lib().flyg_eam_hvp(self.h, nl.h, tp, fp, nl.n, vp, op, w1))
         return out if w4 == DEVICE else k4.reshape(-1, 3)
+
+
+    def hessian_eigenvalues(self, types, frozen, nl: "List"):
+        """Sorted eigenvalues of the mass-weighted Hessian of the non-frozen atoms (host arrays)."""
+        t = np.ascontiguousarray(types, dtype=np.uint32).reshape(nl.n)
+        f = np.ascontiguousarray(frozen, dtype=np.uint8).reshape(nl.n)
+        w = np.zeros(3 * nl.n)
+        m = C.c_int64()
+        self.ctx.check(lib().flyg_eam_hessian_eigenvalues(self.h, nl.h, C.c_void_p(t.ctypes.data),
+                                                          C.c_void_p(f.ctypes.data), nl.n, HOST,
+                                                          w.ctypes.data_as(_d), C.byref(m)))
+        return w[:m.value]
+
+
+def log_prod_eigen(eigenvalues, zero_tol=1e-6):
+    """(sum of log lambda over lambda > zero_tol, zero modes, negative modes), as Master reads a spectrum."""
+    w = np.ascontiguousarray(eigenvalues, dtype=np.float64)
+    lp = C.c_double()
+    z, neg = C.c_int64(), C.c_int64()
+    rc = lib().flyg_log_prod_eigen(w.ctypes.data_as(_d), w.size, float(zero_tol), C.byref(lp), C.byref(z),
+                                   C.byref(neg))
+    if rc:
+        _raise(None, rc)
+    return lp.value, z.value, neg.value
+
+
+def prefactor(log_prod_min, log_prod_sp):
+    """Mechanism::kinetic_pre (find.cpp:287) from the two log-products (Hz)."""
+    a = C.c_double()
+    rc = lib().flyg_prefactor(float(log_prod_min), float(log_prod_sp), C.byref(a))
+    if rc:
+        _raise(None, rc)
+    return a.value
 
 
 DataEAM = EAM
