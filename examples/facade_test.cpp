@@ -156,6 +156,55 @@ int main(int argc, char** argv) {
     REQUIRE(dmax < 1e-10);   // positions return to x up to the rounding of x + d - d
   }
 
+  // ---- Master::find_batch on a vacancy (find.cpp:684-755): 16 searches at a vacancy neighbour, the site's
+  //      cache growing with the converged saddles, pathways with a THz-range prefactor ----
+  {
+    auto data = std::make_shared<potential::DataEAM>(potential::DataEAM::Options{}, std::ifstream{path});
+    Cell cell = make_super();
+    potential::Generic pot{potential::EAM(cell.map, data)};
+    minimise::LBFGS minimiser({}, cell.box);
+    system::SoA<Position, PotentialGradient> relaxed(cell.atoms.size());
+    REQUIRE(!minimiser.minimise(relaxed, cell.atoms, pot, 1));
+    system::SoA<Position, TypeID, Frozen> in(cell.atoms.size());
+    for (std::int64_t i = 0; i < cell.atoms.size(); ++i) {
+      in(r_, i) = Vec(relaxed(r_, i));
+      in(id_, i) = 0;
+      in(fzn_, i) = false;
+    }
+    saddle::Master::Options mopt;
+    mopt.batch_size = 16;
+    mopt.active_radius = 6.0;
+    mopt.seed = 12345;
+    minimise::LBFGS::Options lo;
+    lo.f2norm = 1e-5;
+    minimise::LBFGS relaxer(lo, cell.box);
+    saddle::Dimer::Options dopt;
+    dopt.f2norm = 1e-4;
+    saddle::Dimer dimer(dopt, {}, cell.box);
+    system::TypeMap<Mass> mass_map(2);
+    saddle::Master master(mopt, cell.box, mass_map, pot, relaxer, dimer);
+    std::vector<system::SoA<Position>> cache;
+    std::vector<saddle::Master::Mech> mechs;
+    auto found = master.find_batch(0, 0, in, cache, &mechs);
+    int converged = 0, pathways = 0, fresh = 0;
+    for (auto const& f : found) {
+      if (f.exit == saddle::Dimer::success) ++converged;
+      if (f.mech.code == 0) {
+        ++pathways;
+        fresh += f.is_new;
+        REQUIRE(f.kinetic_pre > 1e11 && f.kinetic_pre < 1e15);
+      }
+    }
+    std::printf("Master::find_batch: %d of %zu converged, %d pathways (%d distinct), cache %zu\n", converged,
+                found.size(), pathways, fresh, cache.size());
+    REQUIRE(found.size() == 16);
+    REQUIRE(static_cast<int>(cache.size()) == converged);
+    REQUIRE(converged >= 1);
+    REQUIRE(static_cast<int>(mechs.size()) == fresh && fresh <= pathways);
+    // Every pathway is known to the site's list afterwards: a second look finds none new.
+    for (auto const& m : mechs) REQUIRE(!master.is_new_mech(m, mechs));
+  }
+
   // ---- Spline::OutOfBounds surfaces as the reference's exception type ----
   {
     auto data = std::make_shared<potential::DataEAM>(potential::DataEAM::Options{}, std::ifstream{path});

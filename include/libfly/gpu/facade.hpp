@@ -11,6 +11,7 @@
 // ThreadData, find.hpp:310-317).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -23,6 +24,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <random>
 #include <utility>
 #include <vector>
 
@@ -132,6 +134,7 @@ namespace fly {
   struct TypeID : system::Property<std::uint32_t> {};
   struct Frozen : system::Property<bool> {};
   struct Index : system::Property<std::int64_t> {};
+  struct Mass : system::Property<double> {};
 
   inline constexpr Position r_{};
   inline constexpr Axis ax_{};
@@ -536,7 +539,7 @@ namespace fly {
         bool debug = false;
       };
 
-      LBFGS(Options const& opt, system::Box const& box) : m_ctx(gpu::context()) {
+      LBFGS(Options const& opt, system::Box const& box) : m_ctx(gpu::context()), m_opt(opt) {
         flyg_lbfgs_options o{opt.n, opt.iter_max, opt.f2norm, opt.skin_frac, opt.proj_tol,
                              opt.max_trust, opt.min_trust, opt.grow_trust, opt.shrink_trust};
         gpu::check(flyg_lbfgs_create(m_ctx, &o, box.basis_data(), box.periodic_data(), &m_h), m_ctx);
@@ -559,8 +562,11 @@ namespace fly {
         return failed != 0;
       }
 
+      Options const& options() const { return m_opt; }
+
     private:
       flyg_ctx m_ctx;
+      Options m_opt;
       flyg_lbfgs m_h = nullptr;
     };
   }  // namespace minimise
@@ -630,7 +636,8 @@ namespace fly {
       // The reference's values (dimer.hpp:123-129); the C-ABI reports them without the sign.
       enum Exit : int { success = 0, convex = -1, iter_max = -2, collision = -3, uninit = -4 };
 
-      Dimer(Options const& opt, Rotor::Options const& ropt, system::Box const& box) : m_ctx(gpu::context()) {
+      Dimer(Options const& opt, Rotor::Options const& ropt, system::Box const& box)
+          : m_ctx(gpu::context()), m_opt(opt), m_ropt(ropt) {
         flyg_dimer_options o{opt.n, opt.max_steps, opt.hist_check_freq, opt.f2norm, opt.skin_frac, opt.proj_tol,
                              opt.max_trust, opt.min_trust, opt.grow_trust, opt.shrink_trust, opt.convex_max,
                              opt.use_history ? 1 : 0};
@@ -669,11 +676,254 @@ namespace fly {
       }
 
       flyg_search_stats const& last_stats() const { return m_last; }
+      Options const& options() const { return m_opt; }
+      Rotor::Options const& rotor_options() const { return m_ropt; }
 
     private:
       flyg_ctx m_ctx;
+      Options m_opt;
+      Rotor::Options m_ropt;
       flyg_dimer m_h = nullptr;
       flyg_search_stats m_last{};
+    };
+
+    // Master (find.hpp:260-306, find.cpp:178-194): the saddle-search core of Master::find_mechs, with the
+    // searches of a site run as one device batch.  find_mechs itself (catalogue geometries, symmetry
+    // expansion, mechanism deduplication) needs the reference's env layer and is not part of this facade;
+    // find_batch below is its per-site step (find.cpp:684-755) with the mechanism checks of saddle_2_mech
+    // (find.cpp:757-837) and the prefactor of the mass-weighted Hessians (find.cpp:838-871, 287).
+    class Master {
+    public:
+      struct Options {
+        double r_pert = 4;
+        double stddev = 0.6;
+        double basin_tol = 0.05;
+        double mech_tol = 0.1;
+        double stationary_tol = 0.025;   // basin_tol / 2
+        double hessian_eigen_zero_tol = 1e-6;
+        int num_threads = 1;
+        int max_searches = 250;
+        int max_failed_searches = 100;
+        int batch_size = 100;            // searches of a site that run as one device batch
+        double nudge_frac = 0.01;
+        bool debug = false;
+        // Device form: atoms farther than this from a search's centre are frozen during that search.
+        double active_radius = 9.0;
+        // Seed of the searches' Xoshiro256** streams; 0 seeds from std::random_device as the reference does.
+        std::uint64_t seed = 0;
+      };
+
+      // A mechanism over a site's atoms (its search region, ascending index): the saddle and the forward
+      // minimum relative to the reverse one, and the unit dimer axis (env::Mechanism's delta_sp, delta_fwd,
+      // axis, find.cpp:816-834).
+      struct Mech {
+        std::vector<double> delta_sp, delta_fwd, axis;
+      };
+
+      // One search of find_batch: its exit, saddle, and (for a min->sp->min pathway) the mechanism data.
+      struct Search {
+        Dimer::Exit exit;
+        flyg_search_stats stats;
+        flyg_mechanism mech;            // mech.code 0: a pathway (saddle_2_mech's checks passed)
+        double kinetic_pre = 0;         // Mechanism::kinetic_pre of a pathway (find.cpp:287), Hz
+        bool is_new = false;            // a pathway not within mech_tol of an earlier one at the site
+        std::vector<double> sp, axis;   // 3n saddle positions and dimer axis
+      };
+
+      // Master::is_new_mech (find.cpp:1216-1277): a mechanism is known when, for some earlier one, the
+      // saddle and forward displacements are both within mech_tol (root of the summed squared per-atom
+      // differences), the saddle displacements point within 10 degrees and the axes within 5 (either sign).
+      bool is_new_mech(Mech const& maybe, std::vector<Mech> const& hist) const {
+        for (Mech const& m : hist) {
+          double d_sp = 0, d_fwd = 0, dot = 0, n1 = 0, n2 = 0, dot_ax = 0;
+          for (std::size_t i = 0; i < m.delta_sp.size(); ++i) {
+            double a = maybe.delta_sp[i] - m.delta_sp[i], b = maybe.delta_fwd[i] - m.delta_fwd[i];
+            d_sp += a * a;
+            d_fwd += b * b;
+            n1 += maybe.delta_sp[i] * maybe.delta_sp[i];
+            n2 += m.delta_sp[i] * m.delta_sp[i];
+            dot += maybe.delta_sp[i] * m.delta_sp[i];
+            dot_ax += maybe.axis[i] * m.axis[i];
+          }
+          double theta = std::acos(std::clamp(dot / (std::sqrt(n1) * std::sqrt(n2)), -1., 1.)) / (2 * M_PI) * 360;
+          double ax_theta = std::acos(std::clamp(dot_ax, -1., 1.)) / (2 * M_PI) * 360;
+          ax_theta = std::min(ax_theta, 180 - ax_theta);
+          if (std::sqrt(d_sp) < m_opt.mech_tol && std::sqrt(d_fwd) < m_opt.mech_tol && theta < 10 && ax_theta < 5)
+            return false;
+        }
+        return true;
+      }
+
+      // The masses that weight the Hessians are the potential's (setfl species masses); `map` names types.
+      Master(Options const& opt, system::Box const& box, system::TypeMap<Mass> const&, potential::Generic const& pot,
+             minimise::LBFGS const& min, Dimer const& dimer)
+          : m_opt(opt), m_box(box), m_pot(pot.handle()), m_ctx(gpu::context()), m_min(min.options()),
+            m_dimer(dimer.options()), m_rotor(dimer.rotor_options()) {
+        std::uint64_t s = opt.seed ? opt.seed : (static_cast<std::uint64_t>(std::random_device{}()) << 32) ^ std::random_device{}();
+        m_seed = s;
+      }
+
+      // Master::find_batch (find.cpp:684-755) at `centre` after `tot` searches there: batch_size searches
+      // from Master::perturb starts (re-perturbed up to three times after a spline error), Dimer::find_sp
+      // with `cache` as the known saddles and theta_tol = ((30 - 2.5) e^(-0.02 tot) + 2.5) degrees, then
+      // saddle_2_mech's two relaxations and checks for each converged search.  Converged saddles are
+      // appended to `cache`, as find_batch caches them for the site's later batches.
+      // With `mechs` (the site's mechanisms so far), every pathway is checked by is_new_mech and new ones
+      // are appended (Search::is_new).
+      auto find_batch(int tot, std::int64_t centre, system::SoA<Position const&, TypeID const&, Frozen const&> in,
+                      std::vector<system::SoA<Position>>& cache, std::vector<Mech>* mechs = nullptr)
+          -> std::vector<Search> {
+        std::int64_t const n = in.size();
+        int const k = m_opt.batch_size;
+        flyg_dimer_options dopt{m_dimer.n, m_dimer.max_steps, m_dimer.hist_check_freq, m_dimer.f2norm,
+                                m_dimer.skin_frac, m_dimer.proj_tol, m_dimer.max_trust, m_dimer.min_trust,
+                                m_dimer.grow_trust, m_dimer.shrink_trust, m_dimer.convex_max,
+                                m_dimer.use_history ? 1 : 0};
+        flyg_rotor_options ropt{m_rotor.n, m_rotor.iter_max_rot, m_rotor.delta_r, m_rotor.theta_tol,
+                                m_rotor.relax_in_convex ? 1 : 0};
+        flyg_batch_options bopt;
+        flyg_batch_default_options(&bopt);
+        bopt.precision = FLYG_FP64;   // the reference's precision
+        flyg_batch b = nullptr;
+        gpu::check(flyg_batch_create(m_ctx, m_pot, &dopt, &ropt, &bopt, m_box.basis_data(), m_box.periodic_data(),
+                                     in[r_].data(), in[id_].data(), n, &b),
+                   m_ctx);
+        struct Guard {
+          flyg_batch b;
+          ~Guard() { flyg_batch_destroy(b); }
+        } guard{b};
+        flyg_mech_options mo;
+        flyg_mech_default_options(&mo);
+        mo.enabled = 1;
+        mo.nudge_frac = m_opt.nudge_frac;
+        mo.basin_tol = m_opt.basin_tol;
+        mo.stationary_tol = m_opt.stationary_tol;
+        mo.minimiser = flyg_lbfgs_options{m_min.n, m_min.iter_max, m_min.f2norm, m_min.skin_frac, m_min.proj_tol,
+                                          m_min.max_trust, m_min.min_trust, m_min.grow_trust, m_min.shrink_trust};
+        gpu::check(flyg_batch_set_mechanisms(b, &mo), m_ctx);
+        double theta = 0;
+        gpu::check(flyg_theta_tol(tot, &theta), m_ctx);
+        std::vector<double> hist;
+        for (auto const& sp : cache) {
+          auto v = sp[r_];
+          hist.insert(hist.end(), v.data(), v.data() + v.size());
+        }
+        std::vector<std::int64_t> centres(static_cast<std::size_t>(k), centre), ids(static_cast<std::size_t>(k));
+        std::vector<double> radii(static_cast<std::size_t>(k), m_opt.active_radius);
+        std::vector<std::uint64_t> states(static_cast<std::size_t>(4 * k));
+        for (auto& w : states) w = splitmix64(m_seed);   // one independent Xoshiro256** state per search
+        gpu::check(flyg_batch_submit_perturbed(b, k, centres.data(), radii.data(), states.data(), m_opt.stddev,
+                                               m_opt.r_pert, theta, hist.empty() ? nullptr : hist.data(),
+                                               static_cast<int>(cache.size()), ids.data()),
+                   m_ctx);
+        gpu::check(flyg_batch_run(b), m_ctx);
+        std::vector<Search> out(static_cast<std::size_t>(k));
+        for (int q = 0; q < k; ++q) {
+          Search& r = out[static_cast<std::size_t>(q)];
+          r.sp.resize(static_cast<std::size_t>(3 * n));
+          r.axis.resize(static_cast<std::size_t>(3 * n));
+          gpu::check(flyg_batch_result(b, ids[q], &r.stats, r.sp.data(), r.axis.data()), m_ctx);
+          r.exit = r.stats.exit_code <= 4 ? static_cast<Dimer::Exit>(-r.stats.exit_code) : Dimer::uninit;
+          r.mech = flyg_mechanism{-1, 0, 0.0, 0.0, {0.0, 0.0, 0.0}};
+          if (r.stats.exit_code != FLYG_EXIT_SUCCESS) continue;
+          std::vector<double> rev(static_cast<std::size_t>(3 * n)), fwd(static_cast<std::size_t>(3 * n));
+          gpu::check(flyg_batch_mechanism(b, ids[q], &r.mech, rev.data(), fwd.data()), m_ctx);
+          system::SoA<Position> c(n);
+          auto cv = c[r_];
+          for (std::size_t i = 0; i < cv.size(); ++i) cv[i] = r.sp[i];
+          cache.push_back(std::move(c));
+          if (r.mech.code != 0) continue;
+          r.kinetic_pre = prefactor(centre, in, r.sp);
+          if (mechs) {
+            Mech m;
+            double len = 0;
+            for (std::int64_t i : site_atoms(centre, in)) {
+              for (int k = 0; k < 3; ++k) {
+                std::size_t e = static_cast<std::size_t>(3 * i + k);
+                m.delta_sp.push_back(r.sp[e] - rev[e]);
+                m.delta_fwd.push_back(fwd[e] - rev[e]);
+                m.axis.push_back(r.axis[e]);
+                len += r.axis[e] * r.axis[e];
+              }
+            }
+            for (double& a : m.axis) a /= std::sqrt(len);
+            r.is_new = is_new_mech(m, *mechs);
+            if (r.is_new) mechs->push_back(std::move(m));
+          }
+        }
+        return out;
+      }
+
+    private:
+      Options m_opt;
+      system::Box m_box;
+      flyg_eam m_pot;
+      flyg_ctx m_ctx;
+      minimise::LBFGS::Options m_min;
+      Dimer::Options m_dimer;
+      Rotor::Options m_rotor;
+      std::uint64_t m_seed;
+
+      // splitmix64 (Vigna), the recommended way to fill a Xoshiro256** state from one 64-bit seed.
+      static std::uint64_t splitmix64(std::uint64_t& x) {
+        std::uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return z ^ (z >> 31);
+      }
+
+      // The atoms of a site: those within active_radius of the centre (minimum image), ascending.
+      std::vector<std::int64_t> site_atoms(std::int64_t centre,
+                                           system::SoA<Position const&, TypeID const&, Frozen const&> const& in) const {
+        std::vector<std::int64_t> out;
+        auto pos = in[r_];
+        double const R2 = m_opt.active_radius * m_opt.active_radius;
+        for (std::int64_t i = 0; i < in.size(); ++i) {
+          double d2 = 0;
+          for (int c = 0; c < 3; ++c) {
+            double d = pos[3 * i + c] - pos[3 * centre + c];
+            double L = m_box.basis_data()[4 * c];
+            if (m_box.periodic(c)) d -= L * std::round(d / L);
+            d2 += d * d;
+          }
+          if (!(d2 > R2)) out.push_back(i);
+        }
+        return out;
+      }
+
+      // Mechanism::kinetic_pre (find.cpp:838-871, 287): the spectra of the mass-weighted Hessians at the
+      // initial minimum and at the saddle over the search's mobile atoms (those within active_radius).
+      double prefactor(std::int64_t centre, system::SoA<Position const&, TypeID const&, Frozen const&> const& in,
+                       std::vector<double> const& sp) const {
+        std::int64_t const n = in.size();
+        auto pos = in[r_];
+        std::vector<std::uint8_t> frozen(static_cast<std::size_t>(n), 1);
+        for (std::int64_t i : site_atoms(centre, in)) frozen[static_cast<std::size_t>(i)] = 0;
+        double rc = 0;
+        int nt = 0;
+        gpu::check(flyg_eam_info(m_pot, &nt, &rc, nullptr, nullptr), m_ctx);
+        flyg_list nl = nullptr;
+        gpu::check(flyg_list_create(m_ctx, m_box.basis_data(), m_box.periodic_data(), rc, &nl), m_ctx);
+        std::vector<double> w(static_cast<std::size_t>(3 * n));
+        double lp[2] = {0, 0};
+        std::int64_t nz[2], nneg[2];
+        double const* states[2] = {pos.data(), sp.data()};
+        for (int s = 0; s < 2; ++s) {
+          std::int64_t m = 0;
+          int rc1 = flyg_list_rebuild(nl, states[s], n, FLYG_HOST);
+          if (!rc1) rc1 = flyg_eam_hessian_eigenvalues(m_pot, nl, in[id_].data(), frozen.data(), n, FLYG_HOST, w.data(), &m);
+          if (!rc1) rc1 = flyg_log_prod_eigen(w.data(), m, m_opt.hessian_eigen_zero_tol, &lp[s], &nz[s], &nneg[s]);
+          if (rc1) {
+            flyg_list_destroy(nl);
+            gpu::check(rc1, m_ctx);
+          }
+        }
+        flyg_list_destroy(nl);
+        if (nneg[0] != 0 || nneg[1] != 1) return 0.0;   // not a minimum or not a first-order saddle
+        double a = 0;
+        gpu::check(flyg_prefactor(lp[0], lp[1], &a), m_ctx);
+        return a;
+      }
     };
 
   }  // namespace saddle
