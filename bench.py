@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -52,25 +53,53 @@ def algorithmic_bytes(counts, pos_bytes=8):
     return float(np.sum(8.0 * counts + 6 * pos_bytes + 56.0))
 
 
-def cpu_baseline(sys_, path, target_s=12.0):
-    """FP64 oracle (OpenMP, all host cores): one rebuild, then gradients for about target_s seconds."""
+def native_oracle():
+    """Builds the oracle with the reference's release flags (-O3 -march=native, CMakePresets.json:43) on
+    this host and points the oracle binding at it; returns a note on the build used."""
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                       capture_output=True, timeout=120)
+        os.environ["ORACLE_LIB"] = os.path.join(ROOT, "oracle", "_build_native", "liboracle.so")
+        return "g++ -O3 -march=native -fopenmp"
+    except (subprocess.SubprocessError, OSError) as e:
+        log(f"native oracle build failed ({e}); timing the portable x86-64-v3 build")
+        return "g++ -O3 -march=x86-64-v3 -fopenmp (native build failed)"
+
+
+def cpu_threads():
+    """Host cores this process may use (the affinity mask, else the CPU count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(sys_, path, build, repeats=5, block_s=2.0):
+    """FP64 oracle (OpenMP over the host cores, static schedule as eam.cpp:81,115): one rebuild, one
+    warm-up evaluation, then the median of `repeats` timed blocks of about block_s seconds each."""
     from oracle import pyoracle as po
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
     data = po.DataEAM(path)
     nl = po.List(po.Box(sys_.basis), R_CUT + SKIN)
     nl.rebuild(sys_.pos, threads)
     po.gradient(data, nl, sys_.types, sys_.frozen, threads)   # warm-up
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        po.gradient(data, nl, sys_.types, sys_.frozen, threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= target_s:
-            break
-    return {"value": sys_.n * reps / el, "unit": "atom-grads/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle EAM gradient of the same {sys_.n}-atom system, {reps} evaluations after "
-                      f"one rebuild, {el:.1f} s, OpenMP static schedule over {threads} threads"}
+    rates, evals = [], 0
+    for _ in range(repeats):
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            po.gradient(data, nl, sys_.types, sys_.frozen, threads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= block_s:
+                break
+        rates.append(sys_.n * reps / el)
+        evals += reps
+    return {"value": float(np.median(rates)), "unit": "atom-grads/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle EAM gradient of the same {sys_.n}-atom system after one rebuild: one warm-up, "
+                      f"median of {repeats} blocks of ~{block_s:g} s ({evals} evaluations), OpenMP static schedule "
+                      f"over {threads} threads, built {build}",
+            "repeats": [round(r, 1) for r in rates]}
 
 
 def explicit_starts(fa, s, centres, radii, prng):
@@ -101,10 +130,13 @@ def relax(ctx, fa, pot, s):
     return mn
 
 
-def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
+def search_bench(ctx, fa, systems, n_search=256, plain_sample=16, quick=False):
     """256 dimer searches on the 20^3-cell bcc Fe system with 20 vacancies and 40 interstitials
-    (relaxed: min0), active radius U(4, 9) A around defect-adjacent atoms, kick 0.6 A, |g| < 1e-3 eV/A,
-    FP32 pair arithmetic / FP64 accumulation, curvature, dots and barriers; potential wen.eam.fs."""
+    (relaxed: min0), active radius U(4, 9) A around defect-adjacent atoms, Master::perturb starts
+    (kick 0.6 A) drawn in the library, |g| < 1e-3 eV/A; the batched kernel in both precisions, with and
+    without mechanism checks, and the plain form (host-driven dimer, full-system FP64 gradients) on the
+    first plain_sample starts; potential wen.eam.fs.  quick: the mixed batch only (its autotune run and
+    one timed run), for profiling."""
     s = systems.fe_defects_20(3)
     pot = fa.EAM(ctx, WEN)
     t0 = time.perf_counter()
@@ -118,10 +150,12 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
     box = fa.Box(s.basis)
     dimer = dict(f2norm=1e-3)
     out = {"workload": f"bcc Fe 20^3 + 20 vacancies + 40 interstitials ({s.n} atoms) relaxed to min0, "
-                       f"{n_search} searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, potential wen.eam.fs",
+                       f"{n_search} searches, radius U(4,9) A, Master::perturb starts (stddev 0.6 A, r_pert 4 A), "
+                       "|g| < 1e-3 eV/A, potential wen.eam.fs",
            "relax_seconds": t_relax}
     res = {}
-    for name, prec in (("batched_mixed", fa.MIXED), ("batched_fp64", fa.FP64)):
+    legs = (("batched_mixed", fa.MIXED),) if quick else (("batched_mixed", fa.MIXED), ("batched_fp64", fa.FP64))
+    for name, prec in legs:
         log(f"search leg: {name} (autotune run + timed run of {n_search} searches)")
         b = fa.Batch(ctx, pot, box, s.pos, s.types, dimer=dimer, precision=prec)
         ids = b.submit_perturbed(centres, radii, prng.copy())
@@ -145,17 +179,12 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
                      "mean_mobile": float(np.mean([x["n_mobile"] for x in r])),
                      "mean_region": float(np.mean([x["n_region"] for x in r])),
                      "relaunched": int(sum(1 for x in r if x["launches"] > 1)),
+                     "reperturbed": int(sum(1 for x in r if x["retries"] > 0)),
                      "search_device_ms_p50": float(np.percentile(ms, 50)),
-                     "search_device_ms_p99": float(np.percentile(ms, 99))}
-    # Agreement of the mixed form with the FP64 form from the same starts (the FP64 batched path matches
-    # the FP64 oracle's exits, steps and barriers in tests/test_gpu_batch.py).
-    m, f = res["batched_mixed"], res["batched_fp64"]
-    agree = [abs(a["barrier"] - b["barrier"]) for a, b in zip(m, f) if a["exit"] == 0 and b["exit"] == 0]
-    good = sum(1 for a, b in zip(m, f) if a["exit"] == 0 and b["exit"] == 0 and abs(a["barrier"] - b["barrier"]) <= 1e-3)
-    el = out["batched_mixed"]["seconds"]
-    out["batched_mixed"]["matching_fp64_within_1e-3eV"] = good
-    out["batched_mixed"]["searches_per_s_dE_1e-3"] = good / el
-    out["batched_mixed"]["max_abs_dE_vs_fp64"] = float(max(agree)) if agree else None
+                     "search_device_ms_p99": float(np.percentile(ms, 99)),
+                     "search_device_ms_max": float(ms.max())}
+    if quick:
+        return out, None
     # The same mixed batch with every converged saddle turned into a checked min->sp->min mechanism.
     log("search leg: batched_mixed with mechanisms")
     T, L = out["batched_mixed"]["shape"]
@@ -172,61 +201,158 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=4):
                                        "converged": sum(1 for r in rm if r["exit"] == 0), "pathways": codes[0],
                                        "pathways_per_s": codes[0] / el, "mechanism_codes": codes,
                                        "minimiser_f2norm": MECH_F2NORM}
-    # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on a sample.
-    log(f"search leg: plain FP64 sample of {plain_sample}")
+    # Plain form (one search at a time, full-system FP64 gradients, host-driven dimer) on the first starts.
+    log(f"search leg: plain FP64 form on the first {plain_sample} starts")
     dim = fa.Dimer(ctx, box, dimer=dimer)
     starts, axes, frozen = explicit_starts(fa, s, centres[:plain_sample], radii[:plain_sample], prng[:plain_sample])
     t0 = time.perf_counter()
-    plain = []
-    for k in range(plain_sample):
-        plain.append(dim.find_sp(starts[k], axes[k], s.types, frozen[k], s.pos, pot))
+    plain = [dim.find_sp(starts[k], axes[k], s.types, frozen[k], s.pos, pot) for k in range(plain_sample)]
     el = time.perf_counter() - t0
+    # The plain form reports no barrier: FP64 energies at its saddles, after the timed run.
+    nl = fa.List(ctx, box, pot.r_cut())
+    nl.rebuild(s.pos)
+    fz0 = np.zeros(s.n, np.uint8)
+    e0 = pot.energy(s.types, fz0, nl)
+    for r in plain:
+        if r["exit"] == 0:
+            nl.rebuild(r["pos"])
+            r["barrier"] = pot.energy(s.types, fz0, nl) - e0
     out["plain_fp64_gpu"] = {"sample": plain_sample, "seconds": el, "searches_per_s": plain_sample / el,
-                             "exits_match_batched_fp64": int(sum(1 for k in range(plain_sample)
-                                                                 if plain[k]["exit"] == f[k]["exit"]))}
-    return out, (s, centres, radii, prng, dimer, m)
+                             "converged": int(sum(1 for r in plain if r["exit"] == 0)),
+                             "exits": np.bincount([r["exit"] for r in plain], minlength=7).tolist()}
+    return out, dict(s=s, centres=centres, radii=radii, prng=prng, dimer=dimer, res=res, plain=plain)
 
 
-def serve_bench(ctx, fa, systems, n_req=100, rate=200.0, pool_size=12):
-    """Serving: Poisson arrivals at 200/s of requests from 12 kMC systems (bcc Fe, N log-uniform in
-    [2000, 250000], 0.5-2 % vacancies + interstitials, each relaxed), 8-64 searches each, radii U(4, 9) A,
-    served with continuous batching; latency = arrival to last search done."""
+def qualify(out, S, orc):
+    """A search counts only if it converges and its barrier is within 1e-3 eV of the FP64 oracle's barrier
+    from the same start (the oracle must converge too).  Fills searches_per_s_dE_1e-3 and the ratios."""
+    ob = orc["barriers"]
+    def count(results, idx):
+        good, diffs = 0, []
+        for q, r in zip(idx, results):
+            if r["exit"] == 0 and ob[q] is not None:
+                d = abs(r["barrier"] - ob[q])
+                diffs.append(d)
+                good += int(d <= 1e-3)
+        return good, diffs
+    n_all = len(S["centres"])
+    for name in ("batched_mixed", "batched_fp64"):
+        good, diffs = count(S["res"][name], range(n_all))
+        o = out[name]
+        o["within_1e-3eV_of_oracle"] = good
+        o["searches_per_s_dE_1e-3"] = good / o["seconds"]
+        o["max_abs_dE_vs_oracle"] = float(max(diffs)) if diffs else None
+        o["both_converged"] = len(diffs)
+    k = len(S["plain"])
+    good, diffs = count(S["plain"], range(k))
+    p = out["plain_fp64_gpu"]
+    p["within_1e-3eV_of_oracle"] = good
+    p["searches_per_s_dE_1e-3"] = good / p["seconds"]
+    cpu = out["cpu_baseline"]
+    mixed = out["batched_mixed"]["searches_per_s_dE_1e-3"]
+    out["speedups_dE_1e-3"] = {
+        "batched_mixed_vs_plain_gpu": mixed / p["searches_per_s_dE_1e-3"] if good else None,
+        "batched_mixed_vs_cpu": mixed / cpu["searches_per_s_dE_1e-3"] if cpu["searches_per_s_dE_1e-3"] else None,
+        "note": "qualified searches/s: converged and within 1e-3 eV of the FP64 oracle's barrier from the same "
+                "start; batched over all 256 starts, plain GPU form and CPU over the first 16 of them"}
+
+
+def cpu_search_leg(S, build, sample=16, repeats=5):
+    """FP64 oracle, one search per thread over the host cores (find.cpp:697-732 with threads = 1 in each
+    search, find.cpp:990): (1) all 256 starts once, whose barriers qualify the device's searches;
+    (2) the CPU rate: one warm-up search, then the median of `repeats` runs over the first `sample` starts."""
+    from oracle import pyoracle as po
+    import concurrent.futures as cf
+    import openfly_amd as fa
+    s = S["s"]
+    threads = cpu_threads()
+    data = po.DataEAM(WEN)
+    box = po.Box(s.basis)
+    starts, axes, frozen = explicit_starts(fa, s, S["centres"], S["radii"], S["prng"])
+
+    def one(i):
+        return po.find_sp(box, data, starts[i], axes[i], s.types, frozen[i], s.pos, threads=1, dimer=S["dimer"])
+
+    def run(idx):
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            return list(ex.map(one, idx))
+
+    n_all = len(starts)
+    log(f"search leg: FP64 oracle on all {n_all} starts ({threads} threads)")
+    t0 = time.perf_counter()
+    full = run(range(n_all))
+    el_full = time.perf_counter() - t0
+
+    def energy(p):
+        nl = po.List(box, data.r_cut)
+        nl.rebuild(p, threads)
+        return po.energy(data, nl, s.types, np.zeros(s.n, np.uint8), threads)
+
+    e0 = energy(s.pos)
+    barriers = [energy(r["pos"]) - e0 if r["exit"] == 0 else None for r in full]
+    log(f"search leg: CPU rate, warm-up + median of {repeats} runs over {sample} starts")
+    one(0)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        run(range(sample))
+        times.append(time.perf_counter() - t0)
+    t_med = float(np.median(times))
+    conv_sample = sum(1 for r in full[:sample] if r["exit"] == 0)
+    base = {"value": sample / t_med, "unit": "searches/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle Dimer::find_sp on the first {sample} starts, one search per thread: one warm-up, "
+                      f"median of {repeats} runs ({t_med:.2f} s); built {build}",
+            "converged_in_sample": conv_sample, "searches_per_s_dE_1e-3": conv_sample / t_med,
+            "repeat_seconds": [round(t, 3) for t in times],
+            "all_starts": {"searches": n_all, "seconds": el_full, "searches_per_s": n_all / el_full,
+                           "converged": int(sum(1 for r in full if r["exit"] == 0)),
+                           "exits": np.bincount([r["exit"] for r in full], minlength=7).tolist()}}
+    return base, {"barriers": barriers}
+
+
+def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
+    """Serving: Poisson arrivals at 200/s of requests from a simulated kMC loop, each on its own system
+    (bcc Fe, N log-uniform in [2000, 250000], 0.5-2 % vacancies + interstitials, relaxed), 8-64 searches
+    each (radius U(4, 9) A, Master::perturb starts in the library), served with continuous batching;
+    latency = arrival to last search done.  Then the same systems at 0.9x the throughput sustained under
+    that overload (the highest rate the server keeps up with), for latency below saturation."""
     from openfly_amd import serve
     pot = fa.EAM(ctx, WEN)
-    log(f"serving leg: relaxing {pool_size} kMC systems")
+    log(f"serving leg: building and relaxing {n_req} kMC systems (one per request)")
     t0 = time.perf_counter()
-    pool = serve.make_pool(pool_size, seed=41, relax=lambda sy: relax(ctx, fa, pot, sy))
+    pool = serve.make_pool(n_req, seed=41, relax=lambda sy: relax(ctx, fa, pot, sy))
     t_pool = time.perf_counter() - t0
     srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
     log("serving leg: warm-up")
     serve.serve(srv, serve.make_requests(pool[:2], 4, rate=rate, seed=40, searches=(8, 16)))   # + shape autotune
-    log(f"serving leg: generating {n_req} requests")
-    reqs = serve.make_requests(pool, n_req, rate=rate, seed=42)
-    log("serving leg: replay")
+    reqs = serve.make_requests(pool, n_req, rate=rate, seed=42, one_system_each=True)
+    log(f"serving leg: replay of {n_req} requests at {rate:g}/s")
     out = serve.serve(srv, reqs, report=log)
-    # Latency below saturation: the same pool at half the request rate just measured as sustained.
-    rate2 = 0.5 * out["requests"] / out["wall_s"]
-    log(f"serving leg: replay at {rate2:.1f} requests/s (half the sustained rate)")
-    reqs2 = serve.make_requests(pool, 40, rate=rate2, seed=43)
+    sustained = out["requests"] / out["wall_s"]
+    rate2 = 0.9 * sustained
+    log(f"serving leg: replay at {rate2:.1f} requests/s (0.9x the sustained {sustained:.1f}/s)")
+    reqs2 = serve.make_requests(pool, n_req, rate=rate2, seed=43, one_system_each=True)
     light = serve.serve(srv, reqs2, report=log)
-    out["below_saturation"] = {k: light[k] for k in ("requests", "searches", "converged", "wall_s", "searches_per_s",
-                                                     "latency_p50_s", "latency_p99_s", "latency_max_s",
-                                                     "offered_requests_per_s", "mean_searches_per_request")}
+    keys = ("requests", "searches", "converged", "wall_s", "searches_per_s", "latency_p50_s", "latency_p99_s",
+            "latency_max_s", "mean_queue_wait_s", "offered_requests_per_s", "mean_searches_per_request")
+    out["sustained_requests_per_s"] = sustained
+    out["at_sustainable_rate"] = {k: light[k] for k in keys}
     st = srv.status()
-    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, from {pool_size} relaxed kMC systems (bcc Fe, N "
+    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, each on its own relaxed kMC system (bcc Fe, N "
                         "log-uniform [2000, 250000], 0.5-2% vacancies + interstitials), 8-64 searches per request, "
-                        "radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, FP32 pair / FP64 accumulation, wen.eam.fs",
+                        "radius U(4,9) A, Master::perturb starts, |g| < 1e-3 eV/A, FP32 pair / FP64 accumulation, "
+                        "wen.eam.fs",
                pool_relax_seconds=t_pool, launches=st["launches"], kernel_ms=st["kernel_ms"],
                block_threads=st["block_threads"], lanes_per_atom=st["lanes_per_atom"],
-               pool_atoms=[int(p.n) for p in pool])
+               system_atoms_min=int(min(p.n for p in pool)), system_atoms_max=int(max(p.n for p in pool)))
     return out
 
 
-def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
+def large_bench(ctx, fa, systems, n_search=1024):
     """Large system: bcc Fe 80^3 cells (1,028,096 atoms) with 0.1 % vacancies in clusters of 1-6 and 0.5 %
     interstitials, relaxed (min0); 1024 concurrent searches (radius U(4, 9) A, kick 0.6 A, |g| < 1e-3 eV/A)
-    in one launch of the autotuned persistent kernel; converged barriers re-evaluated as full-system FP64
-    energy differences E(sp) - E(min0) on a sample."""
+    in one launch of the autotuned persistent kernel; every converged barrier re-evaluated as a full-system
+    FP64 energy difference E(sp) - E(min0)."""
     from openfly_amd import serve
     t0 = time.perf_counter()
     s = systems.fe_clusters_80(5)
@@ -258,20 +384,26 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
     el = time.perf_counter() - t0
     st = srv.status()
     conv = [r for r in res if r["exit"] == 0]
-    # FP64 re-evaluation of a sample of converged barriers (full system, fresh list each time).
-    log(f"1M-atom leg: FP64 re-evaluation of {min(n_check, len(conv))} barriers")
+    # FP64 re-evaluation of every converged barrier (full system, fresh list each time).
+    log(f"1M-atom leg: FP64 re-evaluation of all {len(conv)} converged barriers")
+    t0 = time.perf_counter()
     nl = fa.List(ctx, box, pot.r_cut())
     frozen = np.zeros(s.n, np.uint8)
     nl.rebuild(s.pos)
     e0 = pot.energy(s.types, frozen, nl)
+    L = np.diag(s.basis)
     errs, pairs = [], []
-    for r in conv[:n_check]:
-        p = s.pos.copy()
+    p = s.pos.copy()
+    for r in conv:
         p[r["atoms"]] = r["pos"]
         nl.rebuild(p)
         de = pot.energy(s.types, frozen, nl) - e0
+        p[r["atoms"]] = s.pos[r["atoms"]]
         errs.append(abs(de - r["barrier"]))
-        pairs.append((r["barrier"], de, float(np.abs(r["pos"] - s.pos[r["atoms"]]).max())))
+        d = r["pos"] - s.pos[r["atoms"]]
+        d -= L * np.round(d / L)   # minimum image
+        pairs.append((r["barrier"], de, float(np.sqrt(np.einsum("ij,ij->i", d, d)).max())))
+    t_reeval = time.perf_counter() - t0
     ms = np.array([r["device_ms"] for r in res])
     return {"workload": f"bcc Fe 80^3 cells ({s.n} atoms), 0.1% vacancies in clusters of 1-6, 0.5% interstitials, "
                         f"relaxed; {n_search} concurrent searches, radius U(4,9) A, kick 0.6 A, |g| < 1e-3 eV/A, "
@@ -281,50 +413,103 @@ def large_bench(ctx, fa, systems, n_search=1024, n_check=48):
             "exits": np.bincount([r["exit"] for r in res], minlength=7).tolist(),
             "block_threads": st["block_threads"], "lanes_per_atom": st["lanes_per_atom"],
             "search_device_ms_p50": float(np.percentile(ms, 50)), "search_device_ms_p99": float(np.percentile(ms, 99)),
-            "fp64_reevaluated": len(errs), "max_abs_barrier_error_eV": float(max(errs)) if errs else None,
-            "sample_barrier_fp64_maxdisp": [[round(a, 6), round(b, 6), round(c, 3)] for a, b, c in pairs[:5]]}
+            "fp64_reevaluated": len(errs), "fp64_reevaluation_seconds": t_reeval,
+            "max_abs_barrier_error_eV": float(max(errs)) if errs else None,
+            "barrier_error_eV_p50": float(np.median(errs)) if errs else None,
+            "first_barrier_fp64_maxdisp": [[round(a, 6), round(b, 6), round(c, 3)] for a, b, c in pairs[:5]]}
 
 
-def cpu_search_baseline(data_args, cpu_sample=16):
-    """FP64 oracle, one search per thread over the host cores (find.cpp:697-732 with threads = 1), on the
-    first cpu_sample starts of the batched leg; also the agreement of the device's mixed-precision
-    barriers with the oracle's for the same starts."""
-    from oracle import pyoracle as po
-    import concurrent.futures as cf
-    import openfly_amd as fa
-    s, centres, radii, prng, dimer, mixed = data_args
-    threads = os.cpu_count() or 1
-    k = min(cpu_sample, threads)
-    data = po.DataEAM(WEN)
-    box = po.Box(s.basis)
-    starts, axes, frozen = explicit_starts(fa, s, centres[:k], radii[:k], prng[:k])
+def gradient_leg(ctx, fa, torch, s, path, precision, steps, warmup, skin, pool_n=16):
+    """The incremental list + EAM gradient loop on system `s`: each step moves every atom by a random vector
+    of norm <= 0.05 A (pre-generated, resident in HBM), updates the list (a rebuild once the accumulated
+    displacement exceeds skin / 2) and evaluates the gradient.  Returns the wall time of the timed steps,
+    the kernel time per gradient (the library's HIP event pair on its stream) and the algorithmic bytes."""
+    pot = fa.EAM(ctx, path)
+    pot.set_precision(fa.MIXED if precision == "mixed" else fa.FP64)
+    dev = torch.device("cuda", 0)
+    # Inputs resident in HBM before timing.
+    pos = torch.from_numpy(s.pos.copy()).to(dev)
+    types = torch.from_numpy(s.types.astype(np.uint32)).to(dev)
+    frozen = torch.from_numpy(s.frozen).to(dev)
+    grad = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
+    scratch = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(12)
+    pool, pool_neg, pool_max = [], [], []
+    for _ in range(pool_n):
+        d = rng.normal(size=(s.n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        d *= 0.05 * rng.random((s.n, 1)) ** (1.0 / 3.0)   # uniform in the 0.05 A ball
+        pool.append(torch.from_numpy(d).to(dev))
+        pool_neg.append(torch.from_numpy(-d).to(dev))
+        pool_max.append(float(np.max(np.linalg.norm(d, axis=1))))
+    torch.cuda.synchronize()
 
-    def one(i):
-        return po.find_sp(po.Box(s.basis), data, starts[i], axes[i], s.types, frozen[i], s.pos, threads=1,
-                          dimer=dimer)
+    nl = fa.List(ctx, fa.Box(s.basis), R_CUT + skin)
+    nl.rebuild(pos)
+    off, _, _, _ = nl.pairs(R_CUT + skin)   # stored rows (n_i = stored list length)
+    counts = np.diff(off).astype(np.float64)
 
+    state = {"acc": 0.0, "k": 0, "rebuilds": 0}
+    def step():
+        # Steps 2j and 2j+1 apply +d and -d of pool entry j, so positions oscillate about the thermal
+        # configuration instead of drifting; every step still moves every atom by up to 0.05 A.
+        k = (state["k"] // 2) % pool_n
+        sign = 1.0 if state["k"] % 2 == 0 else -1.0
+        state["k"] += 1
+        nl.update(pool[k] if sign > 0 else pool_neg[k])
+        state["acc"] += pool_max[k]
+        if state["acc"] > 0.5 * skin:
+            fa.lib().flyg_list_positions(nl.h, fa.flygpu.C.c_void_p(scratch.data_ptr()), fa.DEVICE)
+            nl.rebuild(scratch)
+            state["acc"] = 0.0
+            state["rebuilds"] += 1
+        pot.gradient(types, frozen, nl, out=grad)
+
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    state["rebuilds"] = 0
+    pot.set_timing(True)   # the library's own HIP event pair around each call's two kernels
     t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=k) as ex:
-        res = list(ex.map(one, range(k)))
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    lib_ms, lib_calls = pot.timing()
+    pot.set_timing(False)
+    # Kernel time per gradient from the event pair the library records on its stream right before the
+    # density launch and right after the force launch (an event pair recorded around the Python call
+    # would also hold the host time of the call whenever the stream had drained before the launch).
+    kern_ms = lib_ms / lib_calls
+    b = algorithmic_bytes(counts, 8)
+    return {"atoms": int(s.n), "precision": precision, "steps": steps, "seconds": el,
+            "atom_grads_per_s": s.n * steps / el, "ms_per_step": el * 1e3 / steps, "kernel_ms": kern_ms,
+            "algorithmic_bytes": b, "achieved_GBs": b / (kern_ms * 1e-3) / 1e9, "rebuilds": state["rebuilds"],
+            "mean_neighbours": float(counts.mean()), "counts": counts}
 
-    def energy(p):
-        nl = po.List(box, data.r_cut)
-        nl.rebuild(p, threads)
-        return po.energy(data, nl, s.types, np.zeros(s.n, np.uint8), threads)
 
-    e0 = energy(s.pos)
-    both, within, worst = 0, 0, 0.0
-    for i, r in enumerate(res):
-        if r["exit"] == 0 and mixed[i]["exit"] == 0:
-            d = abs((energy(r["pos"]) - e0) - mixed[i]["barrier"])
-            both += 1
-            within += int(d <= 1e-3)
-            worst = max(worst, d)
-    return {"value": k / el, "unit": "searches/s", "cores": k, "kind": "port",
-            "sample": f"FP64 oracle Dimer::find_sp on the first {k} of the same starts, one search per thread, "
-                      f"{el:.1f} s, {sum(1 for r in res if r['exit'] == 0)} converged",
-            "oracle_vs_mixed": {"both_converged": both, "within_1e-3eV": within, "max_abs_dE": worst}}
+def extra_gradient_legs(ctx, fa, torch, systems, steps, warmup, skin):
+    """The same loop at reference precision (FP64) on the alloy, on the 16^3 single-vacancy cell (FP64,
+    Fe-only synthetic potential) and on the 1,028,096-atom 80^3 cluster system (mixed and FP64)."""
+    legs = {}
+    cases = (("fe_x_alloy_30_fp64", lambda: systems.fe_x_alloy(seed=2, cells=30), ("Fe", "X"), "fp64"),
+             ("fe_vacancy_16_fp64", lambda: systems.fe_vacancy_16(seed=1), ("Fe",), "fp64"),
+             ("fe_clusters_80_mixed", lambda: systems.fe_clusters_80(5), ("Fe",), "mixed"),
+             ("fe_clusters_80_fp64", lambda: systems.fe_clusters_80(5), ("Fe",), "fp64"))
+    cache = {}
+    for name, make, species, prec in cases:
+        key = name.rsplit("_", 1)[0]
+        if key not in cache:
+            cache.clear()
+            cache[key] = make()
+        s = cache[key]
+        log(f"gradient leg {name}: {s.n} atoms")
+        g = gradient_leg(ctx, fa, torch, s, systems.potential_path(species), prec, steps, warmup, skin)
+        g.pop("counts")
+        legs[name] = g
+    return legs
 
 
 def main():
@@ -336,7 +521,11 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--skin", type=float, default=LIST_SKIN, help="Verlet skin of the gradient leg's list (A)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the FP64, 16^3 and 80^3 gradient legs beside the headline one")
     ap.add_argument("--no-search", action="store_true", help="skip the batched saddle-search leg")
+    ap.add_argument("--search-quick", action="store_true",
+                    help="search leg: the mixed batch only, no FP64 batch, mechanisms or plain form (profiling)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served saddle-search leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 1M-atom, 1024-search leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -357,6 +546,7 @@ def main():
               "single MI355X and is not sharded across GPUs.", file=sys.stderr)
         sys.exit(2)
 
+    build = native_oracle() if not args.no_cpu else None
     import torch
     import openfly_amd as fa
     from openfly_amd import systems
@@ -364,72 +554,9 @@ def main():
     s = systems.fe_x_alloy(seed=2, cells=args.cells)
     path = systems.potential_path(("Fe", "X"))
     ctx = fa.Context(0)
-    pot = fa.EAM(ctx, path)
-    pot.set_precision(fa.MIXED if args.precision == "mixed" else fa.FP64)
-    dev = torch.device("cuda", 0)
-
-    # Inputs resident in HBM before timing.
-    pos = torch.from_numpy(s.pos.copy()).to(dev)
-    types = torch.from_numpy(s.types.astype(np.int32)).to(dev)
-    frozen = torch.from_numpy(s.frozen).to(dev)
-    grad = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
-    scratch = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
-    rng = np.random.default_rng(12)
-    pool_n = 16
-    pool = []
-    pool_neg = []
-    pool_max = []
-    for _ in range(pool_n):
-        d = rng.normal(size=(s.n, 3))
-        d /= np.linalg.norm(d, axis=1, keepdims=True)
-        d *= 0.05 * rng.random((s.n, 1)) ** (1.0 / 3.0)   # uniform in the 0.05 A ball
-        pool.append(torch.from_numpy(d).to(dev))
-        pool_neg.append(torch.from_numpy(-d).to(dev))
-        pool_max.append(float(np.max(np.linalg.norm(d, axis=1))))
-    torch.cuda.synchronize()
-
-    nl = fa.List(ctx, fa.Box(s.basis), R_CUT + args.skin)
-    nl.rebuild(pos)
-    off, j, sh, dr = nl.pairs(R_CUT + args.skin)   # stored rows (n_i = stored list length)
-    counts = np.diff(off).astype(np.float64)
-    bytes_per_grad = algorithmic_bytes(counts, 8)
-
-    state = {"acc": 0.0, "k": 0, "rebuilds": 0}
-    def step():
-        # Steps 2j and 2j+1 apply +d and -d of pool entry j, so positions oscillate about the thermal
-        # configuration instead of drifting; every step still moves every atom by up to 0.05 A.
-        k = (state["k"] // 2) % pool_n
-        sign = 1.0 if state["k"] % 2 == 0 else -1.0
-        state["k"] += 1
-        nl.update(pool[k] if sign > 0 else pool_neg[k])
-        state["acc"] += pool_max[k]
-        if state["acc"] > 0.5 * args.skin:
-            fa.lib().flyg_list_positions(nl.h, fa.flygpu.C.c_void_p(scratch.data_ptr()), fa.DEVICE)
-            nl.rebuild(scratch)
-            state["acc"] = 0.0
-            state["rebuilds"] += 1
-        pot.gradient(types, frozen, nl, out=grad)
-
     log(f"gradient leg: {s.n} atoms, {args.warmup} warm-up + {args.steps} timed steps")
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    state["rebuilds"] = 0
-    pot.set_timing(True)   # the library's own HIP event pair around each call's two kernels
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    lib_ms, lib_calls = pot.timing()
-    pot.set_timing(False)
-
-    # Kernel time per gradient from the event pair the library records on its stream right before the
-    # density launch and right after the force launch (an event pair recorded around the Python call
-    # would also hold the host time of the call whenever the stream had drained before the launch).
-    kern_ms = lib_ms / lib_calls
+    g = gradient_leg(ctx, fa, torch, s, path, args.precision, args.steps, args.warmup, args.skin)
+    el, kern_ms, bytes_per_grad, counts = g["seconds"], g["kernel_ms"], g["algorithmic_bytes"], g["counts"]
     achieved = bytes_per_grad / (kern_ms * 1e-3) / 1e9
     value = s.n * args.steps / el
     out = {
@@ -450,7 +577,7 @@ def main():
                         "random displacement <= 0.05 A per step, list update/rebuild, EAM gradient",
             "atoms": s.n,
             "mean_neighbours": float(counts.mean()),
-            "rebuilds_in_timed_steps": state["rebuilds"],
+            "rebuilds_in_timed_steps": g["rebuilds"],
             "list_cutoff": R_CUT + args.skin,
             "parallelism": "single GPU",
         },
@@ -468,12 +595,14 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_grad,
         },
     }
+    if not args.no_extra:
+        out["gradient_legs"] = extra_gradient_legs(ctx, fa, torch, systems, min(args.steps, 100),
+                                                   min(args.warmup, 10), args.skin)
+        log("extra gradient legs done")
+    sargs = None
     if not args.no_search:
-        sr, sargs = search_bench(ctx, fa, systems)
+        sr, sargs = search_bench(ctx, fa, systems, quick=args.search_quick)
         log("search leg done")
-        if not args.no_cpu:
-            log("search leg: CPU oracle sample")
-            sr["cpu_baseline"] = cpu_search_baseline(sargs)
         out["searches"] = sr
     if not args.no_serve:
         out["serving"] = serve_bench(ctx, fa, systems)
@@ -484,7 +613,11 @@ def main():
         log("1M-atom leg done")
     if not args.no_cpu:
         log("gradient leg: CPU oracle baseline")
-        out["cpu_baseline"] = cpu_baseline(s, path)
+        out["cpu_baseline"] = cpu_baseline(s, path, build)
+        if sargs is not None:
+            sr["cpu_baseline"], orc = cpu_search_leg(sargs, build)
+            qualify(sr, sargs, orc)
+            log("search leg: CPU oracle done")
     print(json.dumps(out), flush=True)
 
 

@@ -97,15 +97,16 @@ def make_pool(count, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), re
     return pool
 
 
-def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0)):
+def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0), one_system_each=False):
     """Poisson arrivals at `rate`/s, each from a uniformly chosen system of `pool`, asking for a uniform
     number of searches in `searches` around defect-adjacent atoms with radii uniform in `radii`; each search
-    starts from Master::perturb (in the library) with its own generator state."""
+    starts from Master::perturb (in the library) with its own generator state.  one_system_each: request q
+    comes from pool[q] (a kMC loop whose system changed since its last request) instead of a random one."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     reqs = []
     for q in range(count):
-        s = pool[int(rng.integers(len(pool)))]
+        s = pool[q % len(pool)] if one_system_each else pool[int(rng.integers(len(pool)))]
         k = int(rng.integers(searches[0], searches[1] + 1))
         centres = sample_centres(s, k, rng)
         rad = rng.uniform(radii[0], radii[1], k)
