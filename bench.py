@@ -35,7 +35,7 @@ MECH_F2NORM = 1e-4   # |g| tolerance of the mixed form's mechanism relaxations (
 # (profiles/eam_gradient_fe30_x_alloy.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
 # reading for a copy kernel of known size (python bench.py --copy-calibration: a 64 MiB copy reads
 # FETCH_SIZE 32780.9 KiB and WRITE_SIZE 65536 KiB).
-GRADIENT_TRAFFIC = {"fetch_kib": 12744.2 + 13129.4, "write_kib": 1704.0 + 1278.6,
+GRADIENT_TRAFFIC = {"fetch_kib": 12523.9 + 13127.6, "write_kib": 1704.0 + 1278.2,
                     "fetch_scale": 65536.0 / 32780.9, "write_scale": 1.0}
 
 
@@ -172,7 +172,12 @@ def search_bench(ctx, fa, systems, n_search=256, plain_sample=16, quick=False):
         res[name] = r
         ok = sum(1 for x in r if x["exit"] == 0)
         ms = np.array([x["device_ms"] for x in r])
+        grads = sum(x["grads"] for x in r)
         out[name] = {"seconds": el, "first_run_seconds_with_autotune": first, "shape": [T, L],
+                     # device time per region gradient, and the part of it inside the gradient passes (the
+                     # rest is the rotor / L-BFGS algebra and barriers of the search)
+                     "device_us_per_gradient": 1e3 * float(ms.sum()) / grads,
+                     "gradient_pass_us_per_gradient": 1e3 * sum(x["device_grad_ms"] for x in r) / grads,
                      "converged": ok, "searches_per_s": n_search / el,
                      "converged_per_s": ok / el, "exits": np.bincount([x["exit"] for x in r], minlength=7).tolist(),
                      "mean_grads": float(np.mean([x["grads"] for x in r])),
