@@ -236,11 +236,15 @@ typedef struct {
   double curv;     /* last curvature along the axis */
   double energy;   /* batched searches: FP64 barrier E(saddle) - E(base); plain dimer: 0 */
   double max_disp; /* batched searches: largest displacement of a mobile atom from its base position */
-  int launches;    /* batched searches: kernel launches used (a search that outgrew its region is re-run) */
+  int launches;    /* batched searches: kernel launches used (a search that outgrew its region runs again in a
+                      larger one, continuing from where it left: see resumes) */
   int rot_iters;   /* batched searches: rotor iterations */
-  double device_ms; /* batched searches: device time its workgroup spent on the search (last launch) */
+  double device_ms; /* batched searches: device time its workgroups spent on the search (all launches since its
+                       start state) */
   double device_grad_ms; /* ... of which in gradient evaluations */
   int retries;     /* batched searches started by Master::perturb: re-perturbations after a spline error */
+  int resumes;     /* batched searches: launches that continued from the state saved at a region exit (the rest
+                      of launches - 1 restarted from the start state) */
 } flyg_search_stats;
 
 void flyg_dimer_default_options(flyg_dimer_options* opt);

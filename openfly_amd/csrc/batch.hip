@@ -235,6 +235,15 @@ namespace flyg {
       std::vector<int> mobile;      // global indices (sorted)
       std::vector<double> x, ax;    // 3*nm start state (continuous with the base positions)
       std::shared_ptr<Region> region;   // built for the skins, reset when they grow
+      // State saved at a region exit (resume_len doubles) and the loop position it belongs to: the
+      // relaunch in the grown region continues from it instead of from the start.
+      bool resume = false;
+      std::vector<double> rs;
+      int r_step = 0, r_convex = 0, r_k = 0, r_rot_iters = 0, r_refreshes = 0;
+      long r_grads = 0;
+      double r_trust = 0, r_maxd = 0, r_maxp = 0;
+      double prior_ms = 0, prior_grad_ms = 0;   // device time of the launches before this one
+      int resumes = 0;                           // launches that continued from a saved state
     };
 
     struct Result {
@@ -485,6 +494,10 @@ namespace flyg {
       return d;
     }
 
+    // Slots per launch for the saved states of searches that leave their region (a search that finds
+    // them all taken restarts from its start in the grown region instead of continuing).
+    constexpr int RS_SLOTS = 64;
+
     // Pinned host buffer (hipHostMalloc), grown on demand.
     struct Pinned {
       char* p = nullptr;
@@ -529,6 +542,9 @@ namespace flyg {
       DBuf<char> d_in, d_scr;
       Pinned h_in, h_out;
       std::size_t o_out = 0, o_ox = 0, o_oax = 0, o_rev = 0, o_fwd = 0, out_bytes = 0;   // offsets in h_out
+      std::size_t o_rs = 0;         // saved states of region exits (RS_SLOTS slots of rs_len doubles)
+      long rs_len = 0;
+      int dopt_n = 0;               // translation core size the states were saved with
       int mech = 0;
       std::vector<long> v3off;      // per search: first element of its 3*nm output vectors
       std::vector<char> harvested;  // per search
@@ -573,6 +589,7 @@ namespace flyg {
       for (auto const& P : work) {
         Region const& g = *P.region;
         int const nh = P.hist ? P.hist->nh : 0;
+        b += P.rs.size() * 8;
         b += sizeof(SearchHdr) + static_cast<std::size_t>(g.nr) * (4 + 4 + 4 + 32 + 32 + 1 + 8 + 8) + g.ent.size() * 8 +
              static_cast<std::size_t>(3 * g.nm) * 8 * (4 + nh + nvec(dopt.n, dopt.rn)) + 8 * nh + sizeof(SearchOut) +
              3 * 256;
@@ -586,7 +603,7 @@ namespace flyg {
       require(!F.busy, "internal: flight already in use");
       F.work = std::move(work);
       long const ns = static_cast<long>(F.work.size());
-      long nreg = 0, nrow = 0, nent = 0, n3 = 0, nws = 0, nhv = 0, nhn = 0;
+      long nreg = 0, nrow = 0, nent = 0, n3 = 0, nws = 0, nhv = 0, nhn = 0, nrs = 0, rs_len = 0;
       std::vector<SearchHdr> hdr(ns);
       for (long k = 0; k < ns; ++k) {
         Pending const& P = F.work[k];
@@ -613,6 +630,22 @@ namespace flyg {
         h.hist_off = nhv;
         h.hn_off = nhn;
         h.cos_tol = P.cos_tol;
+        h.resume = P.resume ? 1 : 0;
+        if (P.resume) {
+          require(static_cast<long>(P.rs.size()) == resume_len(3 * g.nm, dopt.n), "internal: saved search state size");
+          h.r_step = P.r_step;
+          h.r_convex = P.r_convex;
+          h.r_k = P.r_k;
+          h.r_rot_iters = P.r_rot_iters;
+          h.r_refreshes = P.r_refreshes;
+          h.r_grads = P.r_grads;
+          h.r_trust = P.r_trust;
+          h.r_maxd = P.r_maxd;
+          h.r_maxp = P.r_maxp;
+          h.rs_in_off = nrs;
+          nrs += static_cast<long>(P.rs.size());
+        }
+        rs_len = std::max(rs_len, resume_len(3 * g.nm, dopt.n));
         nhv += 3L * g.nm * h.n_hist;
         nhn += h.n_hist;
         nreg += g.nr;
@@ -626,12 +659,12 @@ namespace flyg {
       Layout in;
       std::size_t o_hdr = in.add<SearchHdr>(ns), o_gidx = in.add<int>(nreg), o_row = in.add<int>(nrow),
                   o_ent = in.add<unsigned>(nent), o_sx = in.add<double>(n3), o_sax = in.add<double>(n3),
-                  o_hv = in.add<double>(nhv), o_hn = in.add<double>(nhn);
+                  o_hv = in.add<double>(nhv), o_hn = in.add<double>(nhn), o_rs = in.add<double>(nrs);
       Layout sc;
       std::size_t o_cur = sc.add<double4>(nreg), o_base = sc.add<double4>(nreg), o_type = sc.add<uint8_t>(nreg),
                   o_rhof = sc.add<double>(nreg), o_fp = sc.add<double>(nreg), o_act = sc.add<unsigned>(nent),
                   o_alen = sc.add<int>(nreg), o_ws = sc.add<double>(nws),
-                  o_q = sc.add<unsigned>(1);
+                  o_q = sc.add<unsigned>(1), o_rsc = sc.add<unsigned>(1);
       std::size_t zero_end = sc.size;
       Layout ho;
       std::size_t o_flags = ho.add<int>(ns);
@@ -640,6 +673,8 @@ namespace flyg {
       F.o_oax = ho.add<double>(n3);
       F.o_rev = ho.add<double>(dopt.mech ? n3 : 0);
       F.o_fwd = ho.add<double>(dopt.mech ? n3 : 0);
+      F.rs_len = rs_len;
+      F.o_rs = ho.add<double>(RS_SLOTS * rs_len);
       F.out_bytes = ho.size;
       // Grow with slack: freeing device or pinned memory synchronises the whole device, which would
       // stall the other streams' launches.
@@ -677,6 +712,7 @@ namespace flyg {
               for (int c = 0; c < 3; ++c) hv[(static_cast<long>(q) * g.nm + i) * 3 + c] = H.d[q * H.n3 + 3L * P.mobile[i] + c];
           std::copy(H.norm.begin(), H.norm.end(), reinterpret_cast<double*>(hb + o_hn) + hdr[k].hn_off);
         }
+        if (P.resume) std::copy(P.rs.begin(), P.rs.end(), reinterpret_cast<double*>(hb + o_rs) + hdr[k].rs_in_off);
       }
       F.d_in.reserve(grow(in.size, F.d_in.cap));
       F.d_scr.reserve(grow(sc.size, F.d_scr.cap));
@@ -704,6 +740,11 @@ namespace flyg {
       A.start_ax = reinterpret_cast<double const*>(di + o_sax);
       A.hist = reinterpret_cast<double const*>(di + o_hv);
       A.hnorm = reinterpret_cast<double const*>(di + o_hn);
+      A.rs_in = reinterpret_cast<double const*>(di + o_rs);
+      A.rs_count = reinterpret_cast<unsigned*>(ds + o_rsc);
+      A.rs_slots = RS_SLOTS;
+      A.rs_len = rs_len;
+      A.rs_out = reinterpret_cast<double*>(F.h_out.p + F.o_rs);
       char* hb_out = F.h_out.p;
       std::memset(hb_out + o_flags, 0, ns * sizeof(int));
       A.done = reinterpret_cast<int*>(hb_out + o_flags);
@@ -712,6 +753,7 @@ namespace flyg {
       A.out_rev = reinterpret_cast<double*>(hb_out + F.o_rev);
       A.out_fwd = reinterpret_cast<double*>(hb_out + F.o_fwd);
       F.mech = dopt.mech;
+      F.dopt_n = dopt.n;
       A.out = reinterpret_cast<SearchOut*>(hb_out + F.o_out);
       F.v3off.resize(ns);
       for (long k = 0; k < ns; ++k) F.v3off[k] = hdr[k].v3_off;
@@ -784,6 +826,28 @@ namespace flyg {
           q.skin_f = std::min(cap, std::max(q.skin_f * 1.5, o.max_disp + 1.0));
           q.skin_m = std::min(cap, std::max(std::max(q.skin_m * 1.5, q.skin_f), o.max_pair + 1.5));
           q.region.reset();
+          if (o.saved > 0 && o.saved <= RS_SLOTS) {   // continue where it left its region
+            double const* src = reinterpret_cast<double const*>(F.h_out.p + F.o_rs) + (o.saved - 1) * F.rs_len;
+            q.resume = true;
+            q.rs.assign(src, src + resume_len(3 * nm, F.dopt_n));
+            q.r_step = o.steps;
+            q.r_convex = o.convex;
+            q.r_k = o.lbfgs_k;
+            q.r_rot_iters = o.rot_iters;
+            q.r_refreshes = o.refreshes;
+            q.r_grads = o.grads;
+            q.r_trust = o.trust;
+            q.r_maxd = o.max_disp;
+            q.r_maxp = o.max_pair;
+            q.prior_ms += static_cast<double>(o.ticks) / wall_clock_khz();
+            q.prior_grad_ms += static_cast<double>(o.grad_ticks) / wall_clock_khz();
+            ++q.resumes;
+          } else {   // restart from the start state (the counters start over too)
+            q.resume = false;
+            q.rs.clear();
+            q.prior_ms = q.prior_grad_ms = 0;
+            q.resumes = 0;
+          }
           requeue.push_back(std::move(q));
           continue;
         }
@@ -798,9 +862,10 @@ namespace flyg {
         R.st.max_disp = o.max_disp;
         R.st.launches = P.launches;
         R.st.retries = P.retries;
+        R.st.resumes = P.resumes;
         R.st.rot_iters = o.rot_iters;
-        R.st.device_ms = static_cast<double>(o.ticks) / wall_clock_khz();
-        R.st.device_grad_ms = static_cast<double>(o.grad_ticks) / wall_clock_khz();
+        R.st.device_ms = P.prior_ms + static_cast<double>(o.ticks) / wall_clock_khz();
+        R.st.device_grad_ms = P.prior_grad_ms + static_cast<double>(o.grad_ticks) / wall_clock_khz();
         R.mobile = P.mobile;
         R.x.assign(ox + off, ox + off + 3L * nm);
         R.ax.assign(oax + off, oax + off + 3L * nm);
@@ -1157,6 +1222,7 @@ int flyg_batch_run(flyg_batch b) {
       R.done = 1;
       R.st.exit_code = EXIT_REGION;
       R.st.launches = P.launches;
+      R.st.resumes = P.resumes;
       R.mobile = P.mobile;
     };
     auto free_flight = [&]() -> Flight* {

@@ -38,6 +38,10 @@ namespace flyg {
     // the positions at the last active-list refresh.
     __host__ __device__ inline int nvec(int m, int rm) { return 17 + 2 * (m + rm); }
 
+    // Saved dimer state of a search that left its region (x, ax, the translation core s, y, x_prev,
+    // g_prev, q, then its m rho values): what a relaunch in a grown region continues from.
+    __host__ __device__ inline long resume_len(int n3, int m) { return static_cast<long>(2 * m + 5) * n3 + m; }
+
     // Active lists: the part of each candidate row within r_c + ACT_SKIN_F (pairs with a frozen atom)
     // or r_c + ACT_SKIN_M (mobile pairs) at the last refresh, rebuilt on the device when the mobile
     // atoms have moved too far since (a Verlet list inside the candidate rows).
@@ -64,6 +68,14 @@ namespace flyg {
       int n_hist;
       long hist_off, hn_off;
       double cos_tol;         // cos(theta_tol): a known saddle closer in angle than theta_tol is a collision
+      // Resumption after a region exit: the host grew the skins and the search continues from the state
+      // it saved (SearchOut::saved) at rs_in_off of BatchArgs::rs_in, re-entering its step loop at step
+      // r_step right after that step's translation, with the counters and trust it had.
+      int resume;
+      int r_step, r_convex, r_k, r_rot_iters, r_refreshes;
+      long r_grads;
+      double r_trust, r_maxd, r_maxp;
+      long rs_in_off;
     };
 
     struct SearchOut {
@@ -76,6 +88,11 @@ namespace flyg {
       double drift[3];                   // centroid drift of rev from the initial minimum (all atoms)
       long ticks;        // device wall-clock ticks the workgroup spent on this search
       long grad_ticks;   // ... of which inside region gradients
+      // Region exit inside the step loop: 1 + the slot of BatchArgs::rs_out holding the saved state
+      // (0: not saved, the search restarts), with the translation core's pair count, the convex-step
+      // count and the trust radius at the exit.
+      int saved, lbfgs_k, convex;
+      double trust;
     };
 
     struct DimOpt {
@@ -111,6 +128,11 @@ namespace flyg {
       double* out_rev;           // mechanism minima (when mechanisms are on)
       double* out_fwd;
       SearchOut* out;
+      double const* rs_in;       // saved states of resumed searches (SearchHdr::rs_in_off)
+      double* rs_out;            // rs_slots slots of rs_len doubles (host-visible memory)
+      unsigned* rs_count;        // slots taken
+      int rs_slots;
+      long rs_len;
       unsigned* queue;
       int* done;                 // per-search completion flags (host-visible memory)
       DimOpt o;
@@ -809,6 +831,42 @@ namespace flyg {
       double maxd = 0, maxp = 0;
       int refreshes = 0;
 
+      // Saved state of a region exit (resume_len layout): x, ax, the translation core (s, y, x_prev,
+      // g_prev, q: 2m + 3 consecutive workspace vectors) and its rho values.  Every thread takes part.
+      double* const tblk = vecp(t0);
+      long const nblk = static_cast<long>(2 * o.n + 3) * n3;
+      auto state_elem = [&](long i) -> double* {
+        if (i < n3) return x + i;
+        if (i < 2L * n3) return ax + (i - n3);
+        if (i < 2L * n3 + nblk) return tblk + (i - 2L * n3);
+        return tc.rho + (i - 2L * n3 - nblk);
+      };
+      long const rlen = resume_len(n3, o.n);
+      if (h.resume) {
+        double const* src = A.rs_in + h.rs_in_off;
+        for (long i = tm.tid; i < rlen; i += T) *state_elem(i) = src[i];
+        __syncthreads();
+        tc.k = h.r_k;
+        grads = h.r_grads;
+        rot_iters = h.r_rot_iters;
+        refreshes = h.r_refreshes;
+        maxd = h.r_maxd;
+        maxp = h.r_maxp;
+      }
+      int saved = 0;
+      // A region exit inside the step loop saves the state to a free slot of rs_out, if there is one.
+      auto save_state = [&]() {
+        if (tm.tid == 0) bc[6] = static_cast<double>(atomicAdd(A.rs_count, 1u));
+        __syncthreads();
+        int const slot = static_cast<int>(bc[6]);
+        __syncthreads();
+        if (slot >= A.rs_slots || rlen > A.rs_len) return;
+        double* dst = A.rs_out + static_cast<long>(slot) * A.rs_len;
+        for (long i = tm.tid; i < rlen; i += T) dst[i] = *state_elem(i);
+        saved = slot + 1;
+        __syncthreads();
+      };
+
       // Rows built at the base positions stay complete while every pair that is missing from them stays
       // beyond r_c: a mobile-frozen pair needs d_i < skin_f and a mobile-mobile pair d_i + d_j < skin_m,
       // i.e. d1 < skin_f and d1 + d2 < skin_m for the two largest mobile displacements d1 >= d2 (plus
@@ -986,59 +1044,65 @@ namespace flyg {
         return true;
       };
 
-      // Dimer::find_sp (dimer.cpp:28-171).
+      // Dimer::find_sp (dimer.cpp:28-171).  A resumed search enters the loop at its saved step, right
+      // after that step's translation (the displacement check and the effective gradient come next).
       double mag_g = 0;
-      if (!displacement_ok()) {
+      double trust = h.resume ? h.r_trust : o.min_trust;
+      int convex_count = h.resume ? h.r_convex : 0;
+      bool resumed = h.resume != 0;
+      if (!resumed && !displacement_ok()) {
         ret = EXIT_REGION;
         steps = 0;
-      } else if (!eff_gradient(mag_g, nullptr)) {
+      } else if (!resumed && !eff_gradient(mag_g, nullptr)) {
         ret = EXIT_OOB;
         steps = 0;
       } else {
-        double trust = o.min_trust;
-        int convex_count = 0;
-        for (int i = 0; i < o.max_steps; ++i) {
-          if (mag_g < o.f2norm * o.f2norm) {
-            ret = 0;
-            steps = i;
-            break;
-          } else if (convex_count >= o.convex_max) {
-            ret = 1;
-            steps = i;
-            break;
-          }
-          double* q = nullptr;
-          if (w0) {
-            // History test every hist_check_freq steps (dimer.cpp:102-131): the angle between x - min and
-            // sp_k - min below theta_tol for a known saddle k ends the search as a collision.  Frozen atoms
-            // sit at the minimum, so x - min is zero there and the dot product runs over the mobile atoms.
-            bool hit = false;
-            if (o.use_history && h.n_hist > 0 && i % o.hist_check_freq == 0) {
-              wv.lin(dg, n3, x, 1.0, xb, -1.0);
-              double const on = sqrt(wv.dot(dg, dg, n3));
-              for (int k = 0; k < h.n_hist && !hit; ++k) {
-                double const dt = wv.dot(dg, A.hist + h.hist_off + static_cast<long>(k) * n3, n3);
-                hit = dt / (A.hnorm[h.hn_off + k] * on) > h.cos_tol;
+        for (int i = resumed ? h.r_step : 0; i < o.max_steps; ++i) {
+          double* q = resumed ? tc.q : nullptr;
+          if (!resumed) {
+            if (mag_g < o.f2norm * o.f2norm) {
+              ret = 0;
+              steps = i;
+              break;
+            } else if (convex_count >= o.convex_max) {
+              ret = 1;
+              steps = i;
+              break;
+            }
+            if (w0) {
+              // History test every hist_check_freq steps (dimer.cpp:102-131): the angle between x - min and
+              // sp_k - min below theta_tol for a known saddle k ends the search as a collision.  Frozen atoms
+              // sit at the minimum, so x - min is zero there and the dot product runs over the mobile atoms.
+              bool hit = false;
+              if (o.use_history && h.n_hist > 0 && i % o.hist_check_freq == 0) {
+                wv.lin(dg, n3, x, 1.0, xb, -1.0);
+                double const on = sqrt(wv.dot(dg, dg, n3));
+                for (int k = 0; k < h.n_hist && !hit; ++k) {
+                  double const dt = wv.dot(dg, A.hist + h.hist_off + static_cast<long>(k) * n3, n3);
+                  hit = dt / (A.hnorm[h.hn_off + k] * on) > h.cos_tol;
+                }
               }
+              if (!hit) {
+                q = tc.newton_step(wv, x, eff);
+                double nrm = sqrt(wv.dot(q, q, n3));
+                double scale = fmin(1.0, trust / nrm);
+                wv.lin(q, n3, q, scale);
+                wv.lin(x, n3, x, 1.0, q, -1.0);
+              }
+              if (wv.lane == 0) bc[5] = hit ? 1.0 : 0.0;
             }
-            if (!hit) {
-              q = tc.newton_step(wv, x, eff);
-              double nrm = sqrt(wv.dot(q, q, n3));
-              double scale = fmin(1.0, trust / nrm);
-              wv.lin(q, n3, q, scale);
-              wv.lin(x, n3, x, 1.0, q, -1.0);
+            __syncthreads();
+            if (bc[5] != 0.0) {
+              ret = 3;
+              steps = i;
+              break;
             }
-            if (wv.lane == 0) bc[5] = hit ? 1.0 : 0.0;
-          }
-          __syncthreads();
-          if (bc[5] != 0.0) {
-            ret = 3;
-            steps = i;
-            break;
-          }
+          }   // !resumed
+          resumed = false;
           if (!displacement_ok()) {
             ret = EXIT_REGION;
             steps = i;
+            save_state();
             break;
           }
           if (!eff_gradient(mag_g, q)) {
@@ -1265,6 +1329,10 @@ namespace flyg {
         r.max_pair = maxp;
         r.ticks = static_cast<long>(wall_clock64()) - t_begin;
         r.grad_ticks = gticks;
+        r.saved = saved;
+        r.lbfgs_k = tc.k;
+        r.convex = convex_count;
+        r.trust = trust;
         A.out[sid] = r;
       }
       // Results (host-visible memory) first, then the flag: the host may read a search as soon as its

@@ -76,7 +76,7 @@ class SearchStats(C.Structure):
     _fields_ = [("exit_code", C.c_int), ("steps", C.c_int), ("grads", C.c_int64), ("rebuilds", C.c_int),
                 ("curv", C.c_double), ("energy", C.c_double), ("max_disp", C.c_double), ("launches", C.c_int),
                 ("rot_iters", C.c_int), ("device_ms", C.c_double), ("device_grad_ms", C.c_double),
-                ("retries", C.c_int)]
+                ("retries", C.c_int), ("resumes", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
