@@ -689,9 +689,9 @@ namespace fly {
 
     // Master (find.hpp:260-306, find.cpp:178-194): the saddle-search core of Master::find_mechs, with the
     // searches of a site run as one device batch.  find_mechs itself (catalogue geometries, symmetry
-    // expansion, mechanism deduplication) needs the reference's env layer and is not part of this facade;
-    // find_batch below is its per-site step (find.cpp:684-755) with the mechanism checks of saddle_2_mech
-    // (find.cpp:757-837) and the prefactor of the mass-weighted Hessians (find.cpp:838-871, 287).
+    // expansion) needs the reference's env layer and is not part of this facade; find_batch below is its
+    // per-site step (find.cpp:684-755) with the mechanism checks of saddle_2_mech (find.cpp:757-837), the
+    // prefactor of the mass-weighted Hessians (find.cpp:838-871, 287) and is_new_mech's de-duplication.
     class Master {
     public:
       struct Options {
