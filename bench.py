@@ -318,10 +318,12 @@ def cpu_search_leg(S, build, sample=16, repeats=5):
 def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
     """Serving: Poisson arrivals at 200/s of requests from a simulated kMC loop, each on its own system
     (bcc Fe, N log-uniform in [2000, 250000], 0.5-2 % vacancies + interstitials, relaxed), 8-64 searches
-    each (radius U(4, 9) A, Master::perturb starts in the library), served with continuous batching;
-    latency = arrival to last search done.  Then the same systems at 0.9x the throughput sustained under
+    each centred on the system's catalogue misses (radius U(4, 9) A, Master::perturb starts in the
+    library), served with continuous batching; latency = arrival to last search done.  Then the same systems at 0.9x the throughput sustained under
     that overload (the highest rate the server keeps up with), for latency below saturation."""
     from openfly_amd import serve
+    from openfly_amd.catalogue import Catalogue
+    import torch
     pot = fa.EAM(ctx, WEN)
     log(f"serving leg: building and relaxing {n_req} kMC systems (one per request)")
     t0 = time.perf_counter()
@@ -330,23 +332,51 @@ def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
     srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3))
     log("serving leg: warm-up")
     serve.serve(srv, serve.make_requests(pool[:2], 4, rate=rate, seed=40, searches=(8, 16)))   # + shape autotune
-    reqs = serve.make_requests(pool, n_req, rate=rate, seed=42, one_system_each=True)
+    # Request source: each system is one kMC state whose last hop was at one of its defects; the catalogue
+    # shared by the run looks at the environments within 2 r_env of that site (the ones the hop changed)
+    # and the request's searches go to the new ones (cache misses), computed once per system before the
+    # replays.
+    log("serving leg: catalogue of local environments (cache misses per system)")
+    cat = Catalogue()
+    dev = torch.device("cuda", 0)
+    found = {}
+    rng_hop = np.random.default_rng(44)
+    t0 = time.perf_counter()
+    for q, sy in enumerate(pool):
+        sites = [np.asarray(sy.vacancies)] if getattr(sy, "vacancies", None) is not None and len(sy.vacancies) else []
+        if getattr(sy, "interstitial_ids", None) is not None and len(sy.interstitial_ids):
+            sites.append(sy.pos[sy.interstitial_ids])
+        sites = np.concatenate(sites)
+        hop = sites[rng_hop.integers(len(sites))]
+        near = serve.Near(sy).within(hop, 2.0 * cat.r_env)
+        found[id(sy)] = cat.rebuild(sy.pos, sy.types, sy.frozen, sy.basis, device=dev, atoms=near)
+        if q % 16 == 15:
+            log(f"serving leg: catalogue {q + 1}/{len(pool)} systems, {cat.size} environments")
+    t_cat = time.perf_counter() - t0
+    misses = [len(found[id(sy)]) for sy in pool]
+    reqs = serve.make_requests(pool, n_req, rate=rate, seed=42, one_system_each=True,
+                               misses=lambda sy: found[id(sy)])
     log(f"serving leg: replay of {n_req} requests at {rate:g}/s")
     out = serve.serve(srv, reqs, report=log)
     sustained = out["requests"] / out["wall_s"]
     rate2 = 0.9 * sustained
     log(f"serving leg: replay at {rate2:.1f} requests/s (0.9x the sustained {sustained:.1f}/s)")
-    reqs2 = serve.make_requests(pool, n_req, rate=rate2, seed=43, one_system_each=True)
+    reqs2 = serve.make_requests(pool, n_req, rate=rate2, seed=43, one_system_each=True,
+                                misses=lambda sy: found[id(sy)])
     light = serve.serve(srv, reqs2, report=log)
     keys = ("requests", "searches", "converged", "wall_s", "searches_per_s", "latency_p50_s", "latency_p99_s",
             "latency_max_s", "mean_queue_wait_s", "offered_requests_per_s", "mean_searches_per_request")
     out["sustained_requests_per_s"] = sustained
     out["at_sustainable_rate"] = {k: light[k] for k in keys}
     st = srv.status()
-    out.update(workload=f"{n_req} requests, Poisson {rate:g}/s, each on its own relaxed kMC system (bcc Fe, N "
-                        "log-uniform [2000, 250000], 0.5-2% vacancies + interstitials), 8-64 searches per request, "
-                        "radius U(4,9) A, Master::perturb starts, |g| < 1e-3 eV/A, FP32 pair / FP64 accumulation, "
-                        "wen.eam.fs",
+    out.update(workload=f"{n_req} kMC systems (bcc Fe, N log-uniform [2000, 250000], 0.5-2% vacancies + "
+                        f"interstitials, relaxed), Poisson {rate:g}/s; each system sends one request for 8-64 searches "
+                        "centred on its local environments that a shared catalogue had not seen (cache misses; "
+                        "systems with none send none), radius U(4,9) A, Master::perturb starts, |g| < 1e-3 eV/A, "
+                        "FP32 pair / FP64 accumulation, wen.eam.fs",
+               catalogue={"environments": cat.size, "seconds": t_cat, "misses_per_system_min": int(min(misses)),
+                          "misses_per_system_max": int(max(misses)),
+                          "misses_per_system_mean": float(np.mean(misses))},
                pool_relax_seconds=t_pool, launches=st["launches"], kernel_ms=st["kernel_ms"],
                block_threads=st["block_threads"], lanes_per_atom=st["lanes_per_atom"],
                system_atoms_min=int(min(p.n for p in pool)), system_atoms_max=int(max(p.n for p in pool)))

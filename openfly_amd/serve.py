@@ -97,18 +97,32 @@ def make_pool(count, seed=4, n_min=2000, n_max=250000, defects=(0.005, 0.02), re
     return pool
 
 
-def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0), one_system_each=False):
+def make_requests(pool, count, rate=200.0, seed=5, searches=(8, 64), radii=(4.0, 9.0), one_system_each=False,
+                  misses=None):
     """Poisson arrivals at `rate`/s, each from a uniformly chosen system of `pool`, asking for a uniform
     number of searches in `searches` around defect-adjacent atoms with radii uniform in `radii`; each search
     starts from Master::perturb (in the library) with its own generator state.  one_system_each: request q
-    comes from pool[q] (a kMC loop whose system changed since its last request) instead of a random one."""
+    comes from pool[q] (a kMC loop whose system changed since its last request) instead of a random one.
+
+    misses(system) -> atom indices: the request source of a kMC loop (Master::find_mechs over the
+    catalogue's new environments, find.cpp:211-312): the request's centres are the atoms whose local
+    environment the catalogue has not seen (a shared catalogue.Catalogue across requests), as many searches
+    as the request asks for spread over them in turn; a system with no new environment sends no request."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     reqs = []
     for q in range(count):
         s = pool[q % len(pool)] if one_system_each else pool[int(rng.integers(len(pool)))]
         k = int(rng.integers(searches[0], searches[1] + 1))
-        centres = sample_centres(s, k, rng)
+        if misses is not None:
+            new = np.asarray(misses(s), dtype=np.int64)
+            if len(new) == 0:
+                continue
+            if len(new) > k:
+                new = np.sort(rng.choice(new, k, replace=False))
+            centres = np.resize(new, k)   # several searches per environment when there are fewer than k
+        else:
+            centres = sample_centres(s, k, rng)
         rad = rng.uniform(radii[0], radii[1], k)
         prng = rng.integers(1, 2 ** 63, size=(k, 4), dtype=np.uint64)
         reqs.append(Request(float(t[q]), s, centres, rad, prng))
