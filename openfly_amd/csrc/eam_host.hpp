@@ -61,6 +61,7 @@ struct flyg_eam_s {
   flyg::DBuf<double2> dfp;      // HVP: (F', dF') per atom
   flyg::DBuf<double> eatom;     // per-atom energies
   flyg::DBuf<uint32_t> types;   // staged host inputs
+  flyg::DBuf<uint32_t> gtype;   // types in the list's slot order (gradients with several species)
   flyg::DBuf<uint8_t> frozen;
   flyg::DBuf<double> vec_in, vec_out;
   std::vector<uint32_t> last_types;
@@ -100,6 +101,13 @@ struct flyg_list_s {
   flyg::DBuf<unsigned> sid;
   flyg::DBuf<unsigned> nbr;      // n x K packed entries
   flyg::DBuf<int> ncnt;          // entries per atom
+  // The gradient passes' layout, made at each rebuild: the real atoms in cell order (slot k holds atom
+  // order[k], rank[i] is atom i's slot), their positions by slot (kept current by update), and the rows
+  // by slot with neighbour indices given as slots (the same entries in the same order as nbr).
+  flyg::DBuf<int> order, rank, gflag;
+  flyg::DBuf<double4> gpos;
+  flyg::DBuf<unsigned> grow;
+  flyg::DBuf<int> gcnt;
   flyg::DBuf<int> scan_tmp;
   flyg::DBuf<double> stage;      // host staging of xyz vectors
   flyg::DBuf<unsigned> lstate;   // device build counters
@@ -129,6 +137,9 @@ namespace flyg {
     uint32_t const* types;
     uint8_t const* frozen;
     int t0;   // the one species present (-1: several or unknown); read by the single-species kernels
+    // Null: rows, positions and types are by atom.  Otherwise they are by slot of the list's cell-order
+    // layout (types gathered into slot order) and order[slot] is the atom, for frozen flags and gradients.
+    int const* order = nullptr;
   };
   void eam_setup_tables(flyg_eam_s& e);
   double eam_energy(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen);

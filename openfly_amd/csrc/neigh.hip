@@ -99,7 +99,8 @@ namespace flyg {
       return (dx * dx + dy * dy) + dz * dz;
     }
 
-    __global__ void k_update(double4* __restrict__ pos, double const* __restrict__ d, long n) {
+    __global__ void k_update(double4* __restrict__ pos, double const* __restrict__ d, long n, int const* __restrict__ rank,
+                             double4* __restrict__ gpos) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (i >= n) return;
       double4 p = pos[i];
@@ -107,6 +108,44 @@ namespace flyg {
       p.y -= d[3 * i + 1];
       p.z -= d[3 * i + 2];
       pos[i] = p;
+      gpos[rank[i]] = p;
+    }
+
+    // The gradient layout (cell order of the real atoms): flag the sorted entries that are real atoms,
+    // scan, then place each real at its slot and copy its row with neighbour indices given as slots.
+    __global__ void k_real_flags(unsigned const* __restrict__ sid, long live, int* __restrict__ flag) {
+      long s = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (s < live) flag[s] = (sid[s] >> NB_SHIFT_BITS) == 13u ? 1 : 0;
+    }
+
+    __global__ void k_real_order(unsigned const* __restrict__ sid, int const* __restrict__ before, long live,
+                                 double4 const* __restrict__ pos, int* __restrict__ order, int* __restrict__ rank,
+                                 double4* __restrict__ gpos) {
+      long s = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (s >= live) return;
+      unsigned const id = sid[s];
+      if ((id >> NB_SHIFT_BITS) != 13u) return;
+      int const i = static_cast<int>(id & NB_IDX_MASK);
+      int const k = before[s];
+      order[k] = i;
+      rank[i] = k;
+      gpos[k] = pos[i];
+    }
+
+    __global__ void k_slot_rows(unsigned const* __restrict__ nbr, int const* __restrict__ ncnt,
+                                int const* __restrict__ order, int const* __restrict__ rank, long n, int K,
+                                unsigned* __restrict__ grow, int* __restrict__ gcnt) {
+      long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (t >= n * static_cast<long>(K)) return;
+      long const k = t / K;
+      int const e = static_cast<int>(t % K);
+      int const i = order[k];
+      int const c = ncnt[i];
+      if (e == 0) gcnt[k] = c;
+      if (e < c) {
+        unsigned const en = nbr[static_cast<long>(i) * K + e];
+        grow[t] = (en & ~NB_IDX_MASK) | static_cast<unsigned>(rank[en & NB_IDX_MASK]);
+      }
     }
 
     __global__ void k_positions(double4 const* __restrict__ pos, long n, double* __restrict__ out) {
@@ -483,6 +522,28 @@ namespace flyg {
     }
   }
 
+  namespace {
+    void build_gradient_layout(flyg_list_s& nl) {
+      flyg_ctx_s& ctx = *nl.ctx;
+      hipStream_t st = ctx.stream;
+      long const n = nl.n, live = n + nl.nghost;
+      nl.order.reserve(n);
+      nl.rank.reserve(n);
+      nl.gpos.reserve(n);
+      nl.gcnt.reserve(n);
+      nl.grow.reserve(static_cast<std::size_t>(n) * nl.K);
+      nl.gflag.reserve(live + 1);
+      k_real_flags<<<nblk(live, TPB), TPB, 0, st>>>(nl.sid.get(), live, nl.gflag.get());
+      exclusive_scan(ctx, nl.gflag.get(), static_cast<int>(live), nl.scan_tmp);
+      k_real_order<<<nblk(live, TPB), TPB, 0, st>>>(nl.sid.get(), nl.gflag.get(), live, nl.pos.get(), nl.order.get(),
+                                                    nl.rank.get(), nl.gpos.get());
+      k_slot_rows<<<nblk(n * static_cast<long>(nl.K), TPB), TPB, 0, st>>>(nl.nbr.get(), nl.ncnt.get(), nl.order.get(),
+                                                                          nl.rank.get(), n, nl.K, nl.grow.get(),
+                                                                          nl.gcnt.get());
+      HIPCHK(hipGetLastError());
+    }
+  }  // namespace
+
   // One rebuild costs one host synchronisation: capacities (ghost slots, row length) come from the
   // previous build; the device records what it needed and the build is redone only on overflow.
   void list_rebuild(flyg_list_s& nl, double const* d_xyz, long n) {
@@ -544,6 +605,7 @@ namespace flyg {
         nl.K = (static_cast<int>(maxrow) + 15) / 16 * 16 + 16;
         continue;
       }
+      build_gradient_layout(nl);
       nl.built = true;
       return;
     }
@@ -553,7 +615,7 @@ namespace flyg {
   void list_update(flyg_list_s& nl, double const* d_delta, long n) {
     require(nl.built, "List::update before rebuild");
     require(n == nl.n, "Input to update has the wrong number of atoms");
-    k_update<<<nblk(n, TPB), TPB, 0, nl.ctx->stream>>>(nl.pos.get(), d_delta, n);
+    k_update<<<nblk(n, TPB), TPB, 0, nl.ctx->stream>>>(nl.pos.get(), d_delta, n, nl.rank.get(), nl.gpos.get());
     HIPCHK(hipGetLastError());
   }
 
