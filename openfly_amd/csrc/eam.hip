@@ -424,8 +424,8 @@ namespace flyg {
     // The gradient passes run in the list's cell-order layout: a workgroup's atoms are neighbours in
     // space and so are the rows they gather, which raises the L1/L2 hit rate of the gathers.  Rows hold
     // the same entries in the same order, so the sums (and the bits) are those of the atom layout.  With
-    // several species the types are first gathered into slot order (a copy of n words, outside the
-    // timed region of the gradient events).
+    // several species the types are gathered into slot order: once per list build for staged host types
+    // (the library sees every change of them), on every call for device types.
     PairArgs slot_args(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
       PairArgs A = pair_args(E, nl, types, frozen);
       A.pos = nl.gpos.get();
@@ -433,10 +433,18 @@ namespace flyg {
       A.cnt = nl.gcnt.get();
       A.order = nl.order.get();
       if (A.t0 < 0) {
-        E.gtype.reserve(nl.n);
-        k_types_by_slot<<<static_cast<unsigned>((nl.n + 255) / 256), 256, 0, E.ctx->stream>>>(types, nl.order.get(),
-                                                                                              nl.n, E.gtype.get());
-        HIPCHK(hipGetLastError());
+        bool const staged = types == E.types.get();
+        if (!(staged && E.gtype_build == nl.build_id && E.gtype_stamp == E.types_stamp)) {
+          E.gtype_build = -1;   // the buffer may be reallocated: its contents are no longer known
+          E.gtype.reserve(nl.n);
+          k_types_by_slot<<<static_cast<unsigned>((nl.n + 255) / 256), 256, 0, E.ctx->stream>>>(
+              types, nl.order.get(), nl.n, E.gtype.get());
+          HIPCHK(hipGetLastError());
+          if (staged) {
+            E.gtype_build = nl.build_id;
+            E.gtype_stamp = E.types_stamp;
+          }
+        }
         A.types = E.gtype.get();
       }
       return A;
@@ -609,9 +617,7 @@ namespace flyg {
     require(E.tab.rc <= nl.rc, "r_cut is bigger than the internal r_cut of the list");
     E.posf.reserve(nl.n);
     Shifts S = shifts_of(nl.box);
-    PairArgs A = slot_args(E, nl, types, frozen);
-    int L = tuned_lanes(E, S, A, d_grad, precision);
-    if (E.timing) {
+    if (E.timing) {   // the pair covers the species gather too (and, on a first call, the lane tuning)
       if (E.events_used == E.events.size()) {
         hipEvent_t a, b;
         HIPCHK(hipEventCreate(&a));
@@ -620,6 +626,8 @@ namespace flyg {
       }
       HIPCHK(hipEventRecord(E.events[E.events_used].first, E.ctx->stream));
     }
+    PairArgs A = slot_args(E, nl, types, frozen);
+    int L = tuned_lanes(E, S, A, d_grad, precision);
     gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
     if (E.timing) HIPCHK(hipEventRecord(E.events[E.events_used++].second, E.ctx->stream));
     HIPCHK(hipGetLastError());
@@ -756,6 +764,7 @@ namespace flyg {
     E.frozen.reserve(n);
     if (!same_t) {
       HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, E.ctx->stream));
+      ++E.types_stamp;
       E.staged_single = single;
       E.last_types.assign(types, types + n);
     }

@@ -62,6 +62,8 @@ struct flyg_eam_s {
   flyg::DBuf<double> eatom;     // per-atom energies
   flyg::DBuf<uint32_t> types;   // staged host inputs
   flyg::DBuf<uint32_t> gtype;   // types in the list's slot order (gradients with several species)
+  long types_stamp = 0;         // bumped whenever the staged host types change
+  long gtype_build = -1, gtype_stamp = -1;   // gtype holds the staged types in the layout of this build
   flyg::DBuf<uint8_t> frozen;
   flyg::DBuf<double> vec_in, vec_out;
   std::vector<uint32_t> last_types;
@@ -90,6 +92,7 @@ struct flyg_list_s {
   long nghost = 0;
   int K = 0;       // row capacity of the neighbour table
   bool built = false;
+  long build_id = 0;   // unique per successful rebuild (process-wide), names the slot layout in use
   flyg::DBuf<double4> pos;       // live positions (x, y, z, 0)
   flyg::DBuf<double4> ent;       // reals + ghost images at build time (chained positions)
   flyg::DBuf<unsigned> eid;      // packed (shift, real) of ent

@@ -12,6 +12,7 @@
 // Ghosts are not kept after the build: an entry stores (shift code, real index) and the pair kernels
 // add the shift vector to the live real position, so update() touches only the N real atoms.
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 
 #include "eam_host.hpp"
@@ -606,6 +607,8 @@ namespace flyg {
         continue;
       }
       build_gradient_layout(nl);
+      static std::atomic<long> builds{0};
+      nl.build_id = ++builds;
       nl.built = true;
       return;
     }

@@ -327,3 +327,51 @@ def test_device_arrays_checked(ctx, wen_gpu):
         wen_gpu.energy(tb, f, nl)
     E = wen_gpu.energy(t, f, nl)
     assert abs(E - H.KAT_E0) <= 1e-10 * abs(H.KAT_E0)
+
+
+def test_slot_layout_follows_types_positions_and_rebuilds(ctx, wen_gpu, wen_orc):
+    # The gradient passes run in the list's cell-order layout with the types gathered into it (cached per
+    # list build for staged host types).  Every change the cache must see: new host types on the same
+    # build, an update (positions by slot), a rebuild, device types in between, and two lists in turn.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(41)
+    pos, basis = H.kat_lattice()
+    pos = pos + rng.normal(0, 0.05, pos.shape)
+    n = len(pos)
+    f = np.zeros(n, np.uint8)
+    f[::7] = 1
+    t1 = (rng.random(n) < 0.2).astype(np.uint32)
+    t2 = (rng.random(n) < 0.3).astype(np.uint32)
+
+    def oracle(p, t):
+        onl = po.List(po.Box(basis), wen_orc.r_cut)
+        onl.rebuild(p, 4)
+        return po.gradient(wen_orc, onl, t, f, 4)
+
+    def check(g, p, t):
+        go = oracle(p, t)
+        assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+
+    nl = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut() + 0.5)
+    nl.rebuild(pos)
+    check(wen_gpu.gradient(t1, f, nl), pos, t1)
+    check(wen_gpu.gradient(t2, f, nl), pos, t2)             # new staged types, same build
+    d = rng.normal(0, 0.02, pos.shape)                      # |d| well inside half the 0.5 A skin
+    nl.update(d)                                            # positions move by -d (List::update)
+    moved = pos - d
+    check(wen_gpu.gradient(t2, f, nl), moved, t2)
+    other = moved + rng.normal(0, 0.1, pos.shape)
+    nl.rebuild(other)                                       # a new layout
+    check(wen_gpu.gradient(t2, f, nl), other, t2)
+    dev = torch.device(f"cuda:{ctx.device}")
+    td = torch.from_numpy(t1).to(dev)
+    fd = torch.from_numpy(f).to(dev)
+    gd = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    wen_gpu.gradient(td, fd, nl, gd)                         # device types overwrite the slot-order copy
+    torch.cuda.synchronize()
+    check(gd.cpu().numpy(), other, t1)
+    check(wen_gpu.gradient(t2, f, nl), other, t2)            # staged types again: gathered again
+    nl_b = fa.List(ctx, fa.Box(basis), wen_gpu.r_cut() + 0.5)
+    nl_b.rebuild(pos)
+    for lst, p in ((nl_b, pos), (nl, other), (nl_b, pos)):
+        check(wen_gpu.gradient(t2, f, lst), p, t2)
