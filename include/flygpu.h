@@ -384,6 +384,7 @@ typedef struct {
   int64_t queued, in_flight, systems, device_bytes, budget_bytes, launches, searches_done;
   double kernel_ms;     /* summed kernel time of finished launches */
   int busy_streams, block_threads, lanes_per_atom;
+  double host_ms;       /* host time spent building search regions and packing launches */
 } flyg_server_state;
 
 void flyg_server_default_options(flyg_server_options* opt);

@@ -41,7 +41,9 @@ namespace flyg {
 
   namespace {
 
-    // Periodic cell grid over the base positions (orthorhombic box), cells of at least `rmin`.
+    // Periodic cell grid over the base positions (orthorhombic box), cells of at least `rmin` / 2: a
+    // query of radius ~rmin visits the cells its sphere touches, about a third of the candidates that
+    // cells of rmin (a 5 x 5 x 5 block) would give.
     struct HostGrid {
       double L[3];
       int nc[3];
@@ -53,7 +55,7 @@ namespace flyg {
         pos = p;
         for (int k = 0; k < 3; ++k) {
           L[k] = ext[k];
-          nc[k] = std::max(1, static_cast<int>(std::floor(L[k] / rmin)));
+          nc[k] = std::max(1, static_cast<int>(std::floor(L[k] / (0.5 * rmin))));
           cell[k] = L[k] / nc[k];
         }
         long nct = static_cast<long>(nc[0]) * nc[1] * nc[2];
@@ -83,9 +85,22 @@ namespace flyg {
           reach[k] = static_cast<int>(std::ceil(rad / cell[k]));
         }
         double const r2max = rad * rad;
-        for (int oz = -reach[2]; oz <= reach[2]; ++oz)
-          for (int oy = -reach[1]; oy <= reach[1]; ++oy)
+        // Distance from p to the cell at unwrapped index cc along axis k (0 inside it), less a margin that
+        // covers the rounding of the binning, so no cell holding an atom within rad is skipped.
+        auto gap = [&](int k, int cc) {
+          double const lo = cc * cell[k] - p[k], hi = (cc + 1) * cell[k] - p[k];
+          double const g = lo > 0 ? lo : (hi < 0 ? -hi : 0.0);
+          return std::max(0.0, g - 1e-9 * L[k]);
+        };
+        for (int oz = -reach[2]; oz <= reach[2]; ++oz) {
+          double const gz = gap(2, ci[2] + oz);
+          if (gz * gz > r2max) continue;
+          for (int oy = -reach[1]; oy <= reach[1]; ++oy) {
+            double const gy = gap(1, ci[1] + oy);
+            if (gz * gz + gy * gy > r2max) continue;
             for (int ox = -reach[0]; ox <= reach[0]; ++ox) {
+              double const gx = gap(0, ci[0] + ox);
+              if (gz * gz + gy * gy + gx * gx > r2max) continue;
               int o[3] = {ox, oy, oz}, cw[3], sh[3];
               for (int k = 0; k < 3; ++k) {
                 int cc = ci[k] + o[k];
@@ -103,6 +118,8 @@ namespace flyg {
                 if (r2 <= r2max) f(j, shift_code(sh[0], sh[1], sh[2]), r2);
               }
             }
+          }
+        }
       }
     };
 
@@ -135,8 +152,17 @@ namespace flyg {
     std::size_t sys_bytes_estimate(long n) { return static_cast<std::size_t>(n) * (32 + 8 + 1) + 27 * 24 + 4096; }
     std::size_t sys_transient_estimate(long n) { return static_cast<std::size_t>(n) * 700; }
 
+    // Device scratch of an admission (the full list that gives the base densities), kept by a server
+    // across admissions so that admitting a system frees nothing while other searches run.
+    struct AdmitScratch {
+      std::unique_ptr<flyg_list_s> nl;
+      DBuf<double> pos;
+      DBuf<uint32_t> types;
+    };
+
     std::unique_ptr<SysState> make_system(flyg_ctx_s* ctx, flyg_eam_s* eam, double const basis[9], int const periodic[3],
-                                          double const* min_pos, uint32_t const* types, long n, double list_skin) {
+                                          double const* min_pos, uint32_t const* types, long n, double list_skin,
+                                          AdmitScratch* scratch = nullptr) {
       auto S = std::make_unique<SysState>();
       S->n = n;
       std::memcpy(S->basis, basis, sizeof(S->basis));
@@ -184,15 +210,22 @@ namespace flyg {
       HIPCHK(hipMemcpyAsync(S->d_type.get(), t8.data(), n, hipMemcpyHostToDevice, st));
       HIPCHK(hipMemcpyAsync(S->d_S.get(), sh, sizeof(sh), hipMemcpyHostToDevice, st));
       {   // global FP64 base densities from a full list at r_c
-        auto nl = make_list(ctx, basis, periodic, eam->tab.rc);
-        DBuf<double> dpos;
-        dpos.reserve(3 * n);
-        DBuf<uint32_t> dt;
-        dt.reserve(n);
-        HIPCHK(hipMemcpyAsync(dpos.get(), S->base.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dt.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        list_rebuild(*nl, dpos.get(), n);
-        eam_density(*eam, *nl, dt.get(), S->d_rho.get());
+        AdmitScratch own;
+        AdmitScratch& A = scratch ? *scratch : own;
+        if (!A.nl) {
+          A.nl = make_list(ctx, basis, periodic, eam->tab.rc);
+        } else {   // same buffers, this system's box
+          A.nl->box = make_box(basis, periodic);
+          A.nl->grid = make_grid(A.nl->box, eam->tab.rc);
+          A.nl->rc = eam->tab.rc;
+          A.nl->built = false;
+        }
+        A.pos.reserve(3 * n);
+        A.types.reserve(n);
+        HIPCHK(hipMemcpyAsync(A.pos.get(), S->base.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(A.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        list_rebuild(*A.nl, A.pos.get(), n);
+        eam_density(*eam, *A.nl, A.types.get(), S->d_rho.get());
         unsigned fl = ctx->take_flags();   // synchronises the stream before the scratch goes away
         if (fl & ERR_SPLINE_OOB) throw Error(FLYG_SPLINE_OOB, "x is outside tabulated region of a spline");
         if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
@@ -371,47 +404,62 @@ namespace flyg {
     }
 
     // Region of a search: mobile atoms, then S1 (frozen atoms within rF = r_c + skin_f of a mobile atom),
-    // with rows of packed (shift, local index) entries: a mobile row lists mobile atoms within
-    // rM = r_c + skin_m and S1 atoms within rF; an S1 row lists the mobile atoms within rF.
+    // with rows of packed (shift, local index) entries in ascending order: a mobile row lists mobile atoms
+    // within rM = r_c + skin_m and S1 atoms within rF; an S1 row lists the mobile atoms within rF.  One
+    // grid query per mobile atom (rM >= rF) finds its row and the S1 atoms; the S1 rows are the mobile
+    // rows' S1 entries transposed (the image of i seen from j has the opposite shift), so each pair is
+    // tested once and the two rows always agree.
     void build_region(SysState const& S, Pending const& P, double rc, std::vector<int>& local_of, Region& g) {
-      double const rF = rc + P.skin_f, rM = rc + P.skin_m;
+      double const rF = rc + P.skin_f, rM = rc + P.skin_m, rq = std::max(rF, rM);
       int const nm = static_cast<int>(P.mobile.size());
       g.nm = nm;
       g.gidx = P.mobile;
       for (int i = 0; i < nm; ++i) local_of[P.mobile[i]] = i;
-      std::vector<int> s1;
-      for (int i = 0; i < nm; ++i)
-        S.grid.query(&S.base[3 * P.mobile[i]], rF, [&](int j, int, double r2) {
-          if (r2 < rF * rF && local_of[j] == -1) {
-            local_of[j] = -2;   // collected into S1
-            s1.push_back(j);
+      std::vector<int> s1, hj;
+      std::vector<unsigned char> hs;
+      std::vector<int> hoff(nm + 1, 0);
+      for (int i = 0; i < nm; ++i) {
+        int const gi = P.mobile[i];
+        S.grid.query(&S.base[3 * gi], rq, [&](int j, int sc, double r2) {
+          if (j == gi && sc == 13) return;
+          int const lj = local_of[j];
+          if (lj >= 0) {   // mobile
+            if (!(r2 < rM * rM)) return;
+          } else {
+            if (!(r2 < rF * rF)) return;   // frozen beyond rF of this mobile atom
+            if (lj == -1) {
+              local_of[j] = -2;   // collected into S1
+              s1.push_back(j);
+            }
           }
+          hj.push_back(j);
+          hs.push_back(static_cast<unsigned char>(sc));
         });
+        hoff[i + 1] = static_cast<int>(hj.size());
+      }
       std::sort(s1.begin(), s1.end());
       for (std::size_t k = 0; k < s1.size(); ++k) local_of[s1[k]] = nm + static_cast<int>(k);
       g.gidx.insert(g.gidx.end(), s1.begin(), s1.end());
       g.nr = static_cast<int>(g.gidx.size());
       g.rowlen.assign(g.nr, 0);
-      g.ent.clear();
-      std::vector<unsigned> row;
-      for (int r = 0; r < g.nr; ++r) {
-        int gi = g.gidx[r];
-        bool mob = r < nm;
-        row.clear();
-        S.grid.query(&S.base[3 * gi], mob ? rM : rF, [&](int j, int sc, double r2) {
-          if (j == gi && sc == 13) return;
-          int lj = local_of[j];
-          if (lj < 0) return;   // frozen beyond rF of every mobile atom
-          bool jm = lj < nm;
-          if (!mob && !jm) return;   // S1 rows hold mobile neighbours only
-          double rr = (mob && jm) ? rM : rF;
-          if (!(r2 < rr * rr)) return;
-          row.push_back((static_cast<unsigned>(sc) << NB_SHIFT_BITS) | static_cast<unsigned>(lj));
-        });
-        std::sort(row.begin(), row.end());
-        g.rowlen[r] = static_cast<int>(row.size());
-        g.ent.insert(g.ent.end(), row.begin(), row.end());
+      for (std::size_t h = 0; h < hj.size(); ++h) {
+        int const lj = local_of[hj[h]];
+        if (lj >= nm) ++g.rowlen[lj];
       }
+      for (int i = 0; i < nm; ++i) g.rowlen[i] = hoff[i + 1] - hoff[i];
+      std::vector<long> roff(g.nr + 1, 0);
+      for (int r = 0; r < g.nr; ++r) roff[r + 1] = roff[r] + g.rowlen[r];
+      g.ent.assign(roff[g.nr], 0u);
+      std::vector<long> fill(roff.begin() + nm, roff.end() - 1);
+      for (int i = 0; i < nm; ++i) {
+        for (int h = hoff[i]; h < hoff[i + 1]; ++h) {
+          unsigned const sc = hs[h];
+          int const lj = local_of[hj[h]];
+          g.ent[roff[i] + (h - hoff[i])] = (sc << NB_SHIFT_BITS) | static_cast<unsigned>(lj);
+          if (lj >= nm) g.ent[fill[lj - nm]++] = ((26u - sc) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+        }
+      }
+      for (int r = 0; r < g.nr; ++r) std::sort(g.ent.begin() + roff[r], g.ent.begin() + roff[r + 1]);
       for (int j : g.gidx) local_of[j] = -1;
     }
 
@@ -506,11 +554,11 @@ namespace flyg {
       Pinned(Pinned const&) = delete;
       Pinned& operator=(Pinned const&) = delete;
       ~Pinned() {
-        if (p) (void)hipHostFree(p);
+        if (p) (void)free_or_defer(p, cap, true);
       }
       void reserve(std::size_t n) {
         if (n <= cap) return;
-        if (p) HIPCHK(hipHostFree(p));
+        if (p) HIPCHK(free_or_defer(p, cap, true));
         p = nullptr;
         cap = 0;
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p), n, hipHostMallocDefault));
@@ -947,6 +995,10 @@ struct flyg_batch_s {
 };
 
 struct flyg_server_s {
+  // Buffers given up while searches run (grown launch buffers, removed systems, admission scratch):
+  // freed when no launch is in flight, or at once when admission needs the memory.  First member, so
+  // it is destroyed last.
+  flyg::Graveyard grave;
   flyg_ctx_s* ctx = nullptr;
   flyg_eam_s* eam = nullptr;
   flyg_dimer_options opt;
@@ -954,6 +1006,8 @@ struct flyg_server_s {
   flyg_server_options sopt;
   flyg_mech_options mopt{};
   std::map<int64_t, std::unique_ptr<flyg::SysState>> systems;
+  flyg::AdmitScratch admit;
+  long admit_n = 0;   // largest system the admission scratch has held (its size is estimated from it)
   int64_t next_sys = 0;
   std::deque<flyg::Pending> queue;
   std::vector<std::unique_ptr<flyg::Flight>> flights;
@@ -965,9 +1019,10 @@ struct flyg_server_s {
   int tuned_T = 0, tuned_L = 0;
   int64_t launches = 0, searches_done = 0;
   double kernel_ms = 0;
+  double host_ms = 0;   // region builds and launch packing
 
   std::size_t used() const {
-    std::size_t b = 0;
+    std::size_t b = grave.bytes + flyg::sys_transient_estimate(admit_n);
     for (auto const& kv : systems) b += kv.second->dev_bytes();
     for (auto const& f : flights) b += f->dev_bytes();
     return b;
@@ -984,6 +1039,9 @@ namespace flyg {
     long server_step(flyg_server_s& S) {
       long fin = 0;
       double const rc = S.eam->tab.rc;
+      bool any_busy = false;
+      for (auto const& fp : S.flights) any_busy = any_busy || fp->busy;
+      if (!any_busy || S.grave.bytes > S.budget / 4) S.grave.drain();
       for (auto& fp : S.flights) {
         Flight& F = *fp;
         if (!F.busy) continue;
@@ -1033,6 +1091,7 @@ namespace flyg {
           work.push_back(std::move(S.queue.front()));
           S.queue.pop_front();
         }
+        auto const h0 = std::chrono::steady_clock::now();
         ensure_regions(work, rc);
         // Memory gate: keep the prefix that fits beside everything resident (at least one search when
         // nothing else is in flight, so the queue always drains).
@@ -1046,8 +1105,12 @@ namespace flyg {
           S.queue.push_front(std::move(work.back()));
           work.pop_back();
         }
-        if (work.empty()) break;
+        if (work.empty()) {
+          S.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+          break;
+        }
         flight_launch(*S.eam, F, std::move(work), S.tuned_T, S.tuned_L, d, mix, slots);
+        S.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
         ++S.launches;
       }
       return fin;
@@ -1408,15 +1471,23 @@ int flyg_server_add_system(flyg_server s, const double basis[9], const int perio
                            const uint32_t* types, int64_t n, int64_t* sys_id) {
   return bguard(s->ctx, [&] {
     require(basis && periodic && min_pos && types && sys_id && n > 0, "null or empty argument");
-    std::size_t need = sys_bytes_estimate(n) + sys_transient_estimate(n);
-    if (s->used() + need > s->budget) {   // idle launch buffers are the first to go
+    FreeLater defer(s->grave);
+    auto need_for = [&] {
+      return sys_bytes_estimate(n) + (n > s->admit_n ? sys_transient_estimate(n) - sys_transient_estimate(s->admit_n) : 0);
+    };
+    if (s->used() + need_for() > s->budget) {   // idle launch buffers, admission scratch and given-up ones go first
       for (auto& f : s->flights)
         if (!f->busy) f->release_memory();
+      s->admit = AdmitScratch{};
+      s->admit_n = 0;
+      s->grave.drain();
     }
+    std::size_t const need = need_for();
     if (s->used() + need > s->budget) {
       throw Error(FLYG_BUSY, "device memory budget full: admit the system after running searches finish");
     }
-    auto sys = make_system(s->ctx, s->eam, basis, periodic, min_pos, types, n, s->sopt.list_skin);
+    auto sys = make_system(s->ctx, s->eam, basis, periodic, min_pos, types, n, s->sopt.list_skin, &s->admit);
+    s->admit_n = std::max(s->admit_n, static_cast<long>(n));
     sys->id = s->next_sys++;
     *sys_id = sys->id;
     s->systems[sys->id] = std::move(sys);
@@ -1425,6 +1496,7 @@ int flyg_server_add_system(flyg_server s, const double basis[9], const int perio
 
 int flyg_server_remove_system(flyg_server s, int64_t sys_id) {
   return bguard(s->ctx, [&] {
+    FreeLater defer(s->grave);
     auto it = s->systems.find(sys_id);
     require(it != s->systems.end() && !it->second->retired, "unknown system id");
     it->second->retired = true;
@@ -1484,6 +1556,7 @@ int flyg_server_submit_perturbed(flyg_server s, int64_t sys_id, int count, const
 
 int flyg_server_progress(flyg_server s, double wait_ms, int64_t* done_ids, int cap, int* n_done) {
   return bguard(s->ctx, [&] {
+    FreeLater defer(s->grave);
     require(n_done && (cap == 0 || done_ids), "null argument");
     auto t0 = std::chrono::steady_clock::now();
     while (true) {
@@ -1570,6 +1643,7 @@ int flyg_server_status(flyg_server s, flyg_server_state* st) {
     st->kernel_ms = s->kernel_ms;
     st->block_threads = s->tuned_T;
     st->lanes_per_atom = s->tuned_L;
+    st->host_ms = s->host_ms;
   });
 }
 

@@ -44,6 +44,51 @@ namespace flyg {
   }
 
   // ----------------------------------------------------------------------------------------------
+  // Deferred frees.  hipFree and hipHostFree wait until every kernel on the device has finished, so
+  // code that runs beside long kernels on other streams (the search server) must not call them.
+  // While a FreeLater scope is open on a thread, the buffers that thread's DBufs (and pinned staging)
+  // give up are collected in its Graveyard instead, and released by drain() when the device is idle.
+  // ----------------------------------------------------------------------------------------------
+
+  struct Graveyard {
+    struct Item {
+      void* p;
+      std::size_t bytes;
+      bool host;
+    };
+    std::vector<Item> items;
+    std::size_t bytes = 0;
+    Graveyard() = default;
+    Graveyard(Graveyard const&) = delete;
+    Graveyard& operator=(Graveyard const&) = delete;
+    void drain() {
+      for (auto const& it : items) (void)(it.host ? hipHostFree(it.p) : hipFree(it.p));
+      items.clear();
+      bytes = 0;
+    }
+    ~Graveyard() { drain(); }
+  };
+
+  inline thread_local Graveyard* tl_grave = nullptr;
+
+  struct FreeLater {
+    Graveyard* prev;
+    explicit FreeLater(Graveyard& g) : prev(tl_grave) { tl_grave = &g; }
+    ~FreeLater() { tl_grave = prev; }
+    FreeLater(FreeLater const&) = delete;
+    FreeLater& operator=(FreeLater const&) = delete;
+  };
+
+  inline hipError_t free_or_defer(void* p, std::size_t bytes, bool host) {
+    if (tl_grave) {
+      tl_grave->items.push_back({p, bytes, host});
+      tl_grave->bytes += bytes;
+      return hipSuccess;
+    }
+    return host ? hipHostFree(p) : hipFree(p);
+  }
+
+  // ----------------------------------------------------------------------------------------------
   // Growable device buffer (hipMalloc'd, never shrinks).
   // ----------------------------------------------------------------------------------------------
 
@@ -64,11 +109,11 @@ namespace flyg {
       return *this;
     }
     ~DBuf() {
-      if (p) (void)hipFree(p);
+      if (p) (void)free_or_defer(p, cap * sizeof(T), false);
     }
     void reserve(std::size_t n) {
       if (n <= cap) return;
-      if (p) HIPCHK(hipFree(p));
+      if (p) HIPCHK(free_or_defer(p, cap * sizeof(T), false));
       p = nullptr;
       std::size_t c = n < 64 ? 64 : n;
       HIPCHK(hipMalloc(&p, c * sizeof(T)));

@@ -115,7 +115,7 @@ class ServerState(C.Structure):
     _fields_ = [("queued", C.c_int64), ("in_flight", C.c_int64), ("systems", C.c_int64), ("device_bytes", C.c_int64),
                 ("budget_bytes", C.c_int64), ("launches", C.c_int64), ("searches_done", C.c_int64),
                 ("kernel_ms", C.c_double), ("busy_streams", C.c_int), ("block_threads", C.c_int),
-                ("lanes_per_atom", C.c_int)]
+                ("lanes_per_atom", C.c_int), ("host_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
