@@ -980,6 +980,7 @@ namespace flyg {
 }  // namespace flyg
 
 struct flyg_batch_s {
+  flyg::Graveyard grave;   // buffers given up while relaunches overlap the main launch (freed after the run)
   flyg_ctx_s* ctx = nullptr;
   flyg_eam_s* eam = nullptr;
   flyg_dimer_options opt;
@@ -1248,6 +1249,7 @@ int flyg_batch_submit_perturbed(flyg_batch b, int count, const int64_t* centre, 
 int flyg_batch_run(flyg_batch b) {
   return bguard(b->ctx, [&] {
     if (b->pending.empty()) return;
+    FreeLater defer(b->grave);
     DimOpt d = dim_opt(b->opt, b->ropt, &b->mopt);
     bool mix = b->bopt.precision == FLYG_MIXED;
     int slots = slots_of(*b->ctx, b->bopt.max_concurrent);
@@ -1333,6 +1335,7 @@ int flyg_batch_run(flyg_batch b) {
       if (!busy && todo.empty()) break;
       if (!launched) std::this_thread::sleep_for(std::chrono::microseconds(100));
     }
+    b->grave.drain();   // every launch has finished
   });
 }
 
