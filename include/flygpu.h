@@ -375,7 +375,8 @@ typedef struct {
   int lanes_per_atom;   /* lanes per atom row, 0 = autotuned */
   double list_skin;     /* initial Verlet skin of the per-search lists (A), 0 = default */
   int max_concurrent;   /* resident workgroups per launch, 0 = 2 per compute unit */
-  int streams;          /* concurrent launches, 0 = 4 */
+  int streams;          /* concurrent launches, 0 = 4 (launches overlap only on streams that have a hardware
+                           queue each: HIP gives a process GPU_MAX_HW_QUEUES of them, 4 unless set) */
   int max_batch;        /* searches per launch, 0 = 4 per compute unit */
   int64_t mem_budget;   /* device bytes for systems + launches, 0 = 80% of the free memory at creation */
 } flyg_server_options;
