@@ -107,7 +107,7 @@ struct flyg_list_s {
   // The gradient passes' layout, made at each rebuild: the real atoms in cell order (slot k holds atom
   // order[k], rank[i] is atom i's slot), their positions by slot (kept current by update), and the rows
   // by slot with neighbour indices given as slots (the same entries in the same order as nbr).
-  flyg::DBuf<int> order, rank, gflag;
+  flyg::DBuf<int> order, rank;
   flyg::DBuf<double4> gpos;
   flyg::DBuf<unsigned> grow;
   flyg::DBuf<int> gcnt;

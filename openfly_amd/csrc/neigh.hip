@@ -112,43 +112,6 @@ namespace flyg {
       gpos[rank[i]] = p;
     }
 
-    // The gradient layout (cell order of the real atoms): flag the sorted entries that are real atoms,
-    // scan, then place each real at its slot and copy its row with neighbour indices given as slots.
-    __global__ void k_real_flags(unsigned const* __restrict__ sid, long live, int* __restrict__ flag) {
-      long s = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (s < live) flag[s] = (sid[s] >> NB_SHIFT_BITS) == 13u ? 1 : 0;
-    }
-
-    __global__ void k_real_order(unsigned const* __restrict__ sid, int const* __restrict__ before, long live,
-                                 double4 const* __restrict__ pos, int* __restrict__ order, int* __restrict__ rank,
-                                 double4* __restrict__ gpos) {
-      long s = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (s >= live) return;
-      unsigned const id = sid[s];
-      if ((id >> NB_SHIFT_BITS) != 13u) return;
-      int const i = static_cast<int>(id & NB_IDX_MASK);
-      int const k = before[s];
-      order[k] = i;
-      rank[i] = k;
-      gpos[k] = pos[i];
-    }
-
-    __global__ void k_slot_rows(unsigned const* __restrict__ nbr, int const* __restrict__ ncnt,
-                                int const* __restrict__ order, int const* __restrict__ rank, long n, int K,
-                                unsigned* __restrict__ grow, int* __restrict__ gcnt) {
-      long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (t >= n * static_cast<long>(K)) return;
-      long const k = t / K;
-      int const e = static_cast<int>(t % K);
-      int const i = order[k];
-      int const c = ncnt[i];
-      if (e == 0) gcnt[k] = c;
-      if (e < c) {
-        unsigned const en = nbr[static_cast<long>(i) * K + e];
-        grow[t] = (en & ~NB_IDX_MASK) | static_cast<unsigned>(rank[en & NB_IDX_MASK]);
-      }
-    }
-
     __global__ void k_positions(double4 const* __restrict__ pos, long n, double* __restrict__ out) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (i >= n) return;
@@ -198,16 +161,15 @@ namespace flyg {
     // lstate[0] ghosts generated, [1] longest row, [3] overflow bits.
     enum : unsigned { OV_GHOST = 1u };
 
-    __global__ void k_clear(int* __restrict__ a, long na, int* __restrict__ b, long nb, unsigned* __restrict__ st) {
-      long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (i < na) a[i] = 0;
-      if (i < nb) b[i] = 0;
-      if (i < 4) st[i] = 0u;
-    }
-
+    // Canonical images of the reals; the same launch clears the build's counters (cell counts za[0..na),
+    // cell cursors zb[0..nb), the four state words), which only later kernels of the build use.
     __global__ void k_canon_reals(BoxDesc b, double const* __restrict__ xyz, long n, double4* __restrict__ pos,
-                                  double4* __restrict__ ent, unsigned* __restrict__ eid) {
+                                  double4* __restrict__ ent, unsigned* __restrict__ eid, int* __restrict__ za, long na,
+                                  int* __restrict__ zb, long nb, unsigned* __restrict__ st) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
+      if (i < na) za[i] = 0;
+      if (i < nb) zb[i] = 0;
+      if (i < 4) st[i] = 0u;
       if (i >= n) return;
       double x[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
       canon_image(b, x);
@@ -333,8 +295,11 @@ namespace flyg {
       return n + (g < gcap ? g : gcap);
     }
 
+    // Entries per cell in cnt[0 .. nc1), real atoms per cell in cnt[nc1 .. 2 nc1) (one scan of both gives
+    // the cell starts and, less the first half's total, the first slot of each cell's reals).
     __global__ void k_cell_count_cap(GridDesc g, double4 const* __restrict__ ent, long n, long gcap,
-                                     unsigned const* __restrict__ st, int* __restrict__ ecell, int* __restrict__ cnt) {
+                                     unsigned const* __restrict__ st, int* __restrict__ ecell, int* __restrict__ cnt,
+                                     long nc1) {
       long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (e >= live_entries(st, n, gcap)) return;
       double4 p = ent[e];
@@ -343,6 +308,7 @@ namespace flyg {
       int c = cell_idx(g, x, c3);
       ecell[e] = c;
       atomicAdd(&cnt[c], 1);
+      if (e < n) atomicAdd(&cnt[nc1 + c], 1);
     }
 
     // Counting-sort scatter of the packed ids only (order inside a cell follows the atomics).
@@ -356,21 +322,35 @@ namespace flyg {
     }
 
     // Places every entry at cell start + its rank among the cell's packed ids: cells come out sorted by
-    // id whatever order the atomics took, so rows (and every sum over them) are reproducible.
+    // id whatever order the atomics took, so rows (and every sum over them) are reproducible.  A real
+    // atom also gets its slot of the gradient layout: the reals in this sorted order (the cell's first
+    // real slot + the reals of the cell with smaller ids), with its position by slot.
     __global__ void k_rank_place(double4 const* __restrict__ ent, unsigned const* __restrict__ eid,
                                  int const* __restrict__ ecell, long n, long gcap, unsigned const* __restrict__ st,
-                                 int const* __restrict__ cstart, unsigned const* __restrict__ usid,
-                                 double4* __restrict__ spos, unsigned* __restrict__ sid, int* __restrict__ rslot) {
+                                 int const* __restrict__ cstart, long nc1, unsigned const* __restrict__ usid,
+                                 double4* __restrict__ spos, unsigned* __restrict__ sid, int* __restrict__ rslot,
+                                 int* __restrict__ order, int* __restrict__ rank_of, double4* __restrict__ gpos) {
       long e = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       if (e >= live_entries(st, n, gcap)) return;
       int c = ecell[e];
       unsigned id = eid[e];
       int b = cstart[c], end = cstart[c + 1];
-      int rank = 0;
-      for (int q = b; q < end; ++q) rank += usid[q] < id ? 1 : 0;
+      int rank = 0, reals_before = 0;
+      for (int q = b; q < end; ++q) {
+        unsigned const u = usid[q];
+        bool const lt = u < id;
+        rank += lt ? 1 : 0;
+        reals_before += (lt && (u >> NB_SHIFT_BITS) == 13u) ? 1 : 0;
+      }
       spos[b + rank] = ent[e];
       sid[b + rank] = id;
-      if (e < n) rslot[e] = b + rank;   // the first n entries are the reals, entry e = atom e
+      if (e < n) {   // the first n entries are the reals, entry e = atom e
+        rslot[e] = b + rank;
+        int const slot = cstart[nc1 + c] - cstart[nc1] + reals_before;
+        order[slot] = static_cast<int>(e);
+        rank_of[e] = slot;
+        gpos[slot] = ent[e];
+      }
     }
 
     // List::build_neigh_list (list.cpp:134-165): BL lanes per real atom walk the 3x3x3 block of cells
@@ -380,13 +360,18 @@ namespace flyg {
     constexpr int BL = 8;
     constexpr int BU = 4;   // candidate chunks loaded per step of the row build
 
+    // Atoms are taken in slot (cell) order, so a workgroup's candidates overlap; each row is written
+    // twice: by atom with atom indices (nbr) and by slot with slot indices (grow, the gradient layout).
     __global__ void k_build_rows(GridDesc g, double4 const* __restrict__ spos, unsigned const* __restrict__ sid,
                                  int const* __restrict__ cstart, int const* __restrict__ rslot, long n,
-                                 unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ st) {
+                                 unsigned* __restrict__ nbr, int* __restrict__ ncnt, int K, unsigned* __restrict__ st,
+                                 int const* __restrict__ order, int const* __restrict__ rank_of,
+                                 unsigned* __restrict__ grow, int* __restrict__ gcnt) {
       long gid = xcd_block(blockIdx.x, gridDim.x) * static_cast<long>(blockDim.x) + threadIdx.x;
-      long i = gid / BL;
+      long const slot = gid / BL;
       int const lane = static_cast<int>(gid % BL);
-      if (i >= n) return;   // whole groups leave together (blockDim is a multiple of BL)
+      if (slot >= n) return;   // whole groups leave together (blockDim is a multiple of BL)
+      long const i = order[slot];
       int const gbase = static_cast<int>(threadIdx.x & 63) - lane;
       int const self = rslot[i];
       double4 p = spos[self];
@@ -396,6 +381,7 @@ namespace flyg {
       double const rc2 = g.rc * g.rc;
       int cnt = 0;
       unsigned* row = nbr + i * static_cast<long>(K);
+      unsigned* srow = grow + slot * static_cast<long>(K);
       // The up-to-three x-neighbouring cells of a (y, z) line are consecutive cells (x fastest), so their
       // sorted atoms form one contiguous range: 9 ranges instead of 27 cells, in the same order.  Each
       // step loads BU chunks of BL candidates before testing them in order (more loads in flight, the
@@ -424,7 +410,11 @@ namespace flyg {
               unsigned gm = static_cast<unsigned>((m >> gbase) & ((1ull << BL) - 1ull));
               if (hit[u]) {
                 int k = cnt + __popc(gm & ((1u << lane) - 1u));
-                if (k < K) row[k] = sid[s];
+                if (k < K) {
+                  unsigned const en = sid[s];
+                  row[k] = en;
+                  srow[k] = (en & ~NB_IDX_MASK) | static_cast<unsigned>(rank_of[en & NB_IDX_MASK]);
+                }
               }
               cnt += __popc(gm);
             }
@@ -432,6 +422,7 @@ namespace flyg {
         }
       if (lane == 0) {
         ncnt[i] = cnt;
+        gcnt[slot] = cnt;
         // Only overflowing rows report their length (the host widens the rows to the longest one): an
         // atomic per atom on the one counter serialised the kernel (half of its time).
         if (cnt > K) atomicMax(&st[1], static_cast<unsigned>(cnt));
@@ -468,6 +459,44 @@ namespace flyg {
       if (threadIdx.x == SCAN_B - 1) sums[blockIdx.x] = wsum[SCAN_B / 64 - 1];
     }
 
+    // Exclusive scan of up to SCAN_B * SCAN_RUN values in one workgroup: thread t owns a run of
+    // consecutive values, the runs' totals are scanned across the block, then each run is written out.
+    constexpr int SCAN_RUN = 16;
+
+    __global__ void k_scan_one(int* __restrict__ data, int n) {
+      __shared__ int wsum[SCAN_B / 64];
+      int const per = (n + SCAN_B - 1) / SCAN_B;
+      int const b = static_cast<int>(threadIdx.x) * per;
+      int const e = min(n, b + per);
+      int v = 0;
+      for (int q = b; q < e; ++q) v += data[q];
+      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      int inc = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+      }
+      if (lane == 63) wsum[w] = inc;
+      __syncthreads();
+      if (w == 0) {
+        int s = lane < SCAN_B / 64 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          int t = __shfl_up(s, o, 64);
+          if (lane >= o) s += t;
+        }
+        if (lane < SCAN_B / 64) wsum[lane] = s;
+      }
+      __syncthreads();
+      int run = (w > 0 ? wsum[w - 1] : 0) + inc - v;
+      for (int q = b; q < e; ++q) {
+        int const x = data[q];
+        data[q] = run;
+        run += x;
+      }
+    }
+
     __global__ void k_scan_add(int* __restrict__ data, int n, int const* __restrict__ sums) {
       int gi = blockIdx.x * SCAN_B + threadIdx.x;
       if (gi < n) data[gi] += sums[blockIdx.x];
@@ -479,6 +508,11 @@ namespace flyg {
 
   void exclusive_scan(flyg_ctx_s& ctx, int* data, int n, DBuf<int>& tmp) {
     if (n <= 0) return;
+    if (n <= SCAN_B * SCAN_RUN) {   // one launch: each launch in a stream costs several microseconds
+      k_scan_one<<<1, SCAN_B, 0, ctx.stream>>>(data, n);
+      HIPCHK(hipGetLastError());
+      return;
+    }
     int nb = static_cast<int>((n + SCAN_B - 1) / SCAN_B);
     // Workspace layout: [sums of this level | workspace of the next level]
     std::vector<int> offs;
@@ -523,28 +557,6 @@ namespace flyg {
     }
   }
 
-  namespace {
-    void build_gradient_layout(flyg_list_s& nl) {
-      flyg_ctx_s& ctx = *nl.ctx;
-      hipStream_t st = ctx.stream;
-      long const n = nl.n, live = n + nl.nghost;
-      nl.order.reserve(n);
-      nl.rank.reserve(n);
-      nl.gpos.reserve(n);
-      nl.gcnt.reserve(n);
-      nl.grow.reserve(static_cast<std::size_t>(n) * nl.K);
-      nl.gflag.reserve(live + 1);
-      k_real_flags<<<nblk(live, TPB), TPB, 0, st>>>(nl.sid.get(), live, nl.gflag.get());
-      exclusive_scan(ctx, nl.gflag.get(), static_cast<int>(live), nl.scan_tmp);
-      k_real_order<<<nblk(live, TPB), TPB, 0, st>>>(nl.sid.get(), nl.gflag.get(), live, nl.pos.get(), nl.order.get(),
-                                                    nl.rank.get(), nl.gpos.get());
-      k_slot_rows<<<nblk(n * static_cast<long>(nl.K), TPB), TPB, 0, st>>>(nl.nbr.get(), nl.ncnt.get(), nl.order.get(),
-                                                                          nl.rank.get(), n, nl.K, nl.grow.get(),
-                                                                          nl.gcnt.get());
-      HIPCHK(hipGetLastError());
-    }
-  }  // namespace
-
   // One rebuild costs one host synchronisation: capacities (ghost slots, row length) come from the
   // previous build; the device records what it needed and the build is redone only on overflow.
   void list_rebuild(flyg_list_s& nl, double const* d_xyz, long n) {
@@ -564,10 +576,14 @@ namespace flyg {
     if (nl.K == 0) nl.K = 64;
     int const nc = nl.grid.ncells;
     nl.pos.reserve(n);
-    nl.cstart.reserve(nc + 1);
+    nl.cstart.reserve(2 * (nc + 1));   // cell starts, then the first real slot of each cell (offset)
     nl.ccursor.reserve(nc);
     nl.ncnt.reserve(n);
     nl.rslot.reserve(n);
+    nl.order.reserve(n);
+    nl.rank.reserve(n);
+    nl.gpos.reserve(n);
+    nl.gcnt.reserve(n);
     unsigned* dst = nl.lstate.get();
 
     for (int attempt = 0; attempt < 3; ++attempt) {
@@ -579,20 +595,23 @@ namespace flyg {
       nl.sid.reserve(cap);
       nl.usid.reserve(cap);
       nl.nbr.reserve(static_cast<std::size_t>(n) * nl.K);
-      long nz = std::max<long>(nc + 1, 4);
-      k_clear<<<nblk(nz, TPB), TPB, 0, st>>>(nl.cstart.get(), nc + 1, nl.ccursor.get(), nc, dst);
-      k_canon_reals<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get());
+      nl.grow.reserve(static_cast<std::size_t>(n) * nl.K);
+      long const nc1 = nc + 1;
+      long const nz = std::max<long>(std::max<long>(n, 2 * nc1), 4);
+      k_canon_reals<<<nblk(nz, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get(),
+                                                   nl.cstart.get(), 2 * nc1, nl.ccursor.get(), nc, dst);
       k_ghosts_cap<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), dst, nl.gcap);
       k_cell_count_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), n, nl.gcap, dst, nl.ecell.get(),
-                                                       nl.cstart.get());
-      exclusive_scan(ctx, nl.cstart.get(), nc + 1, nl.scan_tmp);
+                                                       nl.cstart.get(), nc1);
+      exclusive_scan(ctx, nl.cstart.get(), static_cast<int>(2 * nc1), nl.scan_tmp);
       k_scatter_ids<<<nblk(cap, TPB), TPB, 0, st>>>(nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst, nl.cstart.get(),
                                                     nl.ccursor.get(), nl.usid.get());
       k_rank_place<<<nblk(cap, TPB), TPB, 0, st>>>(nl.ent.get(), nl.eid.get(), nl.ecell.get(), n, nl.gcap, dst,
-                                                   nl.cstart.get(), nl.usid.get(), nl.spos.get(), nl.sid.get(),
-                                                   nl.rslot.get());
+                                                   nl.cstart.get(), nc1, nl.usid.get(), nl.spos.get(), nl.sid.get(),
+                                                   nl.rslot.get(), nl.order.get(), nl.rank.get(), nl.gpos.get());
       k_build_rows<<<nblk(n * BL, TPB), TPB, 0, st>>>(nl.grid, nl.spos.get(), nl.sid.get(), nl.cstart.get(), nl.rslot.get(),
-                                                 n, nl.nbr.get(), nl.ncnt.get(), nl.K, dst);
+                                                 n, nl.nbr.get(), nl.ncnt.get(), nl.K, dst, nl.order.get(),
+                                                 nl.rank.get(), nl.grow.get(), nl.gcnt.get());
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(nl.hstate, dst, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -606,7 +625,6 @@ namespace flyg {
         nl.K = (static_cast<int>(maxrow) + 15) / 16 * 16 + 16;
         continue;
       }
-      build_gradient_layout(nl);
       static std::atomic<long> builds{0};
       nl.build_id = ++builds;
       nl.built = true;
