@@ -322,7 +322,7 @@ def cpu_search_leg(S, build, sample=16, repeats=5):
     return base, {"barriers": barriers}
 
 
-def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
+def serve_bench(ctx, fa, systems, n_req=64, rate=200.0, streams=SERVE_STREAMS):
     """Serving: Poisson arrivals at 200/s of requests from a simulated kMC loop, each on its own system
     (bcc Fe, N log-uniform in [2000, 250000], 0.5-2 % vacancies + interstitials, relaxed), 8-64 searches
     each centred on the system's catalogue misses (radius U(4, 9) A, Master::perturb starts in the
@@ -336,7 +336,7 @@ def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
     t0 = time.perf_counter()
     pool = serve.make_pool(n_req, seed=41, relax=lambda sy: relax(ctx, fa, pot, sy))
     t_pool = time.perf_counter() - t0
-    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3), streams=SERVE_STREAMS)
+    srv = fa.Server(ctx, pot, dimer=dict(f2norm=1e-3), streams=streams)
     log("serving leg: warm-up")
     serve.serve(srv, serve.make_requests(pool[:2], 4, rate=rate, seed=40, searches=(8, 16)))   # + shape autotune
     # Request source: each system is one kMC state whose last hop was at one of its defects; the catalogue
@@ -384,7 +384,8 @@ def serve_bench(ctx, fa, systems, n_req=64, rate=200.0):
                catalogue={"environments": cat.size, "seconds": t_cat, "misses_per_system_min": int(min(misses)),
                           "misses_per_system_max": int(max(misses)),
                           "misses_per_system_mean": float(np.mean(misses))},
-               pool_relax_seconds=t_pool, launches=st["launches"], kernel_ms=st["kernel_ms"],
+               pool_relax_seconds=t_pool, launches=st["launches"], kernel_ms=st["kernel_ms"], host_ms=st["host_ms"],
+               streams=streams, hw_queues=os.environ.get("GPU_MAX_HW_QUEUES"),
                block_threads=st["block_threads"], lanes_per_atom=st["lanes_per_atom"],
                system_atoms_min=int(min(p.n for p in pool)), system_atoms_max=int(max(p.n for p in pool)))
     return out
@@ -570,6 +571,8 @@ def main():
                     help="search leg: the mixed batch only, no FP64 batch, mechanisms or plain form (profiling)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served saddle-search leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 1M-atom, 1024-search leg")
+    ap.add_argument("--serve-streams", type=int, default=SERVE_STREAMS,
+                    help="concurrent launches of the serving leg (set GPU_MAX_HW_QUEUES to match)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 64 MiB device copies (the known-size kernel that scales FETCH_SIZE / "
                          "WRITE_SIZE readings to bytes) and exit")
@@ -647,7 +650,7 @@ def main():
         log("search leg done")
         out["searches"] = sr
     if not args.no_serve:
-        out["serving"] = serve_bench(ctx, fa, systems)
+        out["serving"] = serve_bench(ctx, fa, systems, streams=args.serve_streams)
         log("serving leg done")
     if not args.no_large:
         log("1M-atom leg: building and relaxing the system")
