@@ -459,7 +459,39 @@ namespace flyg {
           if (lj >= nm) g.ent[fill[lj - nm]++] = ((26u - sc) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
         }
       }
-      for (int r = 0; r < g.nr; ++r) std::sort(g.ent.begin() + roff[r], g.ent.begin() + roff[r + 1]);
+      // Each row in ascending packed order.  Its entries are distinct (shift code, local index < nr) pairs,
+      // so a bitmap of local indices per shift code present orders a long row in O(nr / 64 + length).
+      std::vector<uint64_t> bits((g.nr + 63) / 64, 0);
+      std::vector<unsigned> tmp;
+      for (int r = 0; r < g.nr; ++r) {
+        unsigned* const b = g.ent.data() + roff[r];
+        unsigned* const e = g.ent.data() + roff[r + 1];
+        if (e - b < 48) {
+          std::sort(b, e);
+          continue;
+        }
+        uint32_t present = 0;
+        for (unsigned const* q = b; q != e; ++q) present |= 1u << (*q >> NB_SHIFT_BITS);
+        tmp.assign(b, e);
+        unsigned* out = b;
+        for (unsigned sc = 0; sc < 27; ++sc) {
+          if (!(present & (1u << sc))) continue;
+          for (unsigned en : tmp)
+            if ((en >> NB_SHIFT_BITS) == sc) {
+              unsigned const l = en & NB_IDX_MASK;
+              bits[l >> 6] |= 1ull << (l & 63);
+            }
+          for (std::size_t w = 0; w < bits.size(); ++w) {
+            uint64_t m = bits[w];
+            bits[w] = 0;
+            while (m) {
+              unsigned const l = static_cast<unsigned>(w * 64 + static_cast<unsigned>(__builtin_ctzll(m)));
+              *out++ = (sc << NB_SHIFT_BITS) | l;
+              m &= m - 1;
+            }
+          }
+        }
+      }
       for (int j : g.gidx) local_of[j] = -1;
     }
 
