@@ -4,8 +4,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -249,6 +252,18 @@ namespace flyg {
       std::vector<double> norm;
     };
 
+    struct Region;
+
+    // A region being built in the background for the skins a search was queued with.
+    struct RegionTask {
+      std::mutex m;
+      std::condition_variable cv;
+      bool done = false;
+      double skin_f = 0, skin_m = 0;
+      std::shared_ptr<Region> region;
+      std::exception_ptr err;
+    };
+
     struct Pending {
       int64_t id;
       int launches = 0;
@@ -268,6 +283,7 @@ namespace flyg {
       std::vector<int> mobile;      // global indices (sorted)
       std::vector<double> x, ax;    // 3*nm start state (continuous with the base positions)
       std::shared_ptr<Region> region;   // built for the skins, reset when they grow
+      std::shared_ptr<RegionTask> task; // its background build (server), used while the skins are unchanged
       // State saved at a region exit (resume_len doubles) and the loop position it belongs to: the
       // relaunch in the grown region continues from it instead of from the start.
       bool resume = false;
@@ -409,17 +425,18 @@ namespace flyg {
     // grid query per mobile atom (rM >= rF) finds its row and the S1 atoms; the S1 rows are the mobile
     // rows' S1 entries transposed (the image of i seen from j has the opposite shift), so each pair is
     // tested once and the two rows always agree.
-    void build_region(SysState const& S, Pending const& P, double rc, std::vector<int>& local_of, Region& g) {
-      double const rF = rc + P.skin_f, rM = rc + P.skin_m, rq = std::max(rF, rM);
-      int const nm = static_cast<int>(P.mobile.size());
+    void build_region(SysState const& S, std::vector<int> const& mobile, double skin_f, double skin_m, double rc,
+                      std::vector<int>& local_of, Region& g) {
+      double const rF = rc + skin_f, rM = rc + skin_m, rq = std::max(rF, rM);
+      int const nm = static_cast<int>(mobile.size());
       g.nm = nm;
-      g.gidx = P.mobile;
-      for (int i = 0; i < nm; ++i) local_of[P.mobile[i]] = i;
+      g.gidx = mobile;
+      for (int i = 0; i < nm; ++i) local_of[mobile[i]] = i;
       std::vector<int> s1, hj;
       std::vector<unsigned char> hs;
       std::vector<int> hoff(nm + 1, 0);
       for (int i = 0; i < nm; ++i) {
-        int const gi = P.mobile[i];
+        int const gi = mobile[i];
         S.grid.query(&S.base[3 * gi], rq, [&](int j, int sc, double r2) {
           if (j == gi && sc == 13) return;
           int const lj = local_of[j];
@@ -511,10 +528,96 @@ namespace flyg {
       for (auto& x : th) x.join();
     }
 
-    // Builds the missing regions of `work` (in parallel; one -1-filled scratch map per thread).
+    // Workers that build search regions in the background: a server queues each search's region when
+    // the search is submitted, so a launch finds it built instead of building it on the caller's thread
+    // (which also harvests results and takes new requests).  Each worker keeps a -1-filled scratch map.
+    class RegionPool {
+     public:
+      explicit RegionPool(unsigned threads) {
+        for (unsigned t = 0; t < threads; ++t) th_.emplace_back([this] { run(); });
+      }
+      ~RegionPool() {
+        {
+          std::lock_guard<std::mutex> l(m_);
+          stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+      }
+      RegionPool(RegionPool const&) = delete;
+      RegionPool& operator=(RegionPool const&) = delete;
+
+      // Builds P's region for its current skins on a worker; P.task receives it.
+      void submit(Pending& P, double rc) {
+        auto t = std::make_shared<RegionTask>();
+        t->skin_f = P.skin_f;
+        t->skin_m = P.skin_m;
+        P.task = t;
+        SysState const* S = P.sys;
+        std::vector<int> mobile = P.mobile;
+        {
+          std::lock_guard<std::mutex> l(m_);
+          q_.push_back([t, S, mobile = std::move(mobile), rc](std::vector<int>& scratch) {
+            std::shared_ptr<Region> g;
+            std::exception_ptr err;
+            try {
+              if (static_cast<long>(scratch.size()) < S->n) scratch.assign(S->n, -1);
+              g = std::make_shared<Region>();
+              build_region(*S, mobile, t->skin_f, t->skin_m, rc, scratch, *g);
+            } catch (...) {
+              err = std::current_exception();
+              std::fill(scratch.begin(), scratch.end(), -1);
+            }
+            {
+              std::lock_guard<std::mutex> l(t->m);
+              t->region = std::move(g);
+              t->err = err;
+              t->done = true;
+            }
+            t->cv.notify_all();
+          });
+        }
+        cv_.notify_one();
+      }
+
+     private:
+      void run() {
+        std::vector<int> scratch;
+        while (true) {
+          std::function<void(std::vector<int>&)> f;
+          {
+            std::unique_lock<std::mutex> l(m_);
+            cv_.wait(l, [&] { return stop_ || !q_.empty(); });
+            if (q_.empty()) return;   // stopping and drained
+            f = std::move(q_.front());
+            q_.pop_front();
+          }
+          f(scratch);
+        }
+      }
+      std::mutex m_;
+      std::condition_variable cv_;
+      std::deque<std::function<void(std::vector<int>&)>> q_;
+      bool stop_ = false;
+      std::vector<std::thread> th_;
+    };
+
+    // Builds the missing regions of `work`: a background build for the search's current skins is
+    // waited for, the rest are built here in parallel (one -1-filled scratch map per thread).
     void ensure_regions(std::vector<Pending>& work, double rc) {
       std::vector<long> need;
       long nmax = 0;
+      for (auto& P : work) {
+        if (P.region || !P.task) continue;
+        RegionTask& t = *P.task;
+        if (t.skin_f == P.skin_f && t.skin_m == P.skin_m) {
+          std::unique_lock<std::mutex> l(t.m);
+          t.cv.wait(l, [&] { return t.done; });
+          if (t.err) std::rethrow_exception(t.err);
+          P.region = t.region;
+        }
+        P.task.reset();
+      }
       for (long k = 0; k < static_cast<long>(work.size()); ++k)
         if (!work[k].region) {
           need.push_back(k);
@@ -526,7 +629,7 @@ namespace flyg {
         if (static_cast<long>(scratch[t].size()) < nmax) scratch[t].assign(nmax, -1);
         Pending& P = work[need[q]];
         auto g = std::make_shared<Region>();
-        build_region(*P.sys, P, rc, scratch[t], *g);
+        build_region(*P.sys, P.mobile, P.skin_f, P.skin_m, rc, scratch[t], *g);
         P.region = std::move(g);
       });
     }
@@ -906,6 +1009,7 @@ namespace flyg {
           q.skin_f = std::min(cap, std::max(q.skin_f * 1.5, o.max_disp + 1.0));
           q.skin_m = std::min(cap, std::max(std::max(q.skin_m * 1.5, q.skin_f), o.max_pair + 1.5));
           q.region.reset();
+          q.task.reset();
           if (o.saved > 0 && o.saved <= RS_SLOTS) {   // continue where it left its region
             double const* src = reinterpret_cast<double const*>(F.h_out.p + F.o_rs) + (o.saved - 1) * F.rs_len;
             q.resume = true;
@@ -1053,6 +1157,10 @@ struct flyg_server_s {
   int64_t launches = 0, searches_done = 0;
   double kernel_ms = 0;
   double host_ms = 0;   // region builds and launch packing
+
+  // Background region builds of submitted searches (last member: its workers finish before the systems
+  // they read go away).
+  std::unique_ptr<flyg::RegionPool> pool;
 
   std::size_t used() const {
     std::size_t b = grave.bytes + flyg::sys_transient_estimate(admit_n);
@@ -1462,6 +1570,7 @@ int flyg_server_create(flyg_ctx ctx, flyg_eam eam, const flyg_dimer_options* opt
     require(ctx && eam && out, "null argument");
     require(eam->ctx == ctx, "potential belongs to another context");
     auto s = std::make_unique<flyg_server_s>();
+    s->pool = std::make_unique<RegionPool>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     s->ctx = ctx;
     s->eam = eam;
     take_options(opt, ropt, s->opt, s->ropt);
@@ -1559,6 +1668,7 @@ int flyg_server_submit(flyg_server s, int64_t sys_id, int count, const int64_t* 
       int64_t id = s->next_id++;
       s->owner[id] = sys_id;
       if (ids) ids[k] = id;
+      s->pool->submit(add[k], s->eam->tab.rc);
       s->queue.push_back(std::move(add[k]));
       ++S.active;
     }
@@ -1583,6 +1693,7 @@ int flyg_server_submit_perturbed(flyg_server s, int64_t sys_id, int count, const
       int64_t id = s->next_id++;
       s->owner[id] = sys_id;
       if (ids) ids[k] = id;
+      s->pool->submit(add[k], s->eam->tab.rc);
       s->queue.push_back(std::move(add[k]));
       ++S.active;
     }
