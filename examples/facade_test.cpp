@@ -197,6 +197,16 @@ int main(int argc, char** argv) {
     }
     std::printf("Master::find_batch: %d of %zu converged, %d pathways (%d distinct), cache %zu\n", converged,
                 found.size(), pathways, fresh, cache.size());
+    // Symmetric copies (Master::append_syms, find.cpp:1279-1303): atom 0 sits beside the vacancy, so its
+    // environment keeps C3v's six operations about the vacancy direction.
+    auto syms = env::site_symmetries(cell.box, in, 0, master.site_atoms(0, in));
+    std::vector<saddle::Master::Mech> all = mechs;
+    std::size_t copies = 0;
+    for (auto const& m : mechs) copies += master.append_syms(syms, m, all);
+    std::printf("append_syms: %zu symmetries, %zu distinct pathways + %zu symmetric copies\n", syms.size(),
+                mechs.size(), copies);
+    REQUIRE(syms.size() == 6);
+    REQUIRE(all.size() == mechs.size() + copies);
     REQUIRE(found.size() == 16);
     REQUIRE(static_cast<int>(cache.size()) == converged);
     REQUIRE(converged >= 1);

@@ -19,6 +19,7 @@
 #include <fstream>
 #include <iterator>
 #include <memory>
+#include <numeric>
 #include <optional>
 #include <sstream>
 #include <stdexcept>
@@ -572,6 +573,221 @@ namespace fly {
   }  // namespace minimise
 
   // ---- saddle (rotor.hpp, dimer.hpp) -------------------------------------------------------------
+  // ---- env: self-symmetries of a local environment (env/geometry.hpp:125-281, catalogue.cpp:33-64) ------
+  namespace env {
+
+    // A symmetry of a geometry: the permutation `perm` and orthogonal map O with O x[perm[i]] ~ x[i]
+    // (Catalogue::SelfSymetry, catalogue.hpp:157-160).
+    struct SelfSymmetry {
+      Mat O;
+      std::vector<std::size_t> perm;
+    };
+
+    namespace detail {
+      using P3 = std::array<double, 3>;
+
+      inline double dist(P3 const& a, P3 const& b) {
+        double const x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+        return std::sqrt(x * x + y * y + z * z);
+      }
+
+      inline bool invert(Mat const& a, Mat& inv) {
+        double const det = a(0, 0) * (a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1)) -
+                           a(0, 1) * (a(1, 0) * a(2, 2) - a(1, 2) * a(2, 0)) +
+                           a(0, 2) * (a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0));
+        if (!(std::abs(det) > 1e-300)) return false;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            int const r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+            inv(r, c) = (a(r1, c1) * a(r2, c2) - a(r1, c2) * a(r2, c1)) / det;
+          }
+        return true;
+      }
+
+      // ortho_onto (geometry.hpp:125-160): the orthogonal O minimising sum |O x_i - y_i|^2, reflections
+      // allowed.  It is the orthogonal polar factor of M = sum y_i x_i^T (for H = sum x_i y_i^T = U S V^T it
+      // equals V U^T), found by the Newton iteration X <- (X + X^-T) / 2 from X = M (Higham, Functions of
+      // Matrices, ch. 8).  False when M is singular (a planar geometry).
+      inline bool ortho_onto(std::vector<P3> const& x, std::vector<P3> const& y, Mat& O) {
+        Mat X;
+        for (std::size_t i = 0; i < x.size(); ++i)
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) X(r, c) += y[i][static_cast<std::size_t>(r)] * x[i][static_cast<std::size_t>(c)];
+        for (int it = 0; it < 100; ++it) {
+          Mat inv, next;
+          if (!invert(X, inv)) return false;
+          double change = 0, size = 0;
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+              next(r, c) = 0.5 * (X(r, c) + inv(c, r));
+              change += std::abs(next(r, c) - X(r, c));
+              size += std::abs(next(r, c));
+            }
+          X = next;
+          if (change <= 1e-15 * size) break;
+        }
+        O = X;
+        return true;
+      }
+
+      // grmsd (geometry.hpp:100-123): sqrt(sum |O x_i - y_i|^2).
+      inline double grmsd(Mat const& O, std::vector<P3> const& x, std::vector<P3> const& y) {
+        double s = 0;
+        for (std::size_t i = 0; i < x.size(); ++i)
+          for (int r = 0; r < 3; ++r) {
+            double v = -y[i][static_cast<std::size_t>(r)];
+            for (int c = 0; c < 3; ++c) v += O(r, c) * x[i][static_cast<std::size_t>(c)];
+            s += v * v;
+          }
+        return std::sqrt(s);
+      }
+
+      // GREEDY_PERM (for_equiv_perms_impl, geometry.hpp:186-229): place an atom of matching colour at
+      // position n, keep it while its distances to the atoms already placed match the reference's within
+      // sqrt(2) delta, recurse; a complete permutation counts when its ortho_onto map brings it within
+      // delta (grmsd) of the reference, and every such permutation is collected.  Any permutation that
+      // counts passes every pair's test (|d(y_i, y_j) - d(x_i, x_j)| <= r_i + r_j <= sqrt(2) grmsd), so
+      // testing all placed atoms (the reference tests the first ten) finds the same set while pruning the
+      // search far harder; positions are taken nearest-first for the same reason.
+      struct Greedy {
+        std::vector<P3> const& ref;
+        std::vector<int> const& col;
+        double delta;
+        std::vector<std::size_t> idx;   // idx[k]: the atom placed at position k
+        std::vector<P3> mut;
+        std::vector<SelfSymmetry> out;
+        long nodes = 0;
+
+        bool within_tol(std::size_t n) const {
+          double const tol = delta * std::sqrt(2.0);
+          for (std::size_t i = 0; i < n; ++i)
+            if (std::abs(dist(ref[n], ref[i]) - dist(mut[n], mut[i])) > tol) return false;
+          return true;
+        }
+        void rec(std::size_t n) {
+          if (++nodes > 50000000L) throw RuntimeError("self_symmetries: search too large (delta too loose)");
+          if (n >= ref.size()) {
+            Mat O;
+            if (ortho_onto(mut, ref, O) && grmsd(O, mut, ref) < delta) out.push_back(SelfSymmetry{O, idx});
+            return;
+          }
+          for (std::size_t i = n; i < ref.size(); ++i) {
+            if (col[idx[i]] != col[n]) continue;
+            std::swap(idx[n], idx[i]);
+            std::swap(mut[n], mut[i]);
+            if (within_tol(n)) rec(n + 1);
+            std::swap(idx[n], idx[i]);
+            std::swap(mut[n], mut[i]);
+          }
+        }
+      };
+    }  // namespace detail
+
+    // Catalogue::calc_self_syms (catalogue.cpp:33-64): every (O, perm) mapping the geometry onto itself
+    // within delta, atom 0 (the centre) fixed.  `pos` relative to the geometry's centroid, `colour` per atom.
+    inline std::vector<SelfSymmetry> self_symmetries(std::vector<std::array<double, 3>> const& pos,
+                                                     std::vector<int> const& colour, double delta) {
+      if (pos.size() != colour.size() || pos.empty()) throw RuntimeError("self_symmetries: bad geometry");
+      // Nearest-first order (atom 0 stays first); the permutations come back in the caller's order.
+      std::vector<std::size_t> ord(pos.size());
+      std::iota(ord.begin(), ord.end(), std::size_t{0});
+      std::stable_sort(ord.begin() + 1, ord.end(), [&](std::size_t a, std::size_t b) {
+        return detail::dist(pos[a], pos[0]) < detail::dist(pos[b], pos[0]);
+      });
+      std::vector<std::array<double, 3>> p2;
+      std::vector<int> c2;
+      for (std::size_t k : ord) {
+        p2.push_back(pos[k]);
+        c2.push_back(colour[k]);
+      }
+      detail::Greedy g{p2, c2, delta, {}, p2, {}};
+      g.idx.resize(p2.size());
+      std::iota(g.idx.begin(), g.idx.end(), std::size_t{0});
+      g.rec(1);
+      std::vector<SelfSymmetry> out;
+      for (auto const& s : g.out) {
+        SelfSymmetry t{s.O, std::vector<std::size_t>(pos.size())};
+        for (std::size_t k = 0; k < ord.size(); ++k) t.perm[ord[k]] = ord[s.perm[k]];
+        out.push_back(std::move(t));
+      }
+      return out;
+    }
+
+    // The symmetries of the local environment of `centre` (atoms within r_env, minimum image, colour
+    // 2 type + frozen, Geometry::centre's centroid origin; delta = 0.4 r_min as Catalogue::calc_delta with
+    // an unbounded delta_max), expressed on `atoms` (e.g. the atoms a Master::Mech runs over): each kept
+    // symmetry maps every one of them onto one of them of the same colour within delta (per atom), and
+    // perm[i] is the atom whose image lands on atom i.  Symmetries of the environment that the wider set
+    // breaks are dropped.
+    inline std::vector<SelfSymmetry> site_symmetries(system::Box const& box,
+                                                     system::SoA<Position const&, TypeID const&, Frozen const&> const& in,
+                                                     std::int64_t centre, std::vector<std::int64_t> const& atoms,
+                                                     double r_env = 5.2) {
+      auto pos = in[r_];
+      auto rel = [&](std::int64_t i) {
+        std::array<double, 3> d{};
+        for (int c = 0; c < 3; ++c) {
+          double v = pos[3 * i + c] - pos[3 * centre + c];
+          double const L = box.basis_data()[4 * c];
+          if (box.periodic(c)) v -= L * std::round(v / L);
+          d[static_cast<std::size_t>(c)] = v;
+        }
+        return d;
+      };
+      auto colour = [&](std::int64_t i) { return 2 * static_cast<int>(in[id_][i]) + (in[fzn_][i] ? 1 : 0); };
+      std::vector<std::array<double, 3>> geo{rel(centre)};
+      std::vector<int> col{colour(centre)};
+      for (std::int64_t i = 0; i < in.size(); ++i) {
+        if (i == centre) continue;
+        auto d = rel(i);
+        if (detail::dist(d, {0, 0, 0}) < r_env) {
+          geo.push_back(d);
+          col.push_back(colour(i));
+        }
+      }
+      std::array<double, 3> cen{};
+      for (auto const& p : geo)
+        for (std::size_t c = 0; c < 3; ++c) cen[c] += p[c] / static_cast<double>(geo.size());
+      for (auto& p : geo)
+        for (std::size_t c = 0; c < 3; ++c) p[c] -= cen[c];
+      double r_min = 1e300;
+      for (std::size_t i = 0; i < geo.size(); ++i)
+        for (std::size_t j = i + 1; j < geo.size(); ++j) r_min = std::min(r_min, detail::dist(geo[i], geo[j]));
+      double const delta = 0.4 * r_min;
+      std::vector<SelfSymmetry> env_syms = self_symmetries(geo, col, delta);
+      // On `atoms`: positions about the same centroid.
+      std::vector<std::array<double, 3>> x;
+      std::vector<int> xc;
+      for (std::int64_t a : atoms) {
+        auto d = rel(a);
+        for (std::size_t c = 0; c < 3; ++c) d[c] -= cen[c];
+        x.push_back(d);
+        xc.push_back(colour(a));
+      }
+      std::vector<SelfSymmetry> out;
+      for (auto const& s : env_syms) {
+        SelfSymmetry t{s.O, std::vector<std::size_t>(x.size(), x.size())};
+        bool ok = true;
+        for (std::size_t a = 0; a < x.size() && ok; ++a) {
+          std::array<double, 3> y{};
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) y[static_cast<std::size_t>(r)] += s.O(r, c) * x[a][static_cast<std::size_t>(c)];
+          std::size_t best = x.size();
+          for (std::size_t b = 0; b < x.size(); ++b)
+            if (xc[b] == xc[a] && detail::dist(x[b], y) < delta) {
+              best = b;
+              break;
+            }
+          if (best == x.size() || t.perm[best] != x.size()) ok = false;
+          else t.perm[best] = a;
+        }
+        if (ok) out.push_back(std::move(t));
+      }
+      return out;
+    }
+
+  }  // namespace env
+
   namespace saddle {
 
     class Rotor {
@@ -688,10 +904,11 @@ namespace fly {
     };
 
     // Master (find.hpp:260-306, find.cpp:178-194): the saddle-search core of Master::find_mechs, with the
-    // searches of a site run as one device batch.  find_mechs itself (catalogue geometries, symmetry
-    // expansion) needs the reference's env layer and is not part of this facade; find_batch below is its
-    // per-site step (find.cpp:684-755) with the mechanism checks of saddle_2_mech (find.cpp:757-837), the
-    // prefactor of the mass-weighted Hessians (find.cpp:838-871, 287) and is_new_mech's de-duplication.
+    // searches of a site run as one device batch.  find_mechs itself (the catalogue's canonical geometries)
+    // needs the reference's env layer and is not part of this facade; find_batch below is its per-site step
+    // (find.cpp:684-755) with the mechanism checks of saddle_2_mech (find.cpp:757-837), the prefactor of the
+    // mass-weighted Hessians (find.cpp:838-871, 287), is_new_mech's de-duplication and append_syms's
+    // symmetric copies (with env::site_symmetries in place of the catalogue's self-symmetries).
     class Master {
     public:
       struct Options {
@@ -734,6 +951,9 @@ namespace fly {
       // saddle and forward displacements are both within mech_tol (root of the summed squared per-atom
       // differences), the saddle displacements point within 10 degrees and the axes within 5 (either sign).
       bool is_new_mech(Mech const& maybe, std::vector<Mech> const& hist) const {
+        return is_new_mech(maybe, hist, m_opt.mech_tol);
+      }
+      static bool is_new_mech(Mech const& maybe, std::vector<Mech> const& hist, double mech_tol) {
         for (Mech const& m : hist) {
           double d_sp = 0, d_fwd = 0, dot = 0, n1 = 0, n2 = 0, dot_ax = 0;
           for (std::size_t i = 0; i < m.delta_sp.size(); ++i) {
@@ -748,10 +968,45 @@ namespace fly {
           double theta = std::acos(std::clamp(dot / (std::sqrt(n1) * std::sqrt(n2)), -1., 1.)) / (2 * M_PI) * 360;
           double ax_theta = std::acos(std::clamp(dot_ax, -1., 1.)) / (2 * M_PI) * 360;
           ax_theta = std::min(ax_theta, 180 - ax_theta);
-          if (std::sqrt(d_sp) < m_opt.mech_tol && std::sqrt(d_fwd) < m_opt.mech_tol && theta < 10 && ax_theta < 5)
-            return false;
+          if (std::sqrt(d_sp) < mech_tol && std::sqrt(d_fwd) < mech_tol && theta < 10 && ax_theta < 5) return false;
         }
         return true;
+      }
+
+      // Master::append_syms (find.cpp:1279-1303): for each symmetry (O, perm) of the mechanism's atoms, the
+      // mechanism with atom i taking O times atom perm[i]'s axis, saddle and forward displacements is appended
+      // when is_new_mech finds it new.  Returns how many were appended.  The symmetries come from
+      // env::site_symmetries over the same atoms.
+      std::size_t append_syms(std::vector<env::SelfSymmetry> const& syms, Mech const& new_mech,
+                              std::vector<Mech>& mechs) const {
+        return append_syms(syms, new_mech, mechs, m_opt.mech_tol);
+      }
+      static std::size_t append_syms(std::vector<env::SelfSymmetry> const& syms, Mech const& new_mech,
+                                     std::vector<Mech>& mechs, double mech_tol) {
+        Mech mech = new_mech;
+        std::size_t count = 0;
+        for (auto const& s : syms) {
+          if (s.perm.size() * 3 != new_mech.delta_sp.size()) throw RuntimeError("append_syms: symmetry and mechanism sizes differ");
+          for (std::size_t i = 0; i < s.perm.size(); ++i) {
+            std::size_t const j = s.perm[i];
+            for (int r = 0; r < 3; ++r) {
+              double a = 0, d = 0, f = 0;
+              for (int c = 0; c < 3; ++c) {
+                a += s.O(r, c) * new_mech.axis[3 * j + static_cast<std::size_t>(c)];
+                d += s.O(r, c) * new_mech.delta_sp[3 * j + static_cast<std::size_t>(c)];
+                f += s.O(r, c) * new_mech.delta_fwd[3 * j + static_cast<std::size_t>(c)];
+              }
+              mech.axis[3 * i + static_cast<std::size_t>(r)] = a;
+              mech.delta_sp[3 * i + static_cast<std::size_t>(r)] = d;
+              mech.delta_fwd[3 * i + static_cast<std::size_t>(r)] = f;
+            }
+          }
+          if (is_new_mech(mech, mechs, mech_tol)) {
+            mechs.push_back(mech);
+            ++count;
+          }
+        }
+        return count;
       }
 
       // The masses that weight the Hessians are the potential's (setfl species masses); `map` names types.
@@ -872,7 +1127,9 @@ namespace fly {
         return z ^ (z >> 31);
       }
 
-      // The atoms of a site: those within active_radius of the centre (minimum image), ascending.
+    public:
+      // The atoms of a site: those within active_radius of the centre (minimum image), ascending.  A Mech
+      // runs over them, and env::site_symmetries expresses a site's symmetries on them.
       std::vector<std::int64_t> site_atoms(std::int64_t centre,
                                            system::SoA<Position const&, TypeID const&, Frozen const&> const& in) const {
         std::vector<std::int64_t> out;
@@ -891,6 +1148,7 @@ namespace fly {
         return out;
       }
 
+    private:
       // Mechanism::kinetic_pre (find.cpp:838-871, 287): the spectra of the mass-weighted Hessians at the
       // initial minimum and at the saddle over the search's mobile atoms (those within active_radius).
       double prefactor(std::int64_t centre, system::SoA<Position const&, TypeID const&, Frozen const&> const& in,
