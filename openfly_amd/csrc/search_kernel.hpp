@@ -704,6 +704,102 @@ namespace flyg {
 
       // The scalars rho and alpha live in LDS; every lane of the team stores the same value.
       __device__ __forceinline__ double* newton_step(TM const& tm, double const* x, double const* g) {
+        if constexpr (std::is_same_v<TM, Wave>) {
+          return newton_step_fused(tm, x, g);
+        } else {
+          return newton_step_passes(tm, x, g);
+        }
+      }
+
+      // On one wavefront: the same recursion with its element passes fused.  A lane owns the same
+      // elements in every pass, so one pass can finish an update of q and start the next dot product
+      // with q's new elements: 2 bound + 1 passes instead of 4 bound + 8.  Every element takes the same
+      // expressions as in the separate passes (lin's v = 1 q; v += c u, dot's __dadd_rn over a * b in
+      // element order, the same 64-lane total), so the results are the same bits.
+      __device__ __forceinline__ double* newton_step_fused(Wave const& w, double const* x, double const* g) {
+        int const prev = k > 0 ? (k - 1) % m : 0;
+        int const incur = k <= m ? 0 : k - m;
+        int const bound = k <= m ? k : m;
+        double* const sp = s + static_cast<long>(prev) * n3;
+        double* const yp = y + static_cast<long>(prev) * n3;
+        // Pass 1: the newest (s, y) pair, the copies of x and g, s.y, y.y and the first loop's first dot.
+        int jd = bound > 0 ? (bound - 1 + incur) % m : 0;
+        double const* dv = s + static_cast<long>(jd) * n3;
+        double sy = 0, yy = 0, d = 0;
+        for (int i = w.lane; i < n3; i += 64) {
+          double const xi = x[i], gi = g[i];
+          if (k > 0) {
+            double vs = 1.0 * xi;
+            vs += -1.0 * px[i];
+            double vy = 1.0 * gi;
+            vy += -1.0 * pg[i];
+            sp[i] = vs;
+            yp[i] = vy;
+            sy = __dadd_rn(sy, vs * vy);
+            yy = __dadd_rn(yy, vy * vy);
+          }
+          px[i] = 1.0 * xi;
+          pg[i] = 1.0 * gi;
+          double const qi = 1.0 * gi;
+          q[i] = qi;
+          if (bound > 0) d = __dadd_rn(d, dv[i] * qi);
+        }
+        double yy_t = 0;
+        if (k > 0) {
+          rho[prev] = 1.0 / fabs(Wave::total(sy));
+          yy_t = Wave::total(yy);
+        }
+        w.sync();
+        if (bound == 0) {
+          ++k;
+          w.sync();
+          return q;
+        }
+        double dot = Wave::total(d);
+        // A pass: q += c u (when u), then q = sc q (when scaled), then the dot of q with dv (when dv).
+        auto pass = [&](double const* u, double c, bool scaled, double sc, double const* dvec) {
+          double acc = 0;
+          for (int i = w.lane; i < n3; i += 64) {
+            double qi = q[i];
+            if (u) {
+              double v = 1.0 * qi;
+              v += c * u[i];
+              qi = v;
+            }
+            if (scaled) qi = sc * qi;
+            q[i] = qi;
+            if (dvec) acc = __dadd_rn(acc, dvec[i] * qi);
+          }
+          return dvec ? Wave::total(acc) : 0.0;
+        };
+        // First loop, newest pair first: alpha_j = rho_j s_j.q, then q -= alpha_j y_j.
+        double const* pu = nullptr;
+        double pc = 0;
+        for (int i = bound - 1; i >= 0; --i) {
+          int const j = (i + incur) % m;
+          if (i < bound - 1) dot = pass(pu, pc, false, 0.0, s + static_cast<long>(j) * n3);
+          double const a = rho[j] * dot;
+          alpha[j] = a;
+          pu = y + static_cast<long>(j) * n3;
+          pc = -a;
+        }
+        // H0 scaling, then the second loop, oldest pair first: beta = rho_j y_j.q, q += (alpha_j - beta) s_j.
+        bool scaled = k > 0;
+        double const scale = k > 0 ? 1.0 / rho[prev] / yy_t : 1.0;
+        for (int i = 0; i < bound; ++i) {
+          int const j = (i + incur) % m;
+          double const beta = rho[j] * pass(pu, pc, scaled, scale, y + static_cast<long>(j) * n3);
+          scaled = false;
+          pu = s + static_cast<long>(j) * n3;
+          pc = alpha[j] - beta;
+        }
+        pass(pu, pc, false, 0.0, nullptr);
+        ++k;
+        w.sync();
+        return q;
+      }
+
+      __device__ __forceinline__ double* newton_step_passes(TM const& tm, double const* x, double const* g) {
         int prev = k > 0 ? (k - 1) % m : 0;
         if (k > 0) {
           double* sp = s + static_cast<long>(prev) * n3;
