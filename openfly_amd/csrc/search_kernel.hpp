@@ -716,7 +716,9 @@ namespace flyg {
       // with q's new elements: 2 bound + 1 passes instead of 4 bound + 8.  Every element takes the same
       // expressions as in the separate passes (lin's v = 1 q; v += c u, dot's __dadd_rn over a * b in
       // element order, the same 64-lane total), so the results are the same bits.
-      __device__ __forceinline__ double* newton_step_fused(Wave const& w, double const* x, double const* g) {
+      // fdot (optional): *fdot_out = dot(result, fdot), taken in the last pass.
+      __device__ __forceinline__ double* newton_step_fused(Wave const& w, double const* x, double const* g,
+                                                           double const* fdot = nullptr, double* fdot_out = nullptr) {
         int const prev = k > 0 ? (k - 1) % m : 0;
         int const incur = k <= m ? 0 : k - m;
         int const bound = k <= m ? k : m;
@@ -725,7 +727,7 @@ namespace flyg {
         // Pass 1: the newest (s, y) pair, the copies of x and g, s.y, y.y and the first loop's first dot.
         int jd = bound > 0 ? (bound - 1 + incur) % m : 0;
         double const* dv = s + static_cast<long>(jd) * n3;
-        double sy = 0, yy = 0, d = 0;
+        double sy = 0, yy = 0, d = 0, fd = 0;
         for (int i = w.lane; i < n3; i += 64) {
           double const xi = x[i], gi = g[i];
           if (k > 0) {
@@ -743,6 +745,7 @@ namespace flyg {
           double const qi = 1.0 * gi;
           q[i] = qi;
           if (bound > 0) d = __dadd_rn(d, dv[i] * qi);
+          if (bound == 0 && fdot) fd = __dadd_rn(fd, qi * fdot[i]);
         }
         double yy_t = 0;
         if (k > 0) {
@@ -751,6 +754,7 @@ namespace flyg {
         }
         w.sync();
         if (bound == 0) {
+          if (fdot) *fdot_out = Wave::total(fd);
           ++k;
           w.sync();
           return q;
@@ -793,7 +797,11 @@ namespace flyg {
           pu = s + static_cast<long>(j) * n3;
           pc = alpha[j] - beta;
         }
-        pass(pu, pc, false, 0.0, nullptr);
+        if (fdot) {
+          *fdot_out = pass(pu, pc, false, 0.0, fdot);
+        } else {
+          pass(pu, pc, false, 0.0, nullptr);
+        }
         ++k;
         w.sync();
         return q;
@@ -1059,16 +1067,47 @@ namespace flyg {
         double b_1 = 0, c_x0 = 0, theta_1 = 0;
         for (int i = 0;; ++i) {
           if (w0) {
-            wv.lin(dg, n3, g1, 1.0, g0, -1.0);
-            double t = wv.dot(dg, ax, n3);
+            // The passes of lin(dg = g1 - g0), dot(dg, ax), lin(dgp), newton_step, dot(theta, ax),
+            // lin(theta -= ot ax), dot(theta, theta), lin(theta *= 1 / |theta|) and dot2(dg, theta, dg, ax),
+            // fused where one pass can finish one step and start the next (same expressions per element, so
+            // the same bits); c0r is dot(dg, ax) again, which t already is.
+            double t;
+            {
+              double acc = 0;
+              for (int e = wv.lane; e < n3; e += 64) {
+                double v = 1.0 * g1[e];
+                v += -1.0 * g0[e];
+                dg[e] = v;
+                acc = __dadd_rn(acc, v * ax[e]);
+              }
+              t = Wave::total(acc);
+            }
             wv.lin(dgp, n3, dg, 1.0, ax, -t);
-            theta = rc.newton_step(wv, ax, dgp);
-            double ot = wv.dot(theta, ax, n3);
-            wv.lin(theta, n3, theta, 1.0, ax, -ot);
-            double nn = wv.dot(theta, theta, n3);
-            wv.lin(theta, n3, theta, 1.0 / sqrt(nn));
-            double b1r, c0r;
-            wv.dot2(dg, theta, dg, ax, n3, b1r, c0r);
+            double ot;
+            theta = rc.newton_step_fused(wv, ax, dgp, ax, &ot);
+            double nn;
+            {
+              double acc = 0;
+              for (int e = wv.lane; e < n3; e += 64) {
+                double v = 1.0 * theta[e];
+                v += -ot * ax[e];
+                theta[e] = v;
+                acc = __dadd_rn(acc, v * v);
+              }
+              nn = Wave::total(acc);
+            }
+            double b1r;
+            double const c0r = t;
+            {
+              double const cn = 1.0 / sqrt(nn);
+              double acc = 0;
+              for (int e = wv.lane; e < n3; e += 64) {
+                double const v = cn * theta[e];
+                theta[e] = v;
+                acc = __dadd_rn(acc, dg[e] * v);
+              }
+              b1r = Wave::total(acc);
+            }
             b_1 = b1r / dr;
             c_x0 = c0r / dr;
             theta_1 = -0.5 * atan(b_1 / fabs(c_x0));
@@ -1093,22 +1132,44 @@ namespace flyg {
           ++grads;
           if (err) return false;
           if (w0) {
-            wv.lin(dg, n3, g1p, 1.0, g0, -1.0);
-            double c_x1 = wv.dot(dg, axp, n3) / dr;
+            double c_x1;
+            {   // lin(dg = g1p - g0) and dot(dg, axp) in one pass
+              double acc = 0;
+              for (int e = wv.lane; e < n3; e += 64) {
+                double v = 1.0 * g1p[e];
+                v += -1.0 * g0[e];
+                dg[e] = v;
+                acc = __dadd_rn(acc, v * axp[e]);
+              }
+              c_x1 = Wave::total(acc) / dr;
+            }
             double a_1 = (c_x0 - c_x1 + b_1 * sin(2 * theta_1)) / (1 - cos(2 * theta_1));
             double theta_min = 0.5 * atan(b_1 / a_1);
             if (a_1 * cos(2 * theta_min) - a_1 + b_1 * sin(2 * theta_min) > 0) theta_min += M_PI / 2;
             double cm = cos(theta_min), sm = sin(theta_min);
-            wv.lin(ax, n3, ax, cm, theta, sm);
             double w1 = sin(theta_1 - theta_min) / sin(theta_1);
             double w2 = sin(theta_min) / sin(theta_1);
             double w3 = 1 - cos(theta_min) - sin(theta_min) * tan(0.5 * theta_1);
-            wv.lin(g1, n3, g1, w1, g1p, w2, g0, w3);
             bool const stop = fabs(theta_min) < o.theta_tol;
             double c = 0;
-            if (stop) {
-              wv.lin(dg, n3, g1, 1.0, g0, -1.0);
-              c = wv.dot(dg, ax, n3) / dr;
+            {   // lin(ax), lin(g1) and, when stopping, lin(dg = g1 - g0) with dot(dg, ax), in one pass
+              double acc = 0;
+              for (int e = wv.lane; e < n3; e += 64) {
+                double va = cm * ax[e];
+                va += sm * theta[e];
+                ax[e] = va;
+                double vg = w1 * g1[e];
+                vg += w2 * g1p[e];
+                vg += w3 * g0[e];
+                g1[e] = vg;
+                if (stop) {
+                  double vd = 1.0 * vg;
+                  vd += -1.0 * g0[e];
+                  dg[e] = vd;
+                  acc = __dadd_rn(acc, vd * va);
+                }
+              }
+              if (stop) c = Wave::total(acc) / dr;
             }
             if (wv.lane == 0) {
               bc[2] = stop ? 1.0 : 0.0;
