@@ -1184,13 +1184,24 @@ namespace flyg {
         }
         if (w0) {
           double p = wv.dot(g0, ax, n3);
-          if (!o.relax_in_convex && curv > 0) {
-            wv.lin(eff, n3, ax, -p);
-          } else {
-            wv.lin(eff, n3, g0, 1.0, ax, -2.0 * p);
+          // lin(eff), dot(eff, eff) and dot(eff, qproj) in one pass (the same expressions per element).
+          bool const only_ax = !o.relax_in_convex && curv > 0;
+          double const cax = only_ax ? -p : -2.0 * p;
+          double amg = 0, apj = 0;
+          for (int e = wv.lane; e < n3; e += 64) {
+            double vv;
+            if (only_ax) {
+              vv = cax * ax[e];
+            } else {
+              vv = 1.0 * g0[e];
+              vv += cax * ax[e];
+            }
+            eff[e] = vv;
+            amg = __dadd_rn(amg, vv * vv);
+            if (qproj) apj = __dadd_rn(apj, vv * qproj[e]);
           }
-          double mg = wv.dot(eff, eff, n3);
-          double pj = qproj ? wv.dot(eff, qproj, n3) : 0.0;
+          double mg = Wave::total(amg);
+          double pj = qproj ? Wave::total(apj) : 0.0;
           if (wv.lane == 0) {
             bc[4] = mg;
             bc[7] = pj;
@@ -1240,11 +1251,17 @@ namespace flyg {
                 }
               }
               if (!hit) {
-                q = tc.newton_step(wv, x, eff);
-                double nrm = sqrt(wv.dot(q, q, n3));
+                double qq;
+                q = tc.newton_step_fused(wv, x, eff, tc.q, &qq);   // dot(q, q) in its last pass
+                double nrm = sqrt(qq);
                 double scale = fmin(1.0, trust / nrm);
-                wv.lin(q, n3, q, scale);
-                wv.lin(x, n3, x, 1.0, q, -1.0);
+                for (int e = wv.lane; e < n3; e += 64) {   // lin(q *= scale) and lin(x -= q) in one pass
+                  double const vq = scale * q[e];
+                  q[e] = vq;
+                  double vx = 1.0 * x[e];
+                  vx += -1.0 * vq;
+                  x[e] = vx;
+                }
               }
               if (wv.lane == 0) bc[5] = hit ? 1.0 : 0.0;
             }
