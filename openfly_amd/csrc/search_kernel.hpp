@@ -878,28 +878,35 @@ namespace flyg {
       bool const w0 = tm.tid < 64;
       Wave const wv{tm.tid & 63};
       int const n3 = v.n3;
-      // Workspace: the vectors the rotor touches every iteration (10 named ones and the rotation core)
-      // come first and live in LDS when they fit; the translation core and xr stay in global memory.
+      // Workspace: the vectors the rotor touches every iteration come first, most used first (the
+      // named ones, then the rotation core's q, x_prev, g_prev, s and y histories), then eff and xb (once
+      // per step); as many of them as fit live in LDS, the rest and the translation core and xr in global
+      // memory.  Where a vector lives changes no result.
       double* W = A.ws + h.ws_off;
       int const hot = 13 + 2 * o.rn;
-      double* Wh = hot * n3 <= lds_ws<T>() ? lds_ws_buf : W;
-      auto vecp = [&](int k) { return (k < hot ? Wh : W) + static_cast<long>(k) * n3; };
+      int const r0v = 8;   // rotation core: q, x_prev, g_prev, then rn s and rn y vectors
+      // The cores address their s and y histories as blocks (s + j n3), so a block is in LDS whole or not
+      // at all: a count of LDS vectors that ends inside a block stops at the block's start.
+      int fit = n3 > 0 ? min(hot, lds_ws<T>() / n3) : hot;
+      if (fit > r0v + 3 && fit < r0v + 3 + o.rn) fit = r0v + 3;
+      if (fit > r0v + 3 + o.rn && fit < r0v + 3 + 2 * o.rn) fit = r0v + 3 + o.rn;
+      auto vecp = [&](int k) { return (k < fit ? lds_ws_buf : W) + static_cast<long>(k) * n3; };
       double* x = vecp(0);
       double* ax = vecp(1);
-      double* eff = vecp(2);
-      double* g0 = vecp(3);
-      double* g1 = vecp(4);
-      double* g1p = vecp(5);
-      double* dg = vecp(6);
-      double* dgp = vecp(7);
-      double* axp = vecp(8);
-      double* xb = vecp(9);
+      double* g0 = vecp(2);
+      double* g1 = vecp(3);
+      double* g1p = vecp(4);
+      double* dg = vecp(5);
+      double* dgp = vecp(6);
+      double* axp = vecp(7);
+      double* eff = vecp(r0v + 3 + 2 * o.rn);
+      double* xb = vecp(r0v + 4 + 2 * o.rn);
       double* xr = vecp(nvec(o.n, o.rn) - 1);   // positions at the last active-list refresh
-      int const r0v = 10, t0 = 13 + 2 * o.rn;
+      int const t0 = 13 + 2 * o.rn;
       Core<Wave> tc{vecp(t0), vecp(t0 + o.n), vecp(t0 + 2 * o.n), vecp(t0 + 2 * o.n + 1), vecp(t0 + 2 * o.n + 2),
                     lds_core, lds_core + 32, o.n, 0, n3};
-      Core<Wave> rc{vecp(r0v), vecp(r0v + o.rn), vecp(r0v + 2 * o.rn), vecp(r0v + 2 * o.rn + 1),
-                    vecp(r0v + 2 * o.rn + 2), lds_core + 64, lds_core + 96, o.rn, 0, n3};
+      Core<Wave> rc{vecp(r0v + 3), vecp(r0v + 3 + o.rn), vecp(r0v + 1), vecp(r0v + 2), vecp(r0v), lds_core + 64,
+                    lds_core + 96, o.rn, 0, n3};
       // The mechanism relaxations keep the workgroup-wide algebra (same memory as the translation core).
       Core<Team<T>> tcm{tc.s, tc.y, tc.px, tc.pg, tc.q, tc.rho, tc.alpha, o.n, 0, n3};
       double const dr = o.delta_r;
