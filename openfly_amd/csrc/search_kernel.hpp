@@ -848,7 +848,7 @@ namespace flyg {
 
     template <int T, int L, bool MIX, bool MULTI>
     __device__ __forceinline__ void run_search(BatchArgs const& A, int sid, Team<T> const& tm, double* lds_core, double3* lds_S,
-                               double4* lds_cur, double* lds_ws_buf, SplineSet const* lds_spl, unsigned* lds_err,
+                               double4* lds_area, SplineSet const* lds_spl, unsigned* lds_err,
                                double* bc) {
       long const t_begin = static_cast<long>(wall_clock64());
       SearchHdr const h = A.hdr[sid];
@@ -861,7 +861,13 @@ namespace flyg {
       v.spl = lds_spl;
       v.srho = h.srho;
       // Region positions (w = F' between the two passes of a gradient) live in LDS when they fit.
-      v.cur = h.nr <= lds_rows<T>() ? lds_cur : A.cur + h.reg_off;
+      bool const pos_lds = h.nr <= lds_rows<T>();
+      v.cur = pos_lds ? lds_area : A.cur + h.reg_off;
+      // The workspace part of the area starts after the positions; what rows a small region leaves
+      // unused goes to the workspace.
+      int const area = 4 * lds_rows<T>() + 4 * (lds_ws<T>() / 4);
+      double* const lds_ws_buf = reinterpret_cast<double*>(lds_area) + (pos_lds ? 4 * h.nr : 0);
+      int const ws_cap = area - (pos_lds ? 4 * h.nr : 0);
       v.base = A.base + h.reg_off;
       v.type = A.rtype + h.reg_off;
       v.gidx = A.rgidx + h.reg_off;
@@ -887,7 +893,7 @@ namespace flyg {
       int const r0v = 8;   // rotation core: q, x_prev, g_prev, then rn s and rn y vectors
       // The cores address their s and y histories as blocks (s + j n3), so a block is in LDS whole or not
       // at all: a count of LDS vectors that ends inside a block stops at the block's start.
-      int fit = n3 > 0 ? min(hot, lds_ws<T>() / n3) : hot;
+      int fit = n3 > 0 ? min(hot, ws_cap / n3) : hot;
       if (fit > r0v + 3 && fit < r0v + 3 + o.rn) fit = r0v + 3;
       if (fit > r0v + 3 + o.rn && fit < r0v + 3 + 2 * o.rn) fit = r0v + 3 + o.rn;
       auto vecp = [&](int k) { return (k < fit ? lds_ws_buf : W) + static_cast<long>(k) * n3; };
@@ -1532,8 +1538,8 @@ namespace flyg {
       __shared__ double lds_core[128];
       __shared__ int s_id;
       __shared__ double3 lds_S[27];
-      __shared__ double4 lds_cur[lds_rows<T>()];
-      __shared__ double lds_wsb[lds_ws<T>()];
+      // One LDS area for a search's region positions (when its rows fit) and, after them, its workspace.
+      __shared__ double4 lds_area[lds_rows<T>() + lds_ws<T>() / 4];
       __shared__ SplineSet lds_spl;
       __shared__ unsigned lds_err;
       __shared__ double lds_bc[8];
@@ -1554,7 +1560,7 @@ namespace flyg {
         int sid = s_id;
         __syncthreads();
         if (sid >= A.nsearch) break;
-        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_cur, lds_wsb, &lds_spl, &lds_err, lds_bc);
+        run_search<T, L, MIX, MULTI>(A, sid, tm, lds_core, lds_S, lds_area, &lds_spl, &lds_err, lds_bc);
       }
     }
 
