@@ -886,8 +886,8 @@ namespace flyg {
       int const n3 = v.n3;
       // Workspace: the vectors the rotor touches every iteration come first, most used first (the
       // named ones, then the rotation core's q, x_prev, g_prev, s and y histories), then eff and xb (once
-      // per step); as many of them as fit live in LDS, the rest and the translation core and xr in global
-      // memory.  Where a vector lives changes no result.
+      // per step); as many of them as fit live in LDS, the rest in global memory.  Where a vector lives
+      // changes no result.
       double* W = A.ws + h.ws_off;
       int const hot = 13 + 2 * o.rn;
       int const r0v = 8;   // rotation core: q, x_prev, g_prev, then rn s and rn y vectors
@@ -896,6 +896,16 @@ namespace flyg {
       int fit = n3 > 0 ? min(hot, ws_cap / n3) : hot;
       if (fit > r0v + 3 && fit < r0v + 3 + o.rn) fit = r0v + 3;
       if (fit > r0v + 3 + o.rn && fit < r0v + 3 + 2 * o.rn) fit = r0v + 3 + o.rn;
+      // Then, when all of those are in LDS, the translation core (its s, y, x_prev, g_prev and q as one
+      // block, which the saved state also addresses as a block) and xr, as far as they fit.
+      int const tend = hot + 2 * o.n + 3;   // xr
+      if (fit == hot && n3 > 0) {
+        if (static_cast<long>(tend + 1) * n3 <= ws_cap) {
+          fit = tend + 1;
+        } else if (static_cast<long>(tend) * n3 <= ws_cap) {
+          fit = tend;
+        }
+      }
       auto vecp = [&](int k) { return (k < fit ? lds_ws_buf : W) + static_cast<long>(k) * n3; };
       double* x = vecp(0);
       double* ax = vecp(1);
