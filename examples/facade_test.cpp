@@ -3,7 +3,10 @@
 // against the GPU facade and linked to libflygpu.so.  Exit status 0 on success.
 #include <cstdio>
 #include <memory>
+#include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "libfly/gpu/facade.hpp"
 
@@ -112,6 +115,32 @@ int main(int argc, char** argv) {
     std::printf("dimer exit %d steps %d grads %ld\n", static_cast<int>(exit), dimer.last_stats().steps,
                 static_cast<long>(dimer.last_stats().grads));
     REQUIRE(exit == saddle::Dimer::success);
+    // Options::fout (dimer.cpp:88-90): the same search writing its trajectory to a GSD file, one frame of
+    // centre positions per iteration, from the start to the saddle.
+    {
+      std::string const traj = "/tmp/facade_dimer_" + std::to_string(static_cast<long>(::getpid())) + ".gsd";
+      std::uint64_t frames = 0;
+      {
+        io::BinaryFile file(traj, io::create);
+        saddle::Dimer::Options dopt;
+        dopt.fout = &file;
+        saddle::Dimer traced(dopt, {}, cell.box);
+        system::SoA<Position, Axis> out2(cell.atoms.size());
+        REQUIRE(traced.find_sp(out2, in, in_min, pot, {}, 0.0, 0, 1) == exit);
+        frames = file.n_frames();
+        REQUIRE(frames == static_cast<std::uint64_t>(traced.last_stats().steps) + 1);
+        system::SoA<Position> first(cell.atoms.size()), last(cell.atoms.size());
+        file.read_to(0, r_, first);
+        file.read_to(frames - 1, r_, last);
+        bool same = true;
+        for (std::size_t k = 0; k < first[r_].size(); ++k) {
+          same = same && first[r_][k] == in[r_][k] && last[r_][k] == out2[r_][k] && out2[r_][k] == out[r_][k];
+        }
+        REQUIRE(same);
+      }
+      std::remove(traj.c_str());
+      std::printf("dimer trajectory (Options::fout): %llu frames\n", static_cast<unsigned long long>(frames));
+    }
     neigh::List nl(cell.box, pot.r_cut());
     system::SoA<Position, TypeID, Frozen> sp(cell.atoms.size());
     for (std::int64_t i = 0; i < cell.atoms.size(); ++i) {

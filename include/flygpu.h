@@ -31,6 +31,7 @@ extern "C" {
 #define FLYG_SPLINE_OOB 2   /* reference: fly::Spline::OutOfBounds (spline.hpp:46-51, thrown at :141-143) */
 #define FLYG_DEVICE_ERROR 3 /* HIP runtime failure */
 #define FLYG_BUSY 4         /* serving: the device memory budget cannot take this now; retry after searches finish */
+#define FLYG_STOPPED 5      /* a step observer (flyg_*_set_observer) asked the run to stop */
 
 /* Pointer location for `where` arguments. */
 #define FLYG_HOST 0
@@ -200,6 +201,14 @@ int flyg_lbfgs_destroy(flyg_lbfgs m);
  * reference returns true on failure, lbfgs.cpp:114).  pos is in/out, grad out. */
 int flyg_lbfgs_minimise(flyg_lbfgs m, flyg_eam eam, double* pos, double* grad, const uint32_t* types,
                         const uint8_t* frozen, int64_t n, int where, int* failed, int* iterations);
+/* Step observer, the reference's Options::fout trajectory dump (LBFGS: lbfgs.cpp:75-77; Dimer:
+ * dimer.cpp:88-90): called on the calling thread at the start of every iteration of flyg_lbfgs_minimise /
+ * flyg_dimer_find_sp, before the convergence test, with the iteration number and the current positions
+ * in host memory (3n doubles, valid during the call).  A run that converges at iteration k calls it k + 1
+ * times.  A nonzero return stops the run, which then returns FLYG_STOPPED.  Costs one device-to-host copy
+ * of the positions per iteration; fn = NULL removes the observer. */
+typedef int (*flyg_step_observer)(void* user, int iteration, const double* xyz, int64_t n);
+int flyg_lbfgs_set_observer(flyg_lbfgs m, flyg_step_observer fn, void* user);
 
 /* ---------------------------------------------------------------------------------------------------
  * Dimer saddle search: replaces saddle::Rotor + saddle::Dimer (rotor.hpp:42-139, dimer.hpp:51-170,
@@ -271,6 +280,8 @@ int flyg_dimer_destroy(flyg_dimer d);
 int flyg_dimer_find_sp(flyg_dimer d, flyg_eam eam, double* pos, double* axis, const uint32_t* types,
                        const uint8_t* frozen, int64_t n, const double* in_min, const double* hist_sp, int n_hist,
                        double theta_tol, int count_frozen, int where, flyg_search_stats* stats);
+/* Step observer of flyg_dimer_find_sp (see flyg_lbfgs_set_observer): the dimer's centre positions. */
+int flyg_dimer_set_observer(flyg_dimer d, flyg_step_observer fn, void* user);
 
 /* ---------------------------------------------------------------------------------------------------
  * Batched saddle searches: the device-side counterpart of Master::find_batch's task-per-search loop

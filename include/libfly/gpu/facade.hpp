@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <exception>
 #include <fstream>
 #include <iterator>
 #include <memory>
@@ -128,14 +129,35 @@ namespace fly {
     };
   }  // namespace system
 
-  struct Position : system::Property<double, 3> {};
-  struct Axis : system::Property<double, 3> {};
-  struct PotentialGradient : system::Property<double, 3> {};
-  struct Delta : system::Property<double, 3> {};
-  struct TypeID : system::Property<std::uint32_t> {};
-  struct Frozen : system::Property<bool> {};
-  struct Index : system::Property<std::int64_t> {};
-  struct Mass : system::Property<double> {};
+  // Each property carries its GSD chunk label (property.hpp:103-230), used by io::BinaryFile.
+  struct Position : system::Property<double, 3> {
+    static constexpr char const* tag = "particles/position";
+  };
+  struct Axis : system::Property<double, 3> {
+    static constexpr char const* tag = "log/particles/axis";
+  };
+  struct PotentialGradient : system::Property<double, 3> {
+    static constexpr char const* tag = "log/particles/potential_gradient";
+  };
+  struct Delta : system::Property<double, 3> {
+    static constexpr char const* tag = "log/particles/delta";
+  };
+  struct TypeID : system::Property<std::uint32_t> {
+    static constexpr char const* tag = "particles/typeid";
+  };
+  struct Frozen : system::Property<bool> {
+    static constexpr char const* tag = "log/particles/frozen";
+  };
+  struct Index : system::Property<std::int64_t> {
+    static constexpr char const* tag = "log/particles/index";
+  };
+  struct Mass : system::Property<double> {
+    static constexpr char const* tag = "particles/mass";
+  };
+  // A species name: 32 characters, NUL padded (property.hpp:103-105).
+  struct Type : system::Property<char, 32> {
+    static constexpr char const* tag = "particles/types";
+  };
 
   inline constexpr Position r_{};
   inline constexpr Axis ax_{};
@@ -250,17 +272,48 @@ namespace fly {
       }
     };
 
-    // ---- TypeMap (typemap.hpp:67-214), names only ------------------------------------------------
+    // ---- TypeMap (typemap.hpp:67-214): a name per species and the values of the per-species
+    // properties Extra... (scalar properties only, e.g. TypeMap<Mass>).
     template <typename... Extra>
     class TypeMap {
+      static_assert(((Extra::size() == 1) && ...), "TypeMap holds scalar per-species properties");
+
     public:
-      explicit TypeMap(int n = 0) : m_names(static_cast<std::size_t>(n)) {}
+      explicit TypeMap(int n = 0) : m_names(static_cast<std::size_t>(n)), m_extra(n) {}
       int num_types() const { return static_cast<int>(m_names.size()); }
-      void set(int i, std::string name) { m_names.at(static_cast<std::size_t>(i)) = std::move(name); }
+
+      // Name and the Extra... values of species i (typemap.hpp:160-200); names must be shorter than
+      // Type::size() characters, as the reference's fixed-width name storage requires.
+      void set(int i, std::string name, typename Extra::scalar_t const&... values) {
+        set_name(i, std::move(name));
+        ((m_extra(Extra{}, i) = values), ...);
+      }
+      void set_name(int i, std::string name) {
+        check(i);
+        if (name.size() >= static_cast<std::size_t>(Type::size())) {
+          throw RuntimeError("Type name length " + std::to_string(name.size()) + " but must be < " +
+                             std::to_string(Type::size()));
+        }
+        m_names[static_cast<std::size_t>(i)] = std::move(name);
+      }
       std::string const& get(int i) const { return m_names.at(static_cast<std::size_t>(i)); }
+      std::string const& get(int i, Type) const { return get(i); }
+      template <typename Tag, typename = std::enable_if_t<!std::is_same_v<Tag, Type>>>
+      typename Tag::scalar_t get(int i, Tag) const {
+        check(i);
+        return m_extra(Tag{}, i);
+      }
+
+      // The per-species values as one SoA of num_types() rows (for io::BinaryFile).
+      SoA<Extra...> const& values() const { return m_extra; }
+      SoA<Extra...>& values() { return m_extra; }
 
     private:
+      void check(int i) const {
+        if (i < 0 || i >= num_types()) throw RuntimeError("TypeMap: species " + std::to_string(i) + " out of range");
+      }
       std::vector<std::string> m_names;
+      SoA<Extra...> m_extra;
     };
 
     // ---- Box (box.hpp:103-120) -------------------------------------------------------------------
@@ -306,6 +359,37 @@ namespace fly {
     };
 
   }  // namespace system
+}  // namespace fly
+
+// fly::io::BinaryFile (GSD files) reads and writes the system types above; the Options::fout hooks of the
+// minimiser and dimer below write frames through it.
+#define FLY_GPU_FACADE_SYSTEM_TYPES 1
+#include "gsd_file.hpp"
+
+namespace fly {
+
+  namespace gpu {
+    // Options::fout of the minimiser and dimer (lbfgs.cpp:75-77, dimer.cpp:88-90): the positions at the
+    // start of every iteration committed as one frame; an exception from the file stops the run and is
+    // re-thrown by the call.
+    struct FrameDump {
+      io::BinaryFile* file;
+      std::exception_ptr error = nullptr;
+
+      static int call(void* user, int, double const* xyz, std::int64_t n) {
+        auto* d = static_cast<FrameDump*>(user);
+        try {
+          system::SoA<Position> s(n);
+          std::copy(xyz, xyz + 3 * n, s[r_].data());
+          d->file->commit([&] { d->file->write(r_, s); });
+          return 0;
+        } catch (...) {
+          d->error = std::current_exception();
+          return 1;
+        }
+      }
+    };
+  }  // namespace gpu
 
   // ---- neigh::List (list.hpp:71-210) -------------------------------------------------------------
   namespace neigh {
@@ -538,6 +622,7 @@ namespace fly {
         double grow_trust = 1.5;
         double shrink_trust = 0.5;
         bool debug = false;
+        io::BinaryFile* fout = nullptr;   // positions at every iteration, one frame each (lbfgs.hpp:80-87)
       };
 
       LBFGS(Options const& opt, system::Box const& box) : m_ctx(gpu::context()), m_opt(opt) {
@@ -557,9 +642,13 @@ namespace fly {
         auto i = in[r_];
         for (std::size_t k = 0; k < o.size(); ++k) o[k] = i[k];
         int failed = 0, it = 0;
-        gpu::check(flyg_lbfgs_minimise(m_h, pot.handle(), o.data(), out[g_].data(), in[id_].data(), in[fzn_].data(),
-                                       in.size(), FLYG_HOST, &failed, &it),
-                   m_ctx);
+        gpu::FrameDump dump{m_opt.fout};
+        gpu::check(flyg_lbfgs_set_observer(m_h, m_opt.fout ? &gpu::FrameDump::call : nullptr, &dump), m_ctx);
+        int const rc = flyg_lbfgs_minimise(m_h, pot.handle(), o.data(), out[g_].data(), in[id_].data(), in[fzn_].data(),
+                                           in.size(), FLYG_HOST, &failed, &it);
+        (void)flyg_lbfgs_set_observer(m_h, nullptr, nullptr);
+        if (dump.error) std::rethrow_exception(dump.error);
+        gpu::check(rc, m_ctx);
         return failed != 0;
       }
 
@@ -847,6 +936,7 @@ namespace fly {
         int convex_max = 3;
         bool use_history = true;
         bool debug = false;
+        io::BinaryFile* fout = nullptr;   // centre positions at every iteration, one frame each (dimer.hpp:90-97)
       };
 
       // The reference's values (dimer.hpp:123-129); the C-ABI reports them without the sign.
@@ -883,10 +973,14 @@ namespace fly {
           hist.insert(hist.end(), v.data(), v.data() + v.size());
         }
         flyg_search_stats st{};
-        gpu::check(flyg_dimer_find_sp(m_h, pot.handle(), ro.data(), ao.data(), in[id_].data(), in[fzn_].data(), n,
-                                      in_min[r_].data(), hist.empty() ? nullptr : hist.data(),
-                                      static_cast<int>(hist_sp.size()), theta_tol, count_frozen, FLYG_HOST, &st),
-                   m_ctx);
+        gpu::FrameDump dump{m_opt.fout};
+        gpu::check(flyg_dimer_set_observer(m_h, m_opt.fout ? &gpu::FrameDump::call : nullptr, &dump), m_ctx);
+        int const rc = flyg_dimer_find_sp(m_h, pot.handle(), ro.data(), ao.data(), in[id_].data(), in[fzn_].data(), n,
+                                          in_min[r_].data(), hist.empty() ? nullptr : hist.data(),
+                                          static_cast<int>(hist_sp.size()), theta_tol, count_frozen, FLYG_HOST, &st);
+        (void)flyg_dimer_set_observer(m_h, nullptr, nullptr);
+        if (dump.error) std::rethrow_exception(dump.error);
+        gpu::check(rc, m_ctx);
         m_last = st;
         return static_cast<Exit>(-st.exit_code);
       }

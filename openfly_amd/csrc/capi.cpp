@@ -528,6 +528,13 @@ int flyg_lbfgs_minimise(flyg_lbfgs m, flyg_eam eam, double* pos, double* grad, c
   });
 }
 
+int flyg_lbfgs_set_observer(flyg_lbfgs m, flyg_step_observer fn, void* user) {
+  return guard(m->ctx, [&] {
+    m->obs = fn;
+    m->obs_user = user;
+  });
+}
+
 // ---------------------------------------------------------------------------------------------------
 // Dimer
 // ---------------------------------------------------------------------------------------------------
@@ -675,6 +682,13 @@ int flyg_dimer_find_sp(flyg_dimer d, flyg_eam eam, double* pos, double* axis, co
     }
     check_errors(ctx);
     if (stats) *stats = st;
+  });
+}
+
+int flyg_dimer_set_observer(flyg_dimer d, flyg_step_observer fn, void* user) {
+  return guard(d->ctx, [&] {
+    d->obs = fn;
+    d->obs_user = user;
   });
 }
 

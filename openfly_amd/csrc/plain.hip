@@ -23,6 +23,20 @@ namespace flyg {
       HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx.stream));
     }
 
+    // Options::fout of the reference (lbfgs.cpp:75-77, dimer.cpp:88-90): the positions at the start of
+    // iteration `it`, copied to the host, go to the caller's observer.
+    void observe(flyg_ctx_s& ctx, flyg_step_observer fn, void* user, int it, double const* d_x, long n,
+                 std::vector<double>& host) {
+      if (!fn) return;
+      host.resize(static_cast<std::size_t>(3 * n));
+      HIPCHK(hipMemcpyAsync(host.data(), d_x, 3 * n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+      HIPCHK(hipStreamSynchronize(ctx.stream));
+      if (int const rc = fn(user, it, host.data(), n)) {
+        throw Error(FLYG_STOPPED, "the step observer stopped the run at iteration " + std::to_string(it) +
+                                      " (returned " + std::to_string(rc) + ")");
+      }
+    }
+
     VecArgs lin(long n, double* out, double const* a, Coef ca, double const* b = nullptr, Coef cb = {},
                 double const* c = nullptr, Coef cc = {}) {
       VecArgs v;
@@ -189,6 +203,7 @@ namespace flyg {
     double trust = o.min_trust, acc = 0;
     for (int i = 0; i < o.iter_max; ++i) {
       if (iterations) *iterations = i;
+      observe(ctx, m.obs, m.obs_user, i, x, n, m.obs_host);
       if (mag_g < o.f2norm * o.f2norm) return false;
       double* q = m.core->newton_step(x, g, N);
       vec(ctx, dots(N, sl, {{q, q}}));
@@ -401,6 +416,7 @@ namespace flyg {
     double trust = o.min_trust, acc = 0, convex_count = 0;
     int ret = FLYG_EXIT_ITER_MAX, steps = o.max_steps;
     for (int i = 0; i < o.max_steps; ++i) {
+      observe(ctx, d.obs, d.obs_user, i, x, n, d.obs_host);
       if (mag_g < o.f2norm * o.f2norm) {
         ret = FLYG_EXIT_SUCCESS;
         steps = i;

@@ -57,6 +57,9 @@ struct flyg_lbfgs_s {
   std::unique_ptr<flyg::DevLBFGS> core;
   flyg::DVec x, g, tmp;
   flyg::DBuf<double> slots;
+  flyg_step_observer obs = nullptr;   // per-iteration positions to the caller (Options::fout)
+  void* obs_user = nullptr;
+  std::vector<double> obs_host;
 };
 
 struct flyg_dimer_s {
@@ -72,6 +75,9 @@ struct flyg_dimer_s {
   std::vector<flyg::DVec> hist_d;   // sp_k - in_min
   flyg::DBuf<double> slots;
   long grads = 0;
+  flyg_step_observer obs = nullptr;
+  void* obs_user = nullptr;
+  std::vector<double> obs_host;
 };
 
 // Rotor (rotor.hpp:42-139): the plain form's rotor state (the dimer's, without its own list).

@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libflygpu.so")
 
-FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR, FLYG_BUSY = 0, 1, 2, 3, 4
+FLYG_OK, FLYG_INVALID_ARG, FLYG_SPLINE_OOB, FLYG_DEVICE_ERROR, FLYG_BUSY, FLYG_STOPPED = 0, 1, 2, 3, 4, 5
 HOST, DEVICE = 0, 1
 FP64, MIXED = 0, 1
 EXIT_NAMES = {0: "success", 1: "convex", 2: "iter_max", 3: "collision", 4: "uninit", 5: "region", 6: "spline OOB"}
@@ -45,12 +45,18 @@ class Busy(FlyError):
     """Serving: the device memory budget cannot take this now (retry after searches finish)."""
 
 
+class Stopped(FlyError):
+    """A step observer asked the run to stop."""
+
+
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)
 _u32 = C.POINTER(C.c_uint32)
 _u8 = C.POINTER(C.c_uint8)
 _vp = C.c_void_p
+# int fn(void* user, int iteration, const double* xyz, int64_t n) (flyg_step_observer)
+STEP_OBSERVER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int64)
 _lib = None
 
 
@@ -162,6 +168,8 @@ _SIGS = {
     "flyg_lbfgs_destroy": [_vp],
     "flyg_lbfgs_minimise": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_int, C.POINTER(C.c_int),
                             C.POINTER(C.c_int)],
+    "flyg_lbfgs_set_observer": [_vp, STEP_OBSERVER, _vp],
+    "flyg_dimer_set_observer": [_vp, STEP_OBSERVER, _vp],
     "flyg_dimer_default_options": [C.POINTER(DimerOptions)],
     "flyg_rotor_default_options": [C.POINTER(RotorOptions)],
     "flyg_dimer_create": [_vp, C.POINTER(DimerOptions), C.POINTER(RotorOptions), _d, C.POINTER(C.c_int),
@@ -244,7 +252,33 @@ def _raise(ctx_h, rc):
         raise DeviceError(msg)
     if rc == FLYG_BUSY:
         raise Busy(msg)
+    if rc == FLYG_STOPPED:
+        raise Stopped(msg)
     raise FlyError(msg)
+
+
+class _Observer:
+    """A Python callable fn(iteration, positions (n, 3) copy) as a step observer (the reference's
+    Options::fout hook): a true return value stops the run; an exception stops it and is re-raised by
+    the call that ran it."""
+
+    def __init__(self, fn):
+        self.fn = fn
+        self.error = None
+
+        def cb(_user, it, xyz, n):
+            try:
+                return 1 if fn(int(it), np.ctypeslib.as_array(xyz, shape=(int(n), 3)).copy()) else 0
+            except BaseException as e:   # noqa: BLE001 - carried to the caller, never across C frames
+                self.error = e
+                return 1
+
+        self.c = STEP_OBSERVER(cb)
+
+    def reraise(self):
+        if self.error is not None:
+            e, self.error = self.error, None
+            raise e
 
 
 def _is_torch(a):
@@ -588,10 +622,21 @@ class LBFGS:
         f = np.ascontiguousarray(frozen, dtype=np.uint8)
         failed = C.c_int()
         it = C.c_int()
-        self.ctx.check(lib().flyg_lbfgs_minimise(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(grad.ctypes.data),
-                                                 C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n, HOST,
-                                                 C.byref(failed), C.byref(it)))
+        rc = lib().flyg_lbfgs_minimise(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(grad.ctypes.data),
+                                       C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n, HOST,
+                                       C.byref(failed), C.byref(it))
+        if self._obs is not None:
+            self._obs.reraise()
+        self.ctx.check(rc)
         return pos, grad, bool(failed.value), it.value
+
+    _obs = None
+
+    def set_observer(self, fn):
+        """fn(iteration, positions) at the start of every iteration (LBFGS::Options::fout, lbfgs.cpp:75-77);
+        None removes it."""
+        self._obs = _Observer(fn) if fn is not None else None
+        self.ctx.check(lib().flyg_lbfgs_set_observer(self.h, self._obs.c if self._obs else STEP_OBSERVER(), None))
 
 
 class Rotor:
@@ -660,13 +705,24 @@ class Dimer:
         if count_frozen is None:
             count_frozen = int(f.sum())
         st = SearchStats()
-        self.ctx.check(lib().flyg_dimer_find_sp(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(ax.ctypes.data),
-                                                C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n,
-                                                C.c_void_p(mn.ctypes.data), C.c_void_p(hs.ctypes.data), len(hist_sp),
-                                                float(theta_tol), int(count_frozen), HOST, C.byref(st)))
+        rc = lib().flyg_dimer_find_sp(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(ax.ctypes.data),
+                                      C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n,
+                                      C.c_void_p(mn.ctypes.data), C.c_void_p(hs.ctypes.data), len(hist_sp),
+                                      float(theta_tol), int(count_frozen), HOST, C.byref(st))
+        if self._obs is not None:
+            self._obs.reraise()
+        self.ctx.check(rc)
         out = st.as_dict()
         out.update(pos=pos, axis=ax, exit=st.exit_code)
         return out
+
+    _obs = None
+
+    def set_observer(self, fn):
+        """fn(iteration, centre positions) at the start of every iteration (Dimer::Options::fout,
+        dimer.cpp:88-90); None removes it."""
+        self._obs = _Observer(fn) if fn is not None else None
+        self.ctx.check(lib().flyg_dimer_set_observer(self.h, self._obs.c if self._obs else STEP_OBSERVER(), None))
 
 
 class Batch:
