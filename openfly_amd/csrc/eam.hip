@@ -17,6 +17,7 @@
 //      pass 2: (Hv)_g = -sum_a [dm u + m (dv - u s)/r],  s = u.dv,  dv = v_a - v_g, u = (r_a - r_g)/r,
 //              dm = (V'' + F'_g phi''_{a->g} + F'_a phi''_{g->a}) s + dF'_g phi'_{a->g} + dF'_a phi'_{g->a}
 //      which is H v for the analytic Hessian of eam.cpp:122-233 (pinned against it in the tests).
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -144,7 +145,7 @@ namespace flyg {
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, A.pos, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
-            int tj = MULTI ? checked_type(A.types[p.j], T.ntypes, e) : A.t0;
+            int tj = MULTI ? checked_type(A.type_in_w ? static_cast<uint32_t>(p.w) : A.types[p.j], T.ntypes, e) : A.t0;
             if (MIX && !ENERGY) {
               float r2f = static_cast<float>(p.r2);
               float rf = r2f * __frsqrt_rn(r2f);
@@ -416,16 +417,23 @@ namespace flyg {
     }
 
     __global__ void k_types_by_slot(uint32_t const* __restrict__ types, int const* __restrict__ order, long n,
-                                    uint32_t* __restrict__ out) {
+                                    uint32_t* __restrict__ out, double4* __restrict__ gpos) {
       long k = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
-      if (k < n) out[k] = types[order[k]];
+      if (k < n) {
+        uint32_t const t = types[order[k]];
+        out[k] = t;
+        gpos[k].w = static_cast<double>(t);
+      }
     }
 
     // The gradient passes run in the list's cell-order layout: a workgroup's atoms are neighbours in
     // space and so are the rows they gather, which raises the L1/L2 hit rate of the gathers.  Rows hold
     // the same entries in the same order, so the sums (and the bits) are those of the atom layout.  With
-    // several species the types are gathered into slot order: once per list build for staged host types
-    // (the library sees every change of them), on every call for device types.
+    // several species the types are gathered into slot order, and into the w slot of the slot-ordered
+    // positions (the density pass reads a neighbour's species from there): once per list build for staged
+    // host types (the library sees every change of them), on every call for device types.  Type stamps
+    // are unique across handles, so a list shared by several potentials or type arrays is re-gathered
+    // whenever another one wrote its w slots last.
     PairArgs slot_args(flyg_eam_s& E, flyg_list_s& nl, uint32_t const* types, uint8_t const* frozen) {
       PairArgs A = pair_args(E, nl, types, frozen);
       A.pos = nl.gpos.get();
@@ -434,18 +442,24 @@ namespace flyg {
       A.order = nl.order.get();
       if (A.t0 < 0) {
         bool const staged = types == E.types.get();
-        if (!(staged && E.gtype_build == nl.build_id && E.gtype_stamp == E.types_stamp)) {
+        bool const cached = staged && E.gtype_build == nl.build_id && E.gtype_stamp == E.types_stamp &&
+                            nl.wtype_build == nl.build_id && nl.wtype_stamp == E.types_stamp;
+        if (!cached) {
           E.gtype_build = -1;   // the buffer may be reallocated: its contents are no longer known
+          nl.wtype_build = -1;
           E.gtype.reserve(nl.n);
           k_types_by_slot<<<static_cast<unsigned>((nl.n + 255) / 256), 256, 0, E.ctx->stream>>>(
-              types, nl.order.get(), nl.n, E.gtype.get());
+              types, nl.order.get(), nl.n, E.gtype.get(), nl.gpos.get());
           HIPCHK(hipGetLastError());
           if (staged) {
             E.gtype_build = nl.build_id;
             E.gtype_stamp = E.types_stamp;
+            nl.wtype_build = nl.build_id;
+            nl.wtype_stamp = E.types_stamp;
           }
         }
         A.types = E.gtype.get();
+        A.type_in_w = true;
       }
       return A;
     }
@@ -732,6 +746,14 @@ namespace flyg {
     if (fl & ERR_TYPE) throw Error(FLYG_INVALID_ARG, "TypeID out of range");
   }
 
+  namespace {
+    // Names one upload of staged host types, unique in the process (lists cache per-slot species by it).
+    long next_types_stamp() {
+      static std::atomic<long> counter{0};
+      return ++counter;
+    }
+  }  // namespace
+
   void stage_atoms(flyg_eam_s& E, uint32_t const* types, uint8_t const* frozen, long n, int where,
                    uint32_t const** d_types, uint8_t const** d_frozen) {
     if (where == FLYG_DEVICE) {
@@ -764,7 +786,7 @@ namespace flyg {
     E.frozen.reserve(n);
     if (!same_t) {
       HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyHostToDevice, E.ctx->stream));
-      ++E.types_stamp;
+      E.types_stamp = next_types_stamp();
       E.staged_single = single;
       E.last_types.assign(types, types + n);
     }

@@ -62,7 +62,7 @@ struct flyg_eam_s {
   flyg::DBuf<double> eatom;     // per-atom energies
   flyg::DBuf<uint32_t> types;   // staged host inputs
   flyg::DBuf<uint32_t> gtype;   // types in the list's slot order (gradients with several species)
-  long types_stamp = 0;         // bumped whenever the staged host types change
+  long types_stamp = 0;         // renewed (process-wide unique) whenever the staged host types change
   long gtype_build = -1, gtype_stamp = -1;   // gtype holds the staged types in the layout of this build
   flyg::DBuf<uint8_t> frozen;
   flyg::DBuf<double> vec_in, vec_out;
@@ -108,7 +108,11 @@ struct flyg_list_s {
   // order[k], rank[i] is atom i's slot), their positions by slot (kept current by update), and the rows
   // by slot with neighbour indices given as slots (the same entries in the same order as nbr).
   flyg::DBuf<int> order, rank;
+  // gpos's w slot holds the species of each slot for a multi-species gradient (written with the slot-order
+  // types; update keeps it): those of the staged host types with stamp wtype_stamp, if wtype_build is the
+  // current build_id, else of the last gradient call's types.
   flyg::DBuf<double4> gpos;
+  long wtype_stamp = -1, wtype_build = -1;
   flyg::DBuf<unsigned> grow;
   flyg::DBuf<int> gcnt;
   flyg::DBuf<int> scan_tmp;
@@ -143,6 +147,9 @@ namespace flyg {
     // Null: rows, positions and types are by atom.  Otherwise they are by slot of the list's cell-order
     // layout (types gathered into slot order) and order[slot] is the atom, for frozen flags and gradients.
     int const* order = nullptr;
+    // The w slot of pos holds each atom's species (as a double), so a neighbour's species arrives with
+    // its position; set for the slot layout with several species.
+    bool type_in_w = false;
   };
   void eam_setup_tables(flyg_eam_s& e);
   double eam_energy(flyg_eam_s& e, flyg_list_s& nl, uint32_t const* d_types, uint8_t const* d_frozen);

@@ -109,7 +109,10 @@ namespace flyg {
       p.y -= d[3 * i + 1];
       p.z -= d[3 * i + 2];
       pos[i] = p;
-      gpos[rank[i]] = p;
+      double4* g = gpos + rank[i];   // its w slot may hold the atom's species: kept
+      g->x = p.x;
+      g->y = p.y;
+      g->z = p.z;
     }
 
     __global__ void k_positions(double4 const* __restrict__ pos, long n, double* __restrict__ out) {

@@ -330,9 +330,10 @@ def test_device_arrays_checked(ctx, wen_gpu):
 
 
 def test_slot_layout_follows_types_positions_and_rebuilds(ctx, wen_gpu, wen_orc):
-    # The gradient passes run in the list's cell-order layout with the types gathered into it (cached per
-    # list build for staged host types).  Every change the cache must see: new host types on the same
-    # build, an update (positions by slot), a rebuild, device types in between, and two lists in turn.
+    # The gradient passes run in the list's cell-order layout with the types gathered into it and into the
+    # w slot of its positions (cached per list build for staged host types).  Every change the cache must
+    # see: new host types on the same build, an update (positions by slot, species kept), a rebuild,
+    # device types in between, two lists in turn, and two potential handles on one list.
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(41)
     pos, basis = H.kat_lattice()
@@ -375,3 +376,8 @@ def test_slot_layout_follows_types_positions_and_rebuilds(ctx, wen_gpu, wen_orc)
     nl_b.rebuild(pos)
     for lst, p in ((nl_b, pos), (nl, other), (nl_b, pos)):
         check(wen_gpu.gradient(t2, f, lst), p, t2)
+    # Two potential handles with their own staged types on one list: the list's per-slot species copy
+    # was last written by the other handle on every call.
+    wen_b = fa.EAM(ctx, H.WEN)
+    for pot_, t_ in ((wen_gpu, t1), (wen_b, t2), (wen_gpu, t1), (wen_b, t2)):
+        check(pot_.gradient(t_, f, nl), other, t_)
