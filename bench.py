@@ -463,11 +463,15 @@ def large_bench(ctx, fa, systems, n_search=1024):
             "first_barrier_fp64_maxdisp": [[round(a, 6), round(b, 6), round(c, 3)] for a, b, c in pairs[:5]]}
 
 
-def gradient_leg(ctx, fa, torch, s, path, precision, steps, warmup, skin, pool_n=16):
+def gradient_leg(ctx, fa, torch, s, path, precision, steps, warmup, skin, pool_n=16, device_types=False):
     """The incremental list + EAM gradient loop on system `s`: each step moves every atom by a random vector
     of norm <= 0.05 A (pre-generated, resident in HBM), updates the list (a rebuild once the accumulated
     displacement exceeds skin / 2) and evaluates the gradient.  Returns the wall time of the timed steps,
-    the kernel time per gradient (the library's HIP event pair on its stream) and the algorithmic bytes."""
+    the kernel time per gradient (the library's HIP event pair on its stream) and the algorithmic bytes.
+    Species and frozen flags are staged in the potential handle once (flyg_eam_stage_atoms, from HBM) and
+    every call uses that copy, which the library gathers into the list's layout once per list build.
+    device_types=True passes the device tensors to every call instead: their contents may change between
+    calls, so each call gathers the species again."""
     pot = fa.EAM(ctx, path)
     pot.set_precision(fa.MIXED if precision == "mixed" else fa.FP64)
     dev = torch.device("cuda", 0)
@@ -475,6 +479,9 @@ def gradient_leg(ctx, fa, torch, s, path, precision, steps, warmup, skin, pool_n
     pos = torch.from_numpy(s.pos.copy()).to(dev)
     types = torch.from_numpy(s.types.astype(np.uint32)).to(dev)
     frozen = torch.from_numpy(s.frozen).to(dev)
+    if not device_types:
+        pot.stage(types, frozen)
+        types = frozen = None
     grad = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
     scratch = torch.zeros((s.n, 3), dtype=torch.float64, device=dev)
     rng = np.random.default_rng(12)
@@ -620,7 +627,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"bcc Fe {args.cells}^3 cells + 1% vacancies + 2% X tetrahedral interstitials, "
-                        "random displacement <= 0.05 A per step, list update/rebuild, EAM gradient",
+                        "random displacement <= 0.05 A per step, list update/rebuild, EAM gradient "
+                        "(species and frozen flags staged in the potential once, from HBM)",
             "atoms": s.n,
             "mean_neighbours": float(counts.mean()),
             "rebuilds_in_timed_steps": g["rebuilds"],

@@ -129,8 +129,17 @@ int flyg_list_positions(flyg_list nl, double* pos, int where);
 
 /* ---------------------------------------------------------------------------------------------------
  * EAM evaluation: replaces potential::EAM / potential::Generic energy, gradient, hessian
- * (generic.hpp:79-136, eam.cpp:33-233).  `types` uint32 and `frozen` uint8 per atom.
+ * (generic.hpp:79-136, eam.cpp:33-233).  `types` uint32 and `frozen` uint8 per atom.  Passing
+ * types = NULL and frozen = NULL (both) to any call of a potential handle that takes them selects the
+ * copy staged in the handle by flyg_eam_stage_atoms (or by the last call with host arrays).
  * ------------------------------------------------------------------------------------------------- */
+/* No counterpart in the reference, whose SoA inputs are host arrays read on every call: copies the species
+ * and frozen flags of n atoms (host or device memory, `where`) into the potential handle's device memory.
+ * Later calls that pass types = frozen = NULL use this copy; the gradient then gathers it into a list's
+ * cell-order layout once per list build instead of on every call, as it does for host arrays (device
+ * arrays passed to each call may change between calls, so they are gathered every time).  Host inputs
+ * are validated here (FLYG_INVALID_ARG); device inputs by the kernels that read them. */
+int flyg_eam_stage_atoms(flyg_eam eam, const uint32_t* types, const uint8_t* frozen, int64_t n, int where);
 /* EAM::energy (eam.cpp:33-51): sum over all atoms, frozen included. */
 int flyg_eam_energy(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
                     double* energy);

@@ -313,6 +313,21 @@ int flyg_eam_energy(flyg_eam eam, flyg_list nl, const uint32_t* types, const uin
   });
 }
 
+int flyg_eam_stage_atoms(flyg_eam eam, const uint32_t* types, const uint8_t* frozen, int64_t n, int where) {
+  return guard(eam->ctx, [&] {
+    require(n > 0, "no atoms");
+    require(types && frozen, "null argument");
+    if (where == FLYG_DEVICE) {
+      stage_atoms_device(*eam, types, frozen, n);
+    } else {
+      require(where == FLYG_HOST, "`where` must be FLYG_HOST or FLYG_DEVICE");
+      uint32_t const* t;
+      uint8_t const* f;
+      stage_atoms(*eam, types, frozen, n, FLYG_HOST, &t, &f);
+    }
+  });
+}
+
 int flyg_eam_gradient(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
                       double* grad) {
   return guard(eam->ctx, [&] {

@@ -756,6 +756,13 @@ namespace flyg {
 
   void stage_atoms(flyg_eam_s& E, uint32_t const* types, uint8_t const* frozen, long n, int where,
                    uint32_t const** d_types, uint8_t const** d_frozen) {
+    if (!types && !frozen) {   // the handle's staged copy (stage_atoms_device or an earlier host call)
+      require(E.staged_n == n, "no staged species and frozen flags for this many atoms");
+      *d_types = E.types.get();
+      *d_frozen = E.frozen.get();
+      return;
+    }
+    require(types && frozen, "types and frozen must both be given or both be null");
     if (where == FLYG_DEVICE) {
       *d_types = types;
       *d_frozen = frozen;
@@ -795,8 +802,23 @@ namespace flyg {
       E.last_frozen.assign(frozen, frozen + n);
     }
     if (!same_t || !same_f) HIPCHK(hipStreamSynchronize(E.ctx->stream));
+    E.staged_n = n;
     *d_types = E.types.get();
     *d_frozen = E.frozen.get();
+  }
+
+  void stage_atoms_device(flyg_eam_s& E, uint32_t const* types, uint8_t const* frozen, long n) {
+    require(types && frozen, "null argument");
+    E.staged_n = -1;
+    E.last_types.clear();   // contents no longer match any host array
+    E.last_frozen.clear();
+    E.types.reserve(n);
+    E.frozen.reserve(n);
+    HIPCHK(hipMemcpyAsync(E.types.get(), types, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, E.ctx->stream));
+    HIPCHK(hipMemcpyAsync(E.frozen.get(), frozen, n * sizeof(uint8_t), hipMemcpyDeviceToDevice, E.ctx->stream));
+    E.types_stamp = next_types_stamp();
+    E.staged_single = -1;   // not inspected on the host: the kernels treat the species as several
+    E.staged_n = n;
   }
 
 }  // namespace flyg

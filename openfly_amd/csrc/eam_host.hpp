@@ -70,6 +70,7 @@ struct flyg_eam_s {
   std::vector<uint8_t> last_frozen;
   int lanes = 0;                                   // lanes per atom row, 0 = timed choice
   int staged_single = -1;                          // one species in the staged host types, else -1
+  long staged_n = -1;                              // atoms in the staged copy (types/frozen), -1 = none
   std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes
   // Optional kernel timing of gradient calls: one event pair around each call's kernels.
   bool timing = false;
@@ -169,9 +170,12 @@ namespace flyg {
                double* d_hv);
   void eam_table_eval(flyg_eam_s& e, int kind, int a, int b, double const* x, long m, int deriv, double* out);
 
-  // Stage host per-atom inputs into the potential handle (re-uploads only when they changed).
+  // Stage host per-atom inputs into the potential handle (re-uploads only when they changed); device
+  // inputs pass through; types = frozen = null selects the staged copy.
   void stage_atoms(flyg_eam_s& e, uint32_t const* types, uint8_t const* frozen, long n, int where,
                    uint32_t const** d_types, uint8_t const** d_frozen);
+  // Copy device per-atom inputs into the handle's staged buffers.
+  void stage_atoms_device(flyg_eam_s& e, uint32_t const* types, uint8_t const* frozen, long n);
 
   // Deterministic device reductions (vecops.hip).
   void dev_sum(flyg_ctx_s& ctx, double const* x, long n, double* d_out);

@@ -169,6 +169,7 @@ _SIGS = {
     "flyg_lbfgs_minimise": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int64, C.c_int, C.POINTER(C.c_int),
                             C.POINTER(C.c_int)],
     "flyg_lbfgs_set_observer": [_vp, STEP_OBSERVER, _vp],
+    "flyg_eam_stage_atoms": [_vp, _vp, _vp, C.c_int64, C.c_int],
     "flyg_dimer_set_observer": [_vp, STEP_OBSERVER, _vp],
     "flyg_dimer_default_options": [C.POINTER(DimerOptions)],
     "flyg_rotor_default_options": [C.POINTER(RotorOptions)],
@@ -420,13 +421,27 @@ class EAM:
         self.ctx.check(lib().flyg_eam_energy(self.h, nl.h, tp, fp, nl.n, w1, C.byref(E)))
         return E.value
 
+    def stage(self, types, frozen):
+        """flyg_eam_stage_atoms: keep species and frozen flags (host or device arrays) in the handle;
+        later calls given types=None, frozen=None use them."""
+        dev = self.ctx.device
+        n = len(types)
+        tp, w1, k1 = _ptr(types, np.uint32, n, dev)
+        fp, w2, k2 = _ptr(frozen, np.uint8, n, dev)
+        if w1 != w2:
+            raise FlyError("types and frozen must both be host or both device")
+        self.ctx.check(lib().flyg_eam_stage_atoms(self.h, tp, fp, n, w1))
+
     def gradient(self, types, frozen, nl: "List", out=None):
         dev = self.ctx.device
-        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
-        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if out is None:
             out = np.zeros((nl.n, 3))
         op, w3, k3 = _ptr(out, np.float64, 3 * nl.n, dev)
+        if types is None and frozen is None:   # the staged copy
+            self.ctx.check(lib().flyg_eam_gradient(self.h, nl.h, None, None, nl.n, w3, op))
+            return out if w3 == DEVICE else k3.reshape(-1, 3)
+        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if not (w1 == w2 == w3):
             raise FlyError("all arrays must be host or all device")
         self.ctx.check(lib().flyg_eam_gradient(self.h, nl.h, tp, fp, nl.n, w1, op))

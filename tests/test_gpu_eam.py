@@ -381,3 +381,57 @@ def test_slot_layout_follows_types_positions_and_rebuilds(ctx, wen_gpu, wen_orc)
     wen_b = fa.EAM(ctx, H.WEN)
     for pot_, t_ in ((wen_gpu, t1), (wen_b, t2), (wen_gpu, t1), (wen_b, t2)):
         check(pot_.gradient(t_, f, nl), other, t_)
+
+
+def test_staged_atoms_follow_stage_calls_and_rebuilds(ctx, wen_orc):
+    # flyg_eam_stage_atoms keeps species and frozen flags in the handle; calls given NULL for both use
+    # them (the list-layout gather is then cached per build, as for host arrays).  Every result must follow
+    # the last staging: from device tensors, from host arrays, after an update and a rebuild; the staged
+    # form gives the bits of the per-call forms; a size mismatch or nothing staged is refused.
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(43)
+    pos, basis = H.kat_lattice()
+    pos = pos + rng.normal(0, 0.05, pos.shape)
+    n = len(pos)
+    f = np.zeros(n, np.uint8)
+    f[::5] = 1
+    t1 = (rng.random(n) < 0.2).astype(np.uint32)
+    t2 = (rng.random(n) < 0.4).astype(np.uint32)
+    dev = torch.device(f"cuda:{ctx.device}")
+
+    def oracle(p, t):
+        onl = po.List(po.Box(basis), wen_orc.r_cut)
+        onl.rebuild(p, 4)
+        return po.gradient(wen_orc, onl, t, f, 4)
+
+    def check(g, p, t):
+        go = oracle(p, t)
+        assert np.max(np.abs(g - go)) <= 1e-10 * np.max(np.abs(go))
+
+    pot = fa.EAM(ctx, H.WEN)
+    nl = fa.List(ctx, fa.Box(basis), pot.r_cut() + 0.5)
+    nl.rebuild(pos)
+    with pytest.raises(fa.FlyError, match="staged"):
+        pot.gradient(None, None, nl)
+    pot.stage(torch.from_numpy(t1).to(dev), torch.from_numpy(f).to(dev))
+    g1 = pot.gradient(None, None, nl)
+    check(g1, pos, t1)
+    assert np.array_equal(g1, pot.gradient(None, None, nl))          # cached gather, same bits
+    gd = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    pot.gradient(torch.from_numpy(t1).to(dev), torch.from_numpy(f).to(dev), nl, gd)
+    torch.cuda.synchronize()
+    assert np.array_equal(g1, gd.cpu().numpy())                      # per-call device arrays: same bits
+    pot.stage(torch.from_numpy(t2).to(dev), torch.from_numpy(f).to(dev))
+    check(pot.gradient(None, None, nl), pos, t2)
+    d = rng.normal(0, 0.02, pos.shape)
+    nl.update(d)
+    check(pot.gradient(None, None, nl), pos - d, t2)
+    other = pos - d + rng.normal(0, 0.1, pos.shape)
+    nl.rebuild(other)
+    check(pot.gradient(None, None, nl), other, t2)
+    pot.stage(t1, f)                                                 # host arrays
+    check(pot.gradient(None, None, nl), other, t1)
+    assert np.array_equal(pot.gradient(t1, f, nl), pot.gradient(None, None, nl))
+    pot.stage(t1[:-1], f[:-1])
+    with pytest.raises(fa.FlyError, match="staged"):
+        pot.gradient(None, None, nl)
