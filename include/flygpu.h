@@ -138,7 +138,9 @@ int flyg_list_positions(flyg_list nl, double* pos, int where);
  * Later calls that pass types = frozen = NULL use this copy; the gradient then gathers it into a list's
  * cell-order layout once per list build instead of on every call, as it does for host arrays (device
  * arrays passed to each call may change between calls, so they are gathered every time).  Host inputs
- * are validated here (FLYG_INVALID_ARG); device inputs by the kernels that read them. */
+ * are validated here (FLYG_INVALID_ARG); device inputs by the kernels that read them.  A device source
+ * is copied asynchronously on the context's stream: work on other streams must not write it until that
+ * stream has passed the copy. */
 int flyg_eam_stage_atoms(flyg_eam eam, const uint32_t* types, const uint8_t* frozen, int64_t n, int where);
 /* EAM::energy (eam.cpp:33-51): sum over all atoms, frozen included. */
 int flyg_eam_energy(flyg_eam eam, flyg_list nl, const uint32_t* types, const uint8_t* frozen, int64_t n, int where,
