@@ -387,8 +387,8 @@ namespace flyg {
       unsigned* srow = grow + slot * static_cast<long>(K);
       // The up-to-three x-neighbouring cells of a (y, z) line are consecutive cells (x fastest), so their
       // sorted atoms form one contiguous range: 9 ranges instead of 27 cells, in the same order.  Each
-      // step loads BU chunks of BL candidates before testing them in order (more loads in flight, the
-      // same rows).
+      // step loads BU chunks of BL candidates (positions and packed ids) before testing them, then the
+      // slots of all its hits, then writes the hits in order (more loads in flight, the same rows).
       int const x0 = max(c3[0] - 1, 0), x1 = min(c3[0] + 2, g.shape[0]);
       for (int z = max(c3[2] - 1, 0); z < min(c3[2] + 2, g.shape[2]); ++z)
         for (int y = max(c3[1] - 1, 0); y < min(c3[1] + 2, g.shape[1]); ++y) {
@@ -396,27 +396,31 @@ namespace flyg {
           int const b = cstart[x0 * g.prod[0] + line], end = cstart[(x1 - 1) * g.prod[0] + line + 1];
           for (int s0 = b; s0 < end; s0 += BU * BL) {
             bool hit[BU];
+            unsigned en[BU];
 #pragma unroll
             for (int u = 0; u < BU; ++u) {
               int const s = s0 + u * BL + lane;
               hit[u] = false;
+              en[u] = 0;
               if (s < end && s != self) {
                 double4 q = spos[s];
+                en[u] = sid[s];
                 double xn[3] = {q.x, q.y, q.z};
                 hit[u] = dist2(xi, xn) < rc2;
               }
             }
+            int rk[BU];
+#pragma unroll
+            for (int u = 0; u < BU; ++u) rk[u] = hit[u] ? rank_of[en[u] & NB_IDX_MASK] : 0;
 #pragma unroll
             for (int u = 0; u < BU; ++u) {
-              int const s = s0 + u * BL + lane;
               unsigned long long m = __ballot(hit[u]);
               unsigned gm = static_cast<unsigned>((m >> gbase) & ((1ull << BL) - 1ull));
               if (hit[u]) {
                 int k = cnt + __popc(gm & ((1u << lane) - 1u));
                 if (k < K) {
-                  unsigned const en = sid[s];
-                  row[k] = en;
-                  srow[k] = (en & ~NB_IDX_MASK) | static_cast<unsigned>(rank_of[en & NB_IDX_MASK]);
+                  row[k] = en[u];
+                  srow[k] = (en[u] & ~NB_IDX_MASK) | static_cast<unsigned>(rk[u]);
                 }
               }
               cnt += __popc(gm);
