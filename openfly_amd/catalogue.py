@@ -135,6 +135,52 @@ class Catalogue:
         tol = torch.clamp(0.4 * torch.clamp(rmin, max=ref[2]), max=delta_max) * _SQRT2
         return (torch.abs(R0 - ref[0]) <= tol[:, None]).all(dim=1) & (torch.abs(RIJ - ref[1]) <= tol[:, None]).all(dim=1)
 
+    def dump(self, file):
+        """Catalogue::dump (catalogue.hpp:341-348): the options and every known environment (per bucket
+        key, the fingerprints in insertion order) to a numpy .npz archive holding plain arrays only."""
+        arrays = {"options": np.array([self.r_env, self.r_edge, self.delta_max]),
+                  "counts": np.array([len(self._stack), self.size], np.int64)}
+        for b, (key, st) in enumerate(self._stack.items()):
+            arrays[f"k{b}_head"] = np.array([key[0], key[1]], np.int64)
+            arrays[f"k{b}_code"] = np.frombuffer(key[2], np.uint8)
+            arrays[f"k{b}_r0"], arrays[f"k{b}_rij"], arrays[f"k{b}_rmin"] = st
+        np.savez(file, **arrays)
+
+    @classmethod
+    def load(cls, file, r_env=5.2, r_edge=3.0, delta_max=0.1, chunk=2048):
+        """Catalogue(opt, fin) (catalogue.hpp:314-339): a catalogue written by dump.  Its options must be
+        the given ones (FlyError otherwise, as the reference's verify), and a file that is not such an
+        archive raises FlyError("Catalogue is a bad binary: ...")."""
+        import zipfile
+
+        from .flygpu import FlyError
+        c = cls(r_env, r_edge, delta_max, chunk)
+        try:
+            with np.load(file, allow_pickle=False) as z:
+                opt = z["options"]
+                nb, size = (int(v) for v in z["counts"])
+                stacks = []
+                for b in range(nb):
+                    head = z[f"k{b}_head"]
+                    key = (int(head[0]), int(head[1]), z[f"k{b}_code"].astype(np.uint8).tobytes())
+                    r0, rij, rmin = (np.asarray(z[f"k{b}_{x}"], dtype=np.float64) for x in ("r0", "rij", "rmin"))
+                    if r0.ndim != 2 or rij.ndim != 2 or rmin.shape != (len(r0),) or len(rij) != len(r0):
+                        raise ValueError(f"bucket {b} has inconsistent shapes")
+                    stacks.append((key, (r0, rij, rmin)))
+        except (KeyError, ValueError, OSError, EOFError, zipfile.BadZipFile) as e:
+            raise FlyError(f"Catalogue is a bad binary: {e}") from None
+        for name, mine, theirs in (("r_env", c.r_env, opt[0]), ("r_edge", c.r_edge, opt[1]),
+                                   ("delta_max", c.delta_max, opt[2])):
+            if mine != float(theirs):
+                raise FlyError(f"Loading a catalogue with contradicting .{name}")
+        for key, st in stacks:
+            c._stack[key] = st
+            c.buckets[key] = [(st[0][i], st[1][i], float(st[2][i])) for i in range(len(st[2]))]
+        c.size = size
+        if c.size != sum(len(v) for v in c.buckets.values()):
+            raise FlyError("Catalogue is a bad binary: environment count does not match its buckets")
+        return c
+
     def _device_ref(self, ref, device):
         import torch
         key = (id(ref), str(device))

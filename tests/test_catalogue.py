@@ -5,6 +5,7 @@ operation adds none; a vacancy adds exactly its neighbour shells within r_env = 
 a sqrt(3)/2, a, a sqrt(2), a sqrt(11)/2, a sqrt(3)); one frozen atom (another colour) adds its own
 environment and those same five shells around it; displacements well inside delta_max add nothing and a
 0.5 A displacement adds environments.  The host and torch fingerprint paths give the same new atoms.
+A dumped and reloaded catalogue answers as the original.
 """
 import numpy as np
 import pytest
@@ -103,3 +104,34 @@ def test_atom_subset_finds_the_changed_environments(crystal):
     new = c.rebuild(vac, t[1:], f[1:], basis, atoms=near)
     assert set(new) <= set(near.tolist()) and len(new) == 5
     assert c.rebuild(vac, t[1:], f[1:], basis) == []
+
+
+def test_dump_and_load(crystal, tmp_path):
+    # Catalogue::dump / Catalogue(opt, fin) (catalogue.hpp:314-348): a reloaded catalogue knows the same
+    # environments (same size, same new atoms on a later system, none on the systems it has seen) and
+    # refuses contradicting options and files that are not such an archive.
+    from openfly_amd.flygpu import FlyError
+    pos, basis, t, f = crystal
+    c = Catalogue()
+    c.rebuild(pos, t, f, basis)
+    vac = np.delete(pos, 0, axis=0)
+    c.rebuild(vac, t[1:], f[1:], basis)
+    path = tmp_path / "cat.npz"
+    c.dump(path)
+    d = Catalogue.load(path)
+    assert d.size == c.size == 6
+    assert d.rebuild(vac, t[1:], f[1:], basis) == [] and d.rebuild(pos, t, f, basis) == []
+    rng = np.random.default_rng(3)
+    moved = vac + rng.normal(0, 0.3, vac.shape) * (np.arange(len(vac)) < 40)[:, None]
+    new_d = d.rebuild(moved, t[1:], f[1:], basis)
+    new_c = c.rebuild(moved, t[1:], f[1:], basis)
+    assert new_d == new_c and len(new_d) > 0
+    assert d.size == c.size
+    with pytest.raises(FlyError, match="contradicting .r_env"):
+        Catalogue.load(path, r_env=5.0)
+    with pytest.raises(FlyError, match="contradicting .delta_max"):
+        Catalogue.load(path, delta_max=0.2)
+    bad = tmp_path / "bad.npz"
+    bad.write_bytes(b"not an archive")
+    with pytest.raises(FlyError, match="bad binary"):
+        Catalogue.load(bad)
