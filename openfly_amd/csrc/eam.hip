@@ -576,26 +576,47 @@ namespace flyg {
 
   namespace {
     // Density then force pass; pf holds the pass-1 output (positions + F') of every atom.
-    void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L,
-                       double4* pf) {
+    // Lanes per atom row of the two passes, packed as density | force << 8 (either count gives the same
+    // sums, Acc16, so each pass takes its own).
+    constexpr int pack_lanes(int ld, int lf) { return ld | (lf << 8); }
+
+    void density_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, int precision, int L, double4* pf) {
       switch (L) {
         case 4:
           launch_density<4>(E, S, A, false, precision, pf);
-          launch_force<4>(E, S, A, d_grad, precision, pf);
           break;
         case 16:
           launch_density<16>(E, S, A, false, precision, pf);
-          launch_force<16>(E, S, A, d_grad, precision, pf);
           break;
         default:
           launch_density<8>(E, S, A, false, precision, pf);
+      }
+    }
+
+    void force_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int L,
+                    double4 const* pf) {
+      switch (L) {
+        case 4:
+          launch_force<4>(E, S, A, d_grad, precision, pf);
+          break;
+        case 16:
+          launch_force<16>(E, S, A, d_grad, precision, pf);
+          break;
+        default:
           launch_force<8>(E, S, A, d_grad, precision, pf);
       }
     }
 
-    // Lanes per atom row: the row sums do not depend on it (Acc16), so it is picked by timing.
+    void gradient_with(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision, int lanes,
+                       double4* pf) {
+      density_with(E, S, A, precision, lanes & 0xff, pf);
+      force_with(E, S, A, d_grad, precision, lanes >> 8, pf);
+    }
+
+    // Lanes per atom row of each pass: the row sums do not depend on it (Acc16), so each pass's count is
+    // picked by timing that pass alone (the force pass reads the density pass's output of the last run).
     int tuned_lanes(flyg_eam_s& E, Shifts const& S, PairArgs const& A, double* d_grad, int precision) {
-      if (E.lanes > 0) return E.lanes;
+      if (E.lanes > 0) return pack_lanes(E.lanes, E.lanes);
       auto key = std::make_pair(A.n, precision);
       auto it = E.tuned.find(key);
       if (it != E.tuned.end()) return it->second;
@@ -603,25 +624,32 @@ namespace flyg {
       hipEvent_t a, b;
       HIPCHK(hipEventCreate(&a));
       HIPCHK(hipEventCreate(&b));
-      int best = 8;
-      float best_ms = 1e30f;
-      for (int L : {4, 8, 16}) {
-        gradient_with(E, S, A, d_grad, precision, L, E.posf.get());   // warm
-        HIPCHK(hipEventRecord(a, st));
-        for (int r = 0; r < 3; ++r) gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
-        HIPCHK(hipEventRecord(b, st));
-        HIPCHK(hipEventSynchronize(b));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, a, b));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = L;
+      auto best_of = [&](auto&& run) {
+        int best = 8;
+        float best_ms = 1e30f;
+        for (int L : {4, 8, 16}) {
+          run(L);   // warm
+          HIPCHK(hipEventRecord(a, st));
+          for (int r = 0; r < 3; ++r) run(L);
+          HIPCHK(hipEventRecord(b, st));
+          HIPCHK(hipEventSynchronize(b));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, a, b));
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = L;
+          }
         }
-      }
+        return best;
+      };
+      int const ld = best_of([&](int L) { density_with(E, S, A, precision, L, E.posf.get()); });
+      density_with(E, S, A, precision, ld, E.posf.get());
+      int const lf = best_of([&](int L) { force_with(E, S, A, d_grad, precision, L, E.posf.get()); });
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
-      E.tuned[key] = best;
-      return best;
+      int const lanes = pack_lanes(ld, lf);
+      E.tuned[key] = lanes;
+      return lanes;
     }
   }  // namespace
 
@@ -641,8 +669,8 @@ namespace flyg {
       HIPCHK(hipEventRecord(E.events[E.events_used].first, E.ctx->stream));
     }
     PairArgs A = slot_args(E, nl, types, frozen);
-    int L = tuned_lanes(E, S, A, d_grad, precision);
-    gradient_with(E, S, A, d_grad, precision, L, E.posf.get());
+    int const lanes = tuned_lanes(E, S, A, d_grad, precision);
+    gradient_with(E, S, A, d_grad, precision, lanes, E.posf.get());
     if (E.timing) HIPCHK(hipEventRecord(E.events[E.events_used++].second, E.ctx->stream));
     HIPCHK(hipGetLastError());
   }
@@ -688,10 +716,10 @@ namespace flyg {
     Shifts S = shifts_of(nl.box);
     PairArgs A = pair_args(E, nl, d_types, d_frozen);
     // The lanes per atom row change no result bit: a fixed choice, or an earlier tuning, without timing.
-    int L = E.lanes > 0 ? E.lanes : 8;
+    int lanes = E.lanes > 0 ? pack_lanes(E.lanes, E.lanes) : pack_lanes(8, 8);
     auto it = E.tuned.find(std::make_pair(A.n, precision));
-    if (E.lanes <= 0 && it != E.tuned.end()) L = it->second;
-    gradient_with(E, S, A, d_grad, precision, L, pf);
+    if (E.lanes <= 0 && it != E.tuned.end()) lanes = it->second;
+    gradient_with(E, S, A, d_grad, precision, lanes, pf);
     HIPCHK(hipGetLastError());
   }
 

@@ -71,7 +71,7 @@ struct flyg_eam_s {
   int lanes = 0;                                   // lanes per atom row, 0 = timed choice
   int staged_single = -1;                          // one species in the staged host types, else -1
   long staged_n = -1;                              // atoms in the staged copy (types/frozen), -1 = none
-  std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes
+  std::map<std::pair<long, int>, int> tuned;       // (atoms, precision) -> lanes: density | force << 8
   // Optional kernel timing of gradient calls: one event pair around each call's kernels.
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
