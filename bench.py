@@ -43,7 +43,7 @@ MECH_F2NORM = 1e-4   # |g| tolerance of the mixed form's mechanism relaxations (
 # (profiles/eam_gradient_fe30_x_alloy.md): FETCH_SIZE and WRITE_SIZE per dispatch in KiB, each scaled by the
 # reading for a copy kernel of known size (python bench.py --copy-calibration: a 64 MiB copy reads
 # FETCH_SIZE 32780.9 KiB and WRITE_SIZE 65536 KiB).
-GRADIENT_TRAFFIC = {"fetch_kib": 12212.4 + 12747.3, "write_kib": 1704.0 + 1481.0,
+GRADIENT_TRAFFIC = {"fetch_kib": 12217.7 + 12747.6, "write_kib": 1704.0 + 1490.2,
                     "fetch_scale": 65536.0 / 32780.9, "write_scale": 1.0}
 
 
