@@ -94,6 +94,33 @@ int main(int argc, char** argv) {
     double g2 = 0;
     for (std::size_t k = 0; k < relaxed[g_].size(); ++k) g2 += relaxed[g_][k] * relaxed[g_][k];
     REQUIRE(std::sqrt(g2) < 1e-6);
+    // LBFGS::Options::fout (lbfgs.cpp:75-77): the same relaxation writing one frame per iteration, the
+    // first the input, the last the relaxed positions.
+    {
+      std::string const traj = "/tmp/facade_lbfgs_" + std::to_string(static_cast<long>(::getpid())) + ".gsd";
+      std::uint64_t frames = 0;
+      {
+        io::BinaryFile file(traj, io::create);
+        minimise::LBFGS::Options lo;
+        lo.fout = &file;
+        minimise::LBFGS traced(lo, cell.box);
+        system::SoA<Position, PotentialGradient> again(cell.atoms.size());
+        REQUIRE(!traced.minimise(again, cell.atoms, pot, 1));
+        frames = file.n_frames();
+        REQUIRE(frames >= 2);
+        system::SoA<Position> first(cell.atoms.size()), last(cell.atoms.size());
+        file.read_to(0, r_, first);
+        file.read_to(frames - 1, r_, last);
+        bool same = true;
+        for (std::size_t k = 0; k < first[r_].size(); ++k) {
+          same = same && first[r_][k] == cell.atoms[r_][k] && last[r_][k] == again[r_][k] &&
+                 again[r_][k] == relaxed[r_][k];
+        }
+        REQUIRE(same);
+      }
+      std::remove(traj.c_str());
+      std::printf("minimiser trajectory (Options::fout): %llu frames\n", static_cast<unsigned long long>(frames));
+    }
 
     // Start: nudge the atom next to the vacancy towards it, axis along the hop.
     system::SoA<Position, Axis, TypeID, Frozen> in(cell.atoms.size());
