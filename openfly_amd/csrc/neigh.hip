@@ -200,17 +200,61 @@ namespace flyg {
       return wbase + (incl - g);
     }
 
+    // The reference's chain of gen_image calls (any box): images of images, axis by axis.
+    __device__ inline void ghosts_chain(BoxDesc const& b, double rc, double4 const& p, bool inr, long i, long n,
+                                        double4* __restrict__ ent, unsigned* __restrict__ eid, unsigned* __restrict__ st,
+                                        long gcap) {
+      double cx[27][3];
+      int cs[27];
+      int m = 1;
+      cx[0][0] = p.x;
+      cx[0][1] = p.y;
+      cx[0][2] = p.z;
+      cs[0] = 13;
+      int const pow3[3] = {1, 3, 9};
+      for (int ax = 0; ax < 3 && inr; ++ax) {
+        if (!b.per[ax]) continue;
+        int const end = m;
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+          for (int c = 0; c < end; ++c) {
+            double im[3] = {cx[c][0], cx[c][1], cx[c][2]};
+            if (gen_image(b, rc, sgn, im, ax)) {
+              cx[m][0] = im[0];
+              cx[m][1] = im[1];
+              cx[m][2] = im[2];
+              cs[m] = cs[c] + sgn * pow3[ax];
+              ++m;
+            }
+          }
+        }
+      }
+      unsigned const g = static_cast<unsigned>(m - 1);
+      unsigned const base = ghost_slots(g, st);
+      if (g == 0) return;
+      for (int k = 1; k < m; ++k) {
+        long slot = static_cast<long>(base) + (k - 1);
+        if (slot >= gcap) {
+          atomicOr(&st[3], static_cast<unsigned>(OV_GHOST));
+          return;
+        }
+        ent[n + slot] = make_double4(cx[k][0], cx[k][1], cx[k][2], 0.0);
+        eid[n + slot] = (static_cast<unsigned>(cs[k]) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+      }
+    }
+
     // make_ghosts with a fixed capacity; the counter keeps counting past it so the host can resize.
     // Launched with whole wavefronts.  Orthorhombic boxes: whether an axis contributes a +1 / -1 image
     // depends only on the atom's own coordinate on that axis, so the images are the product of the
     // per-axis choices (the same set, and the same coordinates, as the reference's chain) and live in
-    // registers; other boxes follow the chain of gen_image calls.
+    // registers; other boxes follow the chain of gen_image calls.  One instantiation per box kind: the
+    // chain's per-lane arrays live in scratch memory, which the orthorhombic form does not need.
+    template <bool ORTHO>
     __global__ void k_ghosts_cap(BoxDesc b, double rc, double4 const* __restrict__ pos, long n, double4* __restrict__ ent,
                                  unsigned* __restrict__ eid, unsigned* __restrict__ st, long gcap) {
       long i = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x;
       bool const inr = i < n;
       double4 p = inr ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-      if (b.ortho) {
+      if constexpr (ORTHO) {
         double const x[3] = {p.x, p.y, p.z};
         bool up[3], dn[3];
         unsigned g = 1;
@@ -253,43 +297,8 @@ namespace flyg {
               ent[n + slot] = make_double4(im[0], im[1], im[2], 0.0);
               eid[n + slot] = (static_cast<unsigned>(shift_code(s0, s1, s2)) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
             }
-        return;
-      }
-      double cx[27][3];
-      int cs[27];
-      int m = 1;
-      cx[0][0] = p.x;
-      cx[0][1] = p.y;
-      cx[0][2] = p.z;
-      cs[0] = 13;
-      int const pow3[3] = {1, 3, 9};
-      for (int ax = 0; ax < 3 && inr; ++ax) {
-        if (!b.per[ax]) continue;
-        int const end = m;
-        for (int sgn = 1; sgn >= -1; sgn -= 2) {
-          for (int c = 0; c < end; ++c) {
-            double im[3] = {cx[c][0], cx[c][1], cx[c][2]};
-            if (gen_image(b, rc, sgn, im, ax)) {
-              cx[m][0] = im[0];
-              cx[m][1] = im[1];
-              cx[m][2] = im[2];
-              cs[m] = cs[c] + sgn * pow3[ax];
-              ++m;
-            }
-          }
-        }
-      }
-      unsigned const g = static_cast<unsigned>(m - 1);
-      unsigned const base = ghost_slots(g, st);
-      if (g == 0) return;
-      for (int k = 1; k < m; ++k) {
-        long slot = static_cast<long>(base) + (k - 1);
-        if (slot >= gcap) {
-          atomicOr(&st[3], static_cast<unsigned>(OV_GHOST));
-          return;
-        }
-        ent[n + slot] = make_double4(cx[k][0], cx[k][1], cx[k][2], 0.0);
-        eid[n + slot] = (static_cast<unsigned>(cs[k]) << NB_SHIFT_BITS) | static_cast<unsigned>(i);
+      } else {
+        ghosts_chain(b, rc, p, inr, i, n, ent, eid, st, gcap);
       }
     }
 
@@ -607,7 +616,13 @@ namespace flyg {
       long const nz = std::max<long>(std::max<long>(n, 2 * nc1), 4);
       k_canon_reals<<<nblk(nz, TPB), TPB, 0, st>>>(nl.box, d_xyz, n, nl.pos.get(), nl.ent.get(), nl.eid.get(),
                                                    nl.cstart.get(), 2 * nc1, nl.ccursor.get(), nc, dst);
-      k_ghosts_cap<<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), dst, nl.gcap);
+      if (nl.box.ortho) {
+        k_ghosts_cap<true><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), dst,
+                                                         nl.gcap);
+      } else {
+        k_ghosts_cap<false><<<nblk(n, TPB), TPB, 0, st>>>(nl.box, nl.rc, nl.pos.get(), n, nl.ent.get(), nl.eid.get(), dst,
+                                                          nl.gcap);
+      }
       k_cell_count_cap<<<nblk(cap, TPB), TPB, 0, st>>>(nl.grid, nl.ent.get(), n, nl.gcap, dst, nl.ecell.get(),
                                                        nl.cstart.get(), nc1);
       exclusive_scan(ctx, nl.cstart.get(), static_cast<int>(2 * nc1), nl.scan_tmp);
