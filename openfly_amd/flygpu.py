@@ -412,9 +412,14 @@ class EAM:
         return out
 
     def energy(self, types, frozen, nl: "List"):
+        """types = frozen = None: the copy kept by stage()."""
         dev = self.ctx.device
-        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
-        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
+        if types is None and frozen is None:
+            tp = fp = None
+            w1 = w2 = HOST
+        else:
+            tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+            fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if w1 != w2:
             raise FlyError("types and frozen must both be host or both be device arrays")
         E = C.c_double()
@@ -448,13 +453,18 @@ class EAM:
         return out if w3 == DEVICE else k3.reshape(-1, 3)
 
     def hvp(self, types, frozen, nl: "List", v, out=None):
+        """types = frozen = None: the copy kept by stage()."""
         dev = self.ctx.device
-        tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
-        fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         vp, w3, k3 = _ptr(v, np.float64, 3 * nl.n, dev)
         if out is None:
             out = np.zeros((nl.n, 3))
         op, w4, k4 = _ptr(out, np.float64, 3 * nl.n, dev)
+        if types is None and frozen is None:
+            tp = fp = None
+            w1 = w2 = w3
+        else:
+            tp, w1, k1 = _ptr(types, np.uint32, nl.n, dev)
+            fp, w2, k2 = _ptr(frozen, np.uint8, nl.n, dev)
         if not (w1 == w2 == w3 == w4):
             raise FlyError("all arrays must be host or all device")
         self.ctx.check(lib().flyg_eam_hvp(self.h, nl.h, tp, fp, nl.n, vp, op, w1))
@@ -629,16 +639,19 @@ class LBFGS:
             self.h = None
 
     def minimise(self, positions, types, frozen, pot: EAM):
-        """Returns (positions, gradient, failed, iterations); failed mirrors the reference's bool."""
+        """Returns (positions, gradient, failed, iterations); failed mirrors the reference's bool.
+        types = frozen = None: the copy kept by pot.stage()."""
         pos = np.array(positions, dtype=np.float64, copy=True).reshape(-1, 3)
         n = len(pos)
         grad = np.zeros_like(pos)
-        t = np.ascontiguousarray(types, dtype=np.uint32)
-        f = np.ascontiguousarray(frozen, dtype=np.uint8)
+        staged = types is None and frozen is None
+        t = None if staged else np.ascontiguousarray(types, dtype=np.uint32)
+        f = None if staged else np.ascontiguousarray(frozen, dtype=np.uint8)
         failed = C.c_int()
         it = C.c_int()
         rc = lib().flyg_lbfgs_minimise(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(grad.ctypes.data),
-                                       C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n, HOST,
+                                       None if staged else C.c_void_p(t.ctypes.data),
+                                       None if staged else C.c_void_p(f.ctypes.data), n, HOST,
                                        C.byref(failed), C.byref(it))
         if self._obs is not None:
             self._obs.reraise()
@@ -713,15 +726,19 @@ class Dimer:
         pos = np.array(positions, dtype=np.float64, copy=True).reshape(-1, 3)
         ax = np.array(axis, dtype=np.float64, copy=True).reshape(-1, 3)
         n = len(pos)
-        t = np.ascontiguousarray(types, dtype=np.uint32)
-        f = np.ascontiguousarray(frozen, dtype=np.uint8)
+        staged = types is None and frozen is None   # the copy kept by pot.stage()
+        t = None if staged else np.ascontiguousarray(types, dtype=np.uint32)
+        f = None if staged else np.ascontiguousarray(frozen, dtype=np.uint8)
         mn = np.ascontiguousarray(in_min, dtype=np.float64)
         hs = np.ascontiguousarray(np.stack([np.ravel(h) for h in hist_sp]) if len(hist_sp) else np.zeros(1))
         if count_frozen is None:
+            if staged:
+                raise FlyError("count_frozen must be given with staged frozen flags")
             count_frozen = int(f.sum())
         st = SearchStats()
         rc = lib().flyg_dimer_find_sp(self.h, pot.h, C.c_void_p(pos.ctypes.data), C.c_void_p(ax.ctypes.data),
-                                      C.c_void_p(t.ctypes.data), C.c_void_p(f.ctypes.data), n,
+                                      None if staged else C.c_void_p(t.ctypes.data),
+                                      None if staged else C.c_void_p(f.ctypes.data), n,
                                       C.c_void_p(mn.ctypes.data), C.c_void_p(hs.ctypes.data), len(hist_sp),
                                       float(theta_tol), int(count_frozen), HOST, C.byref(st))
         if self._obs is not None:

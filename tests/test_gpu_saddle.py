@@ -150,3 +150,35 @@ def test_rotor_eff_gradient_matches_oracle(ctx, wen, frozen_frac):
     assert np.max(np.abs(ax - axo)) <= 1e-9
     assert np.max(np.abs(eff - effo)) <= 1e-9 * np.max(np.abs(effo))
     assert np.max(np.abs(g_after - g_before)) <= 1e-10 * np.max(np.abs(g_before))
+
+
+def test_staged_atoms_in_every_call(ctx, wen):
+    # types = frozen = NULL selects the copy kept in the potential handle (flyg_eam_stage_atoms) in every
+    # call that takes them: energy, HVP, LBFGS::minimise and Dimer::find_sp give the bits of the same calls
+    # with the arrays passed.
+    pot, data = wen
+    pos, basis = small_vacancy_system(cells=5, seed=4, a=2.855324)
+    n = len(pos)
+    rng = np.random.default_rng(9)
+    t = np.zeros(n, np.uint32)
+    f = (rng.random(n) < 0.1).astype(np.uint8)
+    pot.stage(t, f)
+    nl = fa.List(ctx, fa.Box(basis), pot.r_cut())
+    nl.rebuild(pos)
+    assert pot.energy(None, None, nl) == pot.energy(t, f, nl)
+    v = rng.normal(size=(n, 3))
+    assert np.array_equal(pot.hvp(None, None, nl, v), pot.hvp(t, f, nl, v))
+    m = fa.LBFGS(ctx, fa.Box(basis))
+    a = m.minimise(pos, t, f, pot)
+    b = m.minimise(pos, None, None, pot)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2:] == b[2:]
+    mn = a[0]
+    nl_p = po.List(po.Box(basis), 6.0)
+    nl_p.rebuild(mn, 2)
+    start, axis, _ = po.perturb([5, 3, 3, 4], 0.6, 4.0, nl_p, 1, mn, f)
+    dim = fa.Dimer(ctx, fa.Box(basis))
+    r1 = dim.find_sp(start, axis, t, f, mn, pot)
+    r2 = dim.find_sp(start, axis, None, None, mn, pot, count_frozen=int(f.sum()))
+    assert (r1["exit"], r1["steps"]) == (r2["exit"], r2["steps"]) and np.array_equal(r1["pos"], r2["pos"])
+    with pytest.raises(fa.FlyError, match="count_frozen"):
+        dim.find_sp(start, axis, None, None, mn, pot)
