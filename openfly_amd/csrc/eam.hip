@@ -201,13 +201,20 @@ namespace flyg {
       int lane = static_cast<int>(gid % L);
       bool inr = i < A.n;
       long const ia = inr && A.order ? A.order[i] : i;   // the atom, for its frozen flag and gradient
+      unsigned e = 0;
+      // The row's own data is loaded beside the frozen flag (not after it): the flag waits on `order`.
+      double4 pi = make_double4(0.0, 0.0, 0.0, 0.0);
+      int ti = MULTI ? 0 : A.t0;
+      int c = 0;
+      if (inr) {
+        pi = posf[i];
+        c = min(A.cnt[i], A.K);
+        // Mixed form: the species the density pass packed beside F' (checked there).
+        if (MULTI) ti = MIX ? packed_type(pi.w) : checked_type(A.types[i], T.ntypes, e);
+      }
       bool act = inr && !A.frozen[ia];
       Acc16<L> gx, gy, gz;
-      unsigned e = 0;
       if (act) {
-        double4 pi = posf[i];
-        int ti = MULTI ? checked_type(A.types[i], T.ntypes, e) : A.t0;
-        int const c = min(A.cnt[i], A.K);
         unsigned const* row = A.nbr + i * static_cast<long>(A.K);
         walk_row<L>(row, c, lane, S, posf, pi, [&](int q, PairGeo const& p) {
           if (p.r2 < T.rc2) {
