@@ -5,7 +5,8 @@
 // clear(), per-species values, a triclinic box, and the error paths (missing chunk, wrong rows, columns
 // or type, writing a read-only file, a file that is not GSD, a frame never committed).
 // Usage: gsd_check <directory>; leaves <directory>/layout.gsd for a reader written from the published
-// format to check.  Prints one line per case; exit status 0 on success.
+// format to check.  Prints one line per case; exit status 0 on success.  gsd_check --read <file> prints
+// what it reads from a file another writer made.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -292,9 +293,31 @@ static void layout_file(std::string const& path) {
   w.commit([&] { w.write("log/energy", -123.456); });
 }
 
+// `gsd_check --read <file>`: prints what io::BinaryFile reads from a file another writer made: the frame
+// count, then per frame particles/N, the positions (frame 0's when a frame has none) and log/value.
+static int read_foreign(char const* path) {
+  try {
+    io::BinaryFile f(path);
+    std::printf("frames %llu\n", static_cast<unsigned long long>(f.n_frames()));
+    for (std::uint64_t i = 0; i < f.n_frames(); ++i) {
+      auto const n = f.read<std::uint32_t>(i, "particles/N");
+      system::SoA<Position> s(n);
+      f.read_to(i, r_, s);
+      std::printf("frame %llu N %u value %d pos", static_cast<unsigned long long>(i), n, f.read<std::int32_t>(i, "log/value"));
+      for (std::size_t k = 0; k < s[r_].size(); ++k) std::printf(" %.17g", s[r_][k]);
+      std::printf("\n");
+    }
+  } catch (std::exception const& e) {
+    std::printf("error %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 3 && std::string(argv[1]) == "--read") return read_foreign(argv[2]);
   if (argc < 2) {
-    std::fprintf(stderr, "usage: gsd_check <directory>\n");
+    std::fprintf(stderr, "usage: gsd_check <directory> | gsd_check --read <file>\n");
     return 2;
   }
   std::string const dir = argv[1];

@@ -4,8 +4,9 @@ tests/cpp/gsd_check.cpp runs the reference's own round trip (tests/IO/gsd.cpp:32
 open (header offsets, index and name-list growth, reopen and append, clear, per-species values, a triclinic
 box, error paths).  Here a second reader, written with `struct` from the published GSD 2.0 layout
 (256-byte header, 32-byte index entries ordered by frame then name id, NUL-separated name list), reads the
-file that program leaves and checks every chunk's bytes, so a layout mistake made the same way in the
-writer and the reader of the facade still fails.  Against the gsd library itself (absent here, an empty
+file that program leaves and checks every chunk's bytes, and a second writer built the same way makes a
+file the facade must read, so a layout mistake made the same way in the writer and the reader of the
+facade still fails.  Against the gsd library itself (absent here, an empty
 submodule in the reference) the format is **parity unpinned**."""
 import os
 import shutil
@@ -43,7 +44,37 @@ def read_gsd(path):
                 names=names, frames=frames)
 
 
-def test_gsd_program(tmp_path):
+def write_gsd(path, frames, index_entries=8):
+    """A GSD 2.0 file written with `struct` from the published layout: frames is a list of
+    {name: (type code, 2-D array)}; inside each frame the index entries go in descending name id (the
+    lookup must not depend on their order); the index holds `index_entries` slots."""
+    names = []
+    for fr in frames:
+        for k in fr:
+            if k not in names:
+                names.append(k)
+    nl = b"".join(n.encode() + b"\0" for n in names)
+    nl_bytes = max(64, len(nl) + 1)
+    idx_loc = 256
+    nl_loc = idx_loc + 32 * index_entries
+    data_loc = nl_loc + nl_bytes
+    blobs, entries, at = [], [], data_loc
+    for f, fr in enumerate(frames):
+        for name in sorted(fr, key=names.index, reverse=True):
+            typ, a = fr[name]
+            a = np.ascontiguousarray(a, dtype=DTYPES[typ])
+            entries.append(struct.pack("<QQqIHBB", f, a.shape[0], at, a.shape[1], names.index(name), typ, 0))
+            blobs.append(a.tobytes())
+            at += len(blobs[-1])
+    assert len(entries) <= index_entries
+    head = struct.pack("<QQQQQII64s64s80s", 0x65DF65DF65DF65DF, idx_loc, index_entries, nl_loc, nl_bytes,
+                       (1 << 16) | 4, 2 << 16, b"pytest", b"hoomd", b"")
+    idx = b"".join(entries).ljust(32 * index_entries, b"\0")
+    with open(path, "wb") as fh:
+        fh.write(head + idx + nl.ljust(nl_bytes, b"\0") + b"".join(blobs))
+
+
+def build_checker(tmp_path):
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("no g++")
@@ -51,6 +82,39 @@ def test_gsd_program(tmp_path):
     src = os.path.join(ROOT, "tests", "cpp", "gsd_check.cpp")
     subprocess.run([cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
                     "-o", exe, src], check=True)
+    return exe
+
+
+def test_reads_a_file_written_from_the_published_layout(tmp_path):
+    # The converse check: a file written here with struct (small index, entries of a frame in descending
+    # name id, a frame without positions) read through io::BinaryFile.
+    exe = build_checker(tmp_path)
+    pos = np.array([[0.5, -1.25, 3.0], [1e-3, 2.0, -7.5], [4.25, 0.0, 1.0]])
+    path = os.path.join(tmp_path, "foreign.gsd")
+    write_gsd(path, [{"particles/N": (3, [[3]]), "particles/position": (10, pos), "log/value": (7, [[7]])},
+                     {"particles/N": (3, [[3]]), "log/value": (7, [[9]])}])
+    r = subprocess.run([exe, "--read", path], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[0] == "frames 2"
+    for f, value in ((0, 7), (1, 9)):
+        head = lines[1 + f].split(" pos ")
+        assert head[0] == f"frame {f} N 3 value {value}"
+        assert np.array_equal(np.array([float(x) for x in head[1].split()]).reshape(3, 3), pos)
+    # A file that is not GSD, and one whose index points past its end, are refused.
+    bad = os.path.join(tmp_path, "bad.gsd")
+    with open(bad, "wb") as fh:
+        fh.write(b"\0" * 300)
+    assert "Not a GSD file" in subprocess.run([exe, "--read", bad], capture_output=True, text=True).stdout
+    with open(path, "r+b") as fh:
+        fh.seek(8)
+        fh.write(struct.pack("<Q", 1 << 40))
+    assert "File corrupt" in subprocess.run([exe, "--read", path], capture_output=True, text=True).stdout
+
+
+def test_gsd_program(tmp_path):
+    exe = build_checker(tmp_path)
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stderr
